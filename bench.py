@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit residual-and-update loop on one MI355X.
+
+Workload (default): FP64 Navier-Stokes flow past a cylinder, O-grid of
+2048 azimuthal x 1024 radial cells (geometric stretching 1.006, R=0.5 to 30),
+each quad split into 2 triangles along a seeded diagonal -> 4,194,304 cells;
+M=0.2, Re=100, Rusanov flux, scalar limiter, TVD-RK2 (2 stages per step),
+CFL 0.5.  A step is one full pass of the reference's time loop
+(app/solver.d:119-250): both stages plus RMS residuals, forces, dt/CFL update.
+
+Metric: Mcell-RK-stage updates/s = interior cells * stages * steps / wall time.
+Printed as one JSON line; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes (FP64, triangles): per interior cell-stage for the whole
+# residual+update, and per launch unit for the face kernel (dominant)
+STAGE_BYTES_PER_CELL = 268.0
+FACE_BYTES_PER_FACE = 88.0      # owner/other ids 8 + normal 16 + midpoint 16 + length 8 + len*flux,len*sMax 40
+FACE_BYTES_PER_CELL = 144.0     # state 32 + gradient 64 + limiter 32 + centroid 16, each cell read once
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cylinder_config(M=0.2):
+    q = [M, 0.0, 1.0, 1.0]
+    return {
+        "mesh": "cylinder_ogrid.msh", "dt": 1.0e-6, "tEnd": 1.0e9, "flux": "rusanovFlux", "plotIter": -1,
+        "saveIter": -1, "CFL": 0.5, "integrator": "RK2_TVD", "forceBoundary": "Wall", "limited": True,
+        "order": 2, "initialConditions": q, "viscosity": True, "cflAdjust": True,
+        "physicalConfig": {"Pr": 0.7, "R": 8.3145, "gamma": 1.4, "Re": 100.0, "L": 1.0},
+        "boudaryConditions": [{"tag": "Wall", "type": "viscidWall", "q": q},
+                              {"tag": "Farfield", "type": "fullState", "q": q}],
+    }
+
+
+def cpu_baseline(raw, cfg, steps, n_cells):
+    import oracle
+    oracle.build()
+    m = oracle.Mesh(raw)
+    c = dict(cfg)
+    m.setup(c)
+    t0 = time.perf_counter()
+    m.run(steps)
+    el = time.perf_counter() - t0
+    return n_cells * 2 * steps / el / 1e6, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n-theta", type=int, default=2048)
+    ap.add_argument("--n-r", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--time-steps", type=int, default=20, help="steps timed per kernel with HIP events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the solver keeps the whole mesh on a single MI355X "
+              "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from ebbcfd_amd import Solver, build_mesh, meshgen
+
+    log("generating O-grid %d x %d" % (args.n_theta, args.n_r))
+    raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1)
+    cfg = cylinder_config()
+    mesh = build_mesh(raw)
+    N = mesh.n_interior
+    log("mesh built: %d cells, %d edges" % (N, mesh.n_edges))
+    s = Solver(mesh, cfg)
+    log("solver on device (%.2f GB)" % (s.device_bytes / 1e9))
+    stages = 2
+    s.run(args.warmup)
+    log("warmup done")
+    # timed region: device-resident time loop, K steps, synchronised on both sides
+    t0 = time.perf_counter()
+    hist, t_end, dt_end = s.run(args.steps)
+    el = time.perf_counter() - t0
+    assert len(hist) == args.steps, "solver stopped early"
+    assert np.all(np.isfinite(hist[:, 2:6])), "non-finite residuals"
+    ms_per_step = el / args.steps * 1e3
+    value = N * stages * args.steps / el / 1e6
+    log("timed %d steps: %.3f ms/step, %.1f Mcell-stage/s" % (args.steps, ms_per_step, value))
+
+    # per-kernel device time with HIP events on the solver stream
+    tk = s.time_steps(args.time_steps)
+    face_ms = tk["faces_ms"]
+    F = mesh.n_edges
+    face_bytes = F * FACE_BYTES_PER_FACE + N * FACE_BYTES_PER_CELL
+    achieved = face_bytes / (face_ms * 1e-3) / 1e9
+    stage_ms = tk["step_ms"] / stages
+    stage_gbs = N * STAGE_BYTES_PER_CELL / (stage_ms * 1e-3) / 1e9
+    log("kernels: gradient %.3f ms, faces %.3f ms, update %.3f ms, step %.3f ms"
+        % (tk["gradient_ms"], face_ms, tk["update_ms"], tk["step_ms"]))
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log("CPU baseline: oracle, %d steps on the same mesh, 1 core" % args.cpu_steps)
+        v, el_cpu = cpu_baseline(raw, cfg, args.cpu_steps, N)
+        cpu = {"value": round(v, 4), "unit": "Mcell-stage/s", "cores": 1, "kind": "port",
+               "sample": "oracle (C restatement, gcc -O2 -ffp-contract=off, 1 thread): %d TVD-RK2 steps of the same "
+                         "4.2M-cell cylinder case, %.1f s" % (args.cpu_steps, el_cpu)}
+        log("CPU baseline %.3f Mcell-stage/s (%.1f s)" % (v, el_cpu))
+
+    out = {
+        "metric": "Mcell-RK-stage updates/s @4M cells (%HBM roof)",
+        "value": round(value, 3),
+        "unit": "Mcell-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r), "cells": N,
+                   "edges": F, "flux": "rusanovFlux", "limiter": "scalar (solve.d:360-403)",
+                   "integrator": "RK2_TVD", "stages_per_step": stages, "M": 0.2, "Re": 100, "CFL": 0.5},
+        "roofline": {"bound": "hbm", "kernel": "k_faces", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(face_ms, 4), "algorithmic_bytes_per_launch": int(face_bytes)},
+        "stage_roofline": {"bytes_per_cell_stage": STAGE_BYTES_PER_CELL, "stage_ms": round(stage_ms, 4),
+                           "achieved": round(stage_gbs, 1), "frac": round(stage_gbs / HBM_PEAK_GBS, 4),
+                           "unit": "GB/s"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+// Device data layout shared by the kernels (kernels.hip) and the host driver (solver.cpp).
+//
+// Cells: interior cells 0..N-1 in reference order, ghosts N..N+G-1 in reference
+// boundary-edge order (source/mesh.d:329-381).  Faces: interior faces 0..Fi-1 in
+// reference interior-edge order, boundary faces Fi..Fi+Fb-1 in boundary-edge
+// order (boundary face b owns ghost N+b), then one all-zero padding face used by
+// cells with fewer than S edges.  Every per-cell array is fixed stride S (3 for
+// all-triangle meshes, 4 with quads, 6 otherwise); a short cell repeats itself as
+// neighbour and points at the padding face, which leaves every sum unchanged.
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace ebb {
+
+struct BFace {
+    int32_t type;          // BoundaryType
+    int32_t vertical;      // InviscidWall/Symmetry: edge is vertical (reflection = x-flip)
+    double R[4];           // reflection matrix about the edge line (row-major)
+    double far[4];         // FullState far state / Dirichlet edge state (edge.q[1])
+    double gq[4];          // Dirichlet ghost state at the ghost centroid
+    double gg[8];          // Dirichlet ghost gradient (var, dir)
+    double nrm[2];         // edge unit normal (owner outward)
+    double data[4];        // ConstP: pb; TPInflow: Tt, Pt, cos(aoa), sin(aoa)
+    double Rgas;           // gas constant (TPInflow)
+    double pad;
+};
+
+// Device-resident scalars of the time loop (one per solver).
+struct DevScalars {
+    unsigned long long newDtBits;   // min over cells of CFL*d/sAve, as ordered bits
+    unsigned long long rmaxBits;    // max |R| before scaling, as ordered bits
+    unsigned long long errEdge;     // (stage << 32) | edge key, min wins; ~0 = none
+    unsigned long long errCell;     // (step << 32) | reference cell, min wins; ~0 = none
+    double dt;                      // dt used by the stages of the current step
+    double t;                       // simulation time
+    double CFL;                     // current CFL (cflAdjust lowers it)
+    double residLast;               // residMax of the previous step (cflAdjust)
+    int64_t incIters;               // cflAdjust counter
+    int64_t steps;                  // completed steps
+    int32_t active;                 // 0 once t >= tEnd or an error was recorded
+    int32_t pad;
+    double rms[4];
+    double force[2];
+};
+
+struct DevMesh {
+    uint32_t N, G, Fi, Fb;
+    const uint32_t* __restrict__ nbr;    // N*S neighbour cells (ghosts >= N)
+    const uint32_t* __restrict__ cface;  // N*S face ids, bit 31 set where the cell is not the owner
+    const double* __restrict__ gm;       // N*2*S least-squares weights (dir, nbr)
+    const double2* __restrict__ cen;     // N+G centroids
+    const double* __restrict__ nia;      // N: -(1.0/area)
+    const double* __restrict__ cd;       // N: d = 2A/perim
+    const double* __restrict__ cper;     // N: perim
+    const uint2* __restrict__ fcell;     // Fi+Fb: (owner, other)
+    const double2* __restrict__ fn;      // Fi+Fb: unit normal (owner outward)
+    const double2* __restrict__ fmid;    // Fi+Fb: midpoint
+    const double* __restrict__ flen;     // Fi+Fb: length
+    const BFace* __restrict__ bface;     // Fb
+    const double4* __restrict__ src;     // N MMS source (null if none)
+    const uint32_t* __restrict__ refId;  // N: reference index of each device cell
+};
+
+struct PhysParams {
+    int flux;            // FluxKind
+    int viscous;
+    int order;
+    int limited;
+    double Pr, mu, Rgas, lpThresh;
+    int localTimestep;
+    int mixed;           // 1: fp32 flux/reconstruction/limiter
+};
+
+// One stage of an integrator.  `kind` selects the update applied to the residual:
+enum StageKind {
+    SK_RESIDUAL = 0,   // write R only (ufvmSolver)
+    SK_EULER,          // qout = qn + h*R
+    SK_RK2_1,          // k1 = R; T = qn + (dt*R)
+    SK_RK2_2,          // qout = qn + (h/2)*(k1 + R)
+    SK_TVD_1,          // T = qn + h*R
+    SK_TVD_2,          // qout = 0.5*((qn + T) + h*R)
+    SK_RK4_1,          // acc = R; T = qn + ((dt/2)*R)
+    SK_RK4_2,          // acc = acc + 2R; T = qn + ((dt/2)*R)
+    SK_RK4_3,          // acc = acc + 2R; T = qn + (dt*R)
+    SK_RK4_4,          // qout = qn + (h/6)*(acc + R)
+};
+
+struct StageArgs {
+    double4* qin;          // stage input (ghost slots written by the gradient kernel)
+    const double4* qn;     // state at the start of the step
+    double4* qout;         // stage output (T or q^{n+1})
+    double4* acc;          // k1 (RK2) or the RK4 accumulator
+    double4* R;            // residual output (SK_RESIDUAL) or null
+    double* grad;          // N*8 (var, dir)
+    double* lim;           // N*4 (scalar) or N*8 (LP)
+    double* lenF;          // (F+1)*4: len*flux per face
+    double* lenS;          // (F+1): len*sMax per face
+    double4* estate;       // optional (F)*2 edge states for .esln, null otherwise
+    double* celldt;        // N local time step (localTimestep)
+    DevScalars* sc;
+    int kind;              // StageKind
+    int limit;             // limiter on for this stage
+    int dtUpdate;          // local dt update on this stage
+    int first;             // first stage of a step (resets Rmax)
+    int last;              // last stage of a step (NaN scan of the new state)
+    int checkActive;       // honour sc->active (time loop) or always run
+    int recordErrors;
+    uint32_t stageId;      // global stage counter for error ordering
+    uint32_t stepId;
+};
+
+// Launchers (kernels.hip).  All run on `stream`.
+void launchGradient(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t stream);
+void launchFaces(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t stream);
+void launchUpdate(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t stream);
+
+struct DiagArgs {
+    const double4* qnew;     // state after the step (device order)
+    const double4* qold;     // state before the step
+    uint32_t N, L;           // interior cells, cells incl. ghosts (reference array length)
+    const uint32_t* perm;    // reference index -> device index (null = identity)
+    double* chunkTrees;      // nChunks * 8 * 16 partial pairwise trees
+    uint32_t* chunkMask;     // nChunks * 8 bit masks of the levels present
+    const uint32_t* forceFaces; uint32_t nForce;
+    const double* lenF;
+    double rot[4];           // freestream AoA rotation (row-major)
+    double tEnd;
+    int dynamicDt, cflAdjust;
+    double* hist;            // per step: t, dt, rms[4], Rmax, drag, lift, CFL
+    int64_t histBase;        // step index of hist[0]
+    int64_t histCap;
+    int64_t maxSteps;        // stop after this many completed steps (<0: none)
+    DevScalars* sc;
+};
+constexpr int HIST_FIELDS = 10;
+constexpr uint32_t RMS_CHUNK_BLOCKS = 1024;     // 16-element blocks per workgroup
+void launchRmsChunks(const DiagArgs& d, hipStream_t stream);
+void launchFinalize(const DiagArgs& d, hipStream_t stream);
+void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStream_t stream);
+
+}  // namespace ebb

@@ -1,0 +1,93 @@
+// Host-side data model: raw mesh, built geometry (reference numbering), config.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ebb {
+
+constexpr int MAXE = 6;            // source/mesh.d:26 MAX_EDGES
+constexpr double GAMMA = 1.4;      // source/euler.d:89
+
+enum BoundaryType { FullState = 0, ConstPressure, InviscidWall, ViscousWall, Symmetry, TempPresInflow, Dirichlet };
+enum FluxKind { RusanovFlux = 0, RoeFlux = 1 };
+enum IntegratorKind { IntEuler = 0, IntRK2 = 1, IntRK2TVD = 2, IntRK4 = 3 };
+
+// Mesh as read from a file (source/io.d:689-990) -- buildMesh inputs.
+struct RawMesh {
+    std::vector<double> nodes;          // 2 per node
+    std::vector<uint32_t> eoff;         // element offsets, size nElem+1
+    std::vector<uint32_t> enodes;       // 1-based node ids, counter-clockwise
+    std::vector<uint32_t> bnodes;       // 2 per boundary line, 0-based
+    std::vector<std::string> btags;     // boundary tag names
+    std::vector<uint64_t> bgroupStart;  // first bnode of each tag
+};
+
+// buildMesh output (source/mesh.d:217-643), reference cell/edge numbering.
+struct BuiltMesh {
+    uint32_t nElem = 0, nCells = 0, nEdges = 0;
+    // cells (ghosts have index >= nElem, one per boundary edge in boundaryEdges order)
+    std::vector<uint8_t> nEdgesCell;     // nElem
+    std::vector<uint32_t> cellEdges;     // MAXE per interior cell
+    std::vector<int8_t> fluxMult;        // MAXE per interior cell (+1 owner, -1 otherwise)
+    std::vector<uint32_t> nbr;           // MAXE per interior cell
+    std::vector<double> gradMat;         // 2*MAXE per interior cell, row-major (dir, nbr)
+    std::vector<double> area, d, perim;  // nCells (ghosts zero)
+    std::vector<double> cx, cy;          // nCells (ghost centroids reflected)
+    // edges
+    std::vector<uint32_t> eNode0, eNode1, eCell0, eCell1;
+    std::vector<double> len, nx, ny, mx, my;
+    std::vector<int32_t> eBGroup;        // -1 interior
+    std::vector<uint32_t> interiorEdges, boundaryEdges;
+    std::vector<std::vector<uint32_t>> bGroups;
+};
+
+struct BCConfig {
+    std::string tag;
+    int type = FullState;
+    std::vector<double> data;
+};
+
+// source/config.d:72-94 + loadConfig defaults (config.d:112-313)
+struct Config {
+    std::string meshFile, flux, integrator = "Euler";
+    double dt = 0, tEnd = 0, CFL = 0.5, lpThresh = -1;
+    long long saveIter = -1, plotIter = -1, order = 2;
+    std::vector<std::string> forceBoundary;
+    bool dynamicDt = true, limited = true, localTimestep = false, multistageLimiting = true;
+    bool cflAdjust = true, viscosity = false;
+    double ic[4] = {0, 0, 0, 0};
+    double Pr = 0.7, R = 8.3145, gamma = 1.4, mu = 0, Re = 1000, L = 1;
+    std::vector<BCConfig> bcs;
+    // extensions (namespaced key "extensions"; absent from reference configs)
+    bool mms = false;                 // manufactured solution: dirichlet = exact, source per stage
+    long long maxSteps = -1;          // stop after this many steps (-1: only tEnd)
+    int precision = 0;                // 0 = fp64, 1 = mixed fp32 flux/reconstruction/limiter
+    std::vector<std::string> warnings;
+    int fluxKind() const { return flux == "roeFlux" ? RoeFlux : RusanovFlux; }
+    int integratorKind() const {
+        if (integrator == "RK2") return IntRK2;
+        if (integrator == "RK2_TVD") return IntRK2TVD;
+        if (integrator == "RK4") return IntRK4;
+        return IntEuler;
+    }
+};
+
+// mesh.cpp
+bool parseGmsh(const std::string& path, RawMesh& out, std::string& err);
+bool buildMesh(const RawMesh& raw, BuiltMesh& out, std::string& err);
+
+// config.cpp
+bool parseConfig(const std::string& json, Config& out, std::string& err);
+
+// mms.cpp (source/manufacturedsolution.d)
+void mmsSolution(double x, double y, double q[4]);
+void mmsSolutionGradient(double x, double y, double g[8]);
+void mmsSourceTerm(double x, double y, bool viscosity, double R, double mu, double Pr, double S[4]);
+void freestreamState(double M, double aoaDeg, double rho, double q[4]);
+double aoaRadians(double aoaDeg);
+
+// io.cpp (.esln v2, .frc)
+uint32_t crc32(const uint8_t* p, size_t n);
+
+}  // namespace ebb

@@ -1,0 +1,1072 @@
+// HIP kernels for the explicit residual-and-update loop, FP64, gfx950.
+//
+// One residual evaluation (source/solve.d:322-1122) is three launches:
+//   k_gradient  per interior cell: ghost states of its boundary faces
+//               (solve.d:520-689), LSQ gradient, min/max bounds and the scalar
+//               limiter at neighbour centroids (solve.d:324-403), LP vector
+//               limiter (solve.d:404-497);
+//   k_faces     per face, each computed once: reconstruction at the midpoint with
+//               the p<0 fallback (solve.d:283-319, 970-1011), Roe/Rusanov flux
+//               (flux.d:66-303), viscous flux (flux.d:28-64), boundary fluxes
+//               (solve.d:769-968); stores len*flux and len*sMax per face;
+//   k_update    per interior cell: gathers its faces in the cell's own edge
+//               order with the owner sign (solve.d:1082-1118), CFL time step
+//               and Rmax (block reduction + one ordered-bits atomic per block,
+//               exact because min/max are order independent), the MMS source
+//               (app/ms.d:131-141) and the integrator stage update
+//               (source/integrators.d:22-270) fused in.
+// No atomics touch the residual: each cell sums its own faces in a fixed order,
+// so results are deterministic and bit-identical to the edge-loop form.
+// Built with -ffp-contract=off: every product and sum rounds as written.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+#include "dev.hpp"
+
+namespace ebb {
+
+#define DEV __device__ __forceinline__
+constexpr double G_ = 1.4;
+constexpr int BLK = 256;
+
+enum { BT_FULL = 0, BT_CONSTP, BT_INVWALL, BT_VISCWALL, BT_SYM, BT_TPIN, BT_DIRICHLET };
+
+// workgroup b runs on XCD b % 8; hand each XCD a contiguous slice of the cells
+DEV uint32_t xcdBlock(uint32_t b, uint32_t nb)
+{
+    const uint32_t nx = 8, per = nb / nx, rem = nb % nx, x = b % nx, k = b / nx;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+DEV double pressure(const double q[4])
+{
+    const double a = q[1] / q[0], b = q[2] / q[0];
+    return (G_ - 1) * (q[3] - 0.5 * q[0] * (a * a + b * b));
+}
+
+DEV double dmax(double a, double b) { return a < b ? b : a; }
+
+DEV double harten(double lam, double eps)
+{
+    if (lam < eps) lam = (eps * eps + lam * lam) / (2.0 * eps);
+    return lam;
+}
+
+// flux.d:19-24
+DEV void convective(double p, double u, double v, double rho, double e, double n0, double n1, double out[4])
+{
+    out[0] = (rho * u) * n0 + (rho * v) * n1;
+    out[1] = (p + rho * (u * u)) * n0 + (rho * u * v) * n1;
+    out[2] = (rho * u * v) * n0 + (p + rho * (v * v)) * n1;
+    out[3] = (u * (p + e)) * n0 + (v * (p + e)) * n1;
+}
+
+// flux.d:224-303
+DEV void roeFlux(const double qL[4], const double qR[4], double n0, double n1, double out[4], double& sMax)
+{
+    const double rhoL = qL[0], uL = qL[1] / rhoL, vL = qL[2] / rhoL;
+    const double rvL0 = rhoL * uL, rvL1 = rhoL * vL;
+    const double pL = ((G_ - 1) * (qL[3] - 0.5 * rhoL * (uL * uL + vL * vL)));
+    const double hL = (qL[3] + pL) / rhoL;
+    const double rhoR = qR[0], uR = qR[1] / rhoR, vR = qR[2] / rhoR;
+    const double rvR0 = rhoR * uR, rvR1 = rhoR * vR;
+    const double pR = ((G_ - 1) * (qR[3] - 0.5 * rhoR * (uR * uR + vR * vR)));
+    const double hR = (qR[3] + pR) / rhoR;
+    const double di = sqrt(rhoR / rhoL);
+    const double d1 = 1.0 / (1.0 + di);
+    const double vel0 = (di * uR + uL) * d1, vel1 = (di * vR + vL) * d1;
+    const double u = vel0 * n0 + vel1 * n1;
+    const double h = (di * hR + hL) * d1;
+    const double a = sqrt((G_ - 1.0) * (h - 0.5 * (vel0 * vel0 + vel1 * vel1)));
+    const double dRhoV0 = rvR0 - rvL0, dRhoV1 = rvR1 - rvL1;
+    const double dRho = rhoR - rhoL, dRhoE = qR[3] - qL[3];
+    const double eps = 0.01 * a;
+    const double lam1 = harten(fabs(u + a), eps);
+    const double lam2 = harten(fabs(u - a), eps);
+    const double lam3 = harten(fabs(u), eps);
+    const double lam4 = harten(fabs(u), eps);
+    sMax = dmax(dmax(dmax(fabs(lam1), fabs(lam2)), fabs(lam3)), fabs(lam4));
+    const double s1 = 0.5 * (lam1 + lam2), s2 = 0.5 * (lam1 - lam2);
+    const double qq = sqrt(vel0 * vel0 + vel1 * vel1);
+    const double G1 = (G_ - 1) * (((qq * qq) / 2.0) * dRho - (vel0 * dRhoV0 + vel1 * dRhoV1) + dRhoE);
+    const double G2 = -u * dRho + (dRhoV0 * n0 + dRhoV1 * n1);
+    const double C1 = (G1 / (a * a)) * (s1 - lam3) + G2 / a * s2;
+    const double C2 = (G1 / a) * s2 + (s1 - lam3) * G2;
+    double fL[4], fR[4];
+    convective(pL, uL, vL, rhoL, qL[3], n0, n1, fL);
+    convective(pR, uR, vR, rhoR, qR[3], n0, n1, fR);
+    const double d[4] = {lam3 * dRho + C1, lam3 * dRhoV0 + C1 * vel0 + C2 * n0,
+                         lam3 * dRhoV1 + C1 * vel1 + C2 * n1, lam3 * dRhoE + C1 * h + C2 * u};
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = 0.5 * (fL[k] + fR[k]) - 0.5 * d[k];
+}
+
+// flux.d:66-141 (Roe-type averages, C1 = C2 = 0: central flux + sMax jump)
+DEV void rusanovFlux(const double qL[4], const double qR[4], double n0, double n1, double out[4], double& sMax)
+{
+    const double rhoL = qL[0], uL = qL[1] / rhoL, vL = qL[2] / rhoL;
+    const double rvL0 = rhoL * uL, rvL1 = rhoL * vL;
+    const double pL = ((G_ - 1) * (qL[3] - 0.5 * rhoL * (uL * uL + vL * vL)));
+    const double hL = (qL[3] + pL) / rhoL;
+    const double rhoR = qR[0], uR = qR[1] / rhoR, vR = qR[2] / rhoR;
+    const double rvR0 = rhoR * uR, rvR1 = rhoR * vR;
+    const double pR = ((G_ - 1) * (qR[3] - 0.5 * rhoR * (uR * uR + vR * vR)));
+    const double hR = (qR[3] + pR) / rhoR;
+    const double rho = sqrt(rhoL * rhoR);
+    const double sL = sqrt(rhoL), sR = sqrt(rhoR);
+    const double vel0 = (sL * uL + sR * uR) / (sL + sR);
+    const double vel1 = (sL * vL + sR * vR) / (sL + sR);
+    const double u = vel0 * n0 + vel1 * n1;
+    const double p = (sL * pL + sR * pR) / (sL + sR);
+    const double h = (sL * hL + sR * hR) / (sL + sR);
+    const double a = sqrt(G_ * (p / rho));
+    const double dRhoV0 = rvR0 - rvL0, dRhoV1 = rvR1 - rvL1;
+    const double dRho = rhoR - rhoL, dRhoE = qR[3] - qL[3];
+    const double eps = 0.01 * a;
+    const double lam1 = harten(fabs(u + a), eps);
+    const double lam2 = harten(fabs(u - a), eps);
+    const double lam3 = harten(fabs(u), eps);
+    const double lam4 = harten(fabs(u), eps);
+    const double sm = dmax(dmax(dmax(fabs(lam1), fabs(lam2)), fabs(lam3)), fabs(lam4));
+    sMax = sm;
+    const double s1 = 0.5 * (sm + sm), s2 = 0.5 * (sm - sm);
+    const double qq = sqrt(vel0 * vel0 + vel1 * vel1);
+    const double G1 = (G_ - 1) * ((qq * qq / 2) * dRho - (vel0 * dRhoV0 + vel1 * dRhoV1) + dRhoE);
+    const double G2 = -u * dRho + (dRhoV0 * n0 + dRhoV1 * n1);
+    const double C1 = (G1 / (a * a)) * (s1 - sm) + G2 / a * s2;
+    const double C2 = (G1 / a) * s2 + (s1 - sm) * G2;
+    double fL[4], fR[4];
+    convective(pL, uL, vL, rhoL, qL[3], n0, n1, fL);
+    convective(pR, uR, vR, rhoR, qR[3], n0, n1, fR);
+    const double d[4] = {sm * dRho + C1, sm * (dRhoV0) + C1 * vel0 + C2 * n0,
+                         sm * (dRhoV1) + C1 * vel1 + C2 * n1, sm * dRhoE + C1 * h + C2 * u};
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = 0.5 * (fL[k] + fR[k]) - 0.5 * d[k];
+}
+
+template <int FLUX>
+DEV void numFlux(const double qL[4], const double qR[4], double n0, double n1, double out[4], double& sMax)
+{
+    if (FLUX == 1) roeFlux(qL, qR, n0, n1, out, sMax);
+    else rusanovFlux(qL, qR, n0, n1, out, sMax);
+}
+
+// flux.d:28-64.  The four 4x4 matrices are applied entry by entry in column
+// order; their structural zeros only ever add a signed zero, so they are skipped.
+DEV void viscousFlux(double Pr, double mu, const double q[4], const double dq[8], double n0, double n1, double out[4])
+{
+    const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3] / q[0];
+    const double gp = G_ / Pr;
+    const double s = mu / rho;
+    const double c43 = 4.0 / 3.0, c23 = 2.0 / 3.0, cm43 = -4.0 / 3.0, cm23 = -2.0 / 3.0, c231 = (2.0 / 3.0 - 1.0);
+    const double t0 = (-4.0 / 3.0 + gp) * (u * u) + (-1.0 + gp) * (v * v) - gp * E;
+    const double t1 = (-1.0 + gp) * (u * u) + (-4.0 / 3.0 + gp) * (v * v) - gp * E;
+    const double x0 = dq[0], x1 = dq[2], x2 = dq[4], x3 = dq[6];
+    const double y0 = dq[1], y1 = dq[3], y2 = dq[5], y3 = dq[7];
+    const double uv = c231 * u * v;
+    const double G01 = ((s * (cm43 * u)) * x0 + (s * c43) * x1) + ((s * (c23 * v)) * y0 + (s * cm23) * y2);
+    const double G02 = ((s * -v) * x0 + (s * 1.0) * x2) + ((s * -u) * y0 + (s * 1.0) * y1);
+    const double G03 = ((s * t0) * x0 + (s * ((c43 - gp) * u)) * x1 + (s * ((1.0 - gp) * v)) * x2 + (s * gp) * x3)
+                     + ((s * uv) * y0 + (s * v) * y1 + (s * (cm23 * u)) * y2);
+    const double G11 = ((s * -v) * x0 + (s * 1.0) * x2) + ((s * -u) * y0 + (s * 1.0) * y1);
+    const double G12 = ((s * (c23 * u)) * x0 + (s * cm23) * x2) + ((s * (cm43 * v)) * y0 + (s * c43) * y2);
+    const double G13 = ((s * uv) * x0 + (s * (cm23 * v)) * x1 + (s * u) * x2)
+                     + ((s * t1) * y0 + (s * ((1.0 - gp) * u)) * y1 + (s * ((c43 - gp) * v)) * y2 + (s * gp) * y3);
+    out[0] = 0.0;
+    out[1] = G01 * n0 + G11 * n1;
+    out[2] = G02 * n0 + G12 * n1;
+    out[3] = G03 * n0 + G13 * n1;
+}
+
+DEV void ld4(const double4* p, uint32_t i, double o[4])
+{
+    const double4 t = p[i];
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+DEV void st4(double4* p, uint32_t i, const double o[4]) { p[i] = make_double4(o[0], o[1], o[2], o[3]); }
+
+// ghost state from the owner state (solve.d:520-689); the owner is the only writer
+DEV void ghostState(const BFace& b, const double qo[4], double qg[4], const double qgOld[4])
+{
+    switch (b.type) {
+    case BT_DIRICHLET:
+        qg[0] = b.gq[0]; qg[1] = b.gq[1]; qg[2] = b.gq[2]; qg[3] = b.gq[3];
+        break;
+    case BT_INVWALL:
+    case BT_SYM: {
+        const double v0 = qo[1] / qo[0], v1 = qo[2] / qo[0];
+        double w0, w1;
+        if (!b.vertical) {
+            w0 = 0.0 + b.R[0] * v0 + b.R[1] * v1;
+            w1 = 0.0 + b.R[2] * v0 + b.R[3] * v1;
+        } else {
+            w0 = -v0;
+            w1 = v1;
+        }
+        qg[0] = qo[0]; qg[1] = qo[0] * w0; qg[2] = qo[0] * w1; qg[3] = qo[3];
+        break;
+    }
+    case BT_VISCWALL:
+        qg[0] = qo[0]; qg[1] = -qo[1]; qg[2] = -qo[2]; qg[3] = qo[3];
+        break;
+    case BT_CONSTP: {
+        const double rhop = qo[0], up = qo[1] / qo[0], vp = qo[2] / qo[0];
+        const double pp = pressure(qo);
+        const double ap = sqrt(G_ * (pp / rhop));
+        const double Jp = up + (2.0 * ap) / (G_ - 1);
+        const double Sp = pp / pow(rhop, G_);
+        const double pb = b.data[0];
+        const double rhob = pow(pb / Sp, 1.0 / G_);
+        const double ab = sqrt(G_ * (pb / rhob));
+        const double ub = Jp - (2.0 * ab) / (G_ - 1);
+        const double vb = vp;
+        const double mag = sqrt(ub * ub + vb * vb);
+        const double rEb = pb / (G_ - 1) + 0.5 * rhob * (mag * mag);
+        const double n0 = b.nrm[0], n1 = b.nrm[1];
+        const double vpn = up * n0 + vp * n1, vbn = ub * n0 + vb * n1;
+        qg[0] = rhob;
+        qg[1] = rhob * (up - vpn * n0 + vbn * n0);
+        qg[2] = rhob * (vp - vpn * n1 + vbn * n1);
+        qg[3] = rEb;
+        break;
+    }
+    case BT_TPIN: {
+        // solve.d:625-683, including its four writes to component 0
+        const double Tt = b.data[0], Pt = b.data[1], ni0 = b.data[2], ni1 = b.data[3];
+        const double rhop = qo[0], up = qo[1] / qo[0];
+        const double pp = pressure(qo);
+        const double ap = sqrt(G_ * (pp / rhop));
+        const double Jp = up + (2.0 * ap) / (G_ - 1);
+        const double d = ni0 * b.nrm[0] + ni1 * b.nrm[1];
+        const double Rg = b.Rgas;
+        const double A = G_ * Rg * Tt * (d * d) - 0.5 * (G_ - 1) * (Jp * Jp);
+        const double B = (4.0 * G_ * Rg * Tt * d) / (G_ - 1);
+        const double Cc = (4.0 * G_ * Rg * Tt) / ((G_ - 1) * (G_ - 1)) - Jp * Jp;
+        const double M1 = (-B + sqrt(B * B - 4 * A * Cc)) / (2.0 * A);
+        const double M2 = (-B - sqrt(B * B - 4 * A * Cc)) / (2.0 * A);
+        double Mb = NAN;
+        if (M1 >= 0 && M2 >= 0) Mb = fmin(M1, M2);
+        else if (M1 >= 0 && M2 < 0) Mb = M1;
+        else if (M1 < 0 && M2 >= 0) Mb = M2;
+        const double Tb = Tt / (1 + 0.5 * (G_ - 1) * (Mb * Mb));
+        const double pb = Pt * pow(Tb / Tt, G_ / (G_ - 1));
+        const double rhob = pb / (Rg * Tb);
+        const double ab = sqrt(G_ * (pb / rhob));
+        const double vb0 = Mb * ab * ni0, vb1 = Mb * ab * ni1;
+        const double mag = sqrt(vb0 * vb0 + vb1 * vb1);
+        qg[0] = pb / (G_ - 1) + 0.5 * rhob * (mag * mag);
+        qg[1] = qgOld[1]; qg[2] = qgOld[2]; qg[3] = qgOld[3];
+        break;
+    }
+    default:  // FullState
+        qg[0] = qo[0]; qg[1] = qo[1]; qg[2] = qo[2]; qg[3] = qo[3];
+        break;
+    }
+}
+
+// solve.d:27-173 LP vector limiter (all-inequality simplex, at most 16 pivots)
+DEV void lpSolve(int rows, const double* A, const double* b, const double c[2], double xk[2])
+{
+    int W0 = 0, W1 = 1;
+    double Ak[4] = {A[0], A[1], A[2], A[3]};
+    double gam[12];
+    int D[12];
+    for (int k = 0; k <= 15; k++) {
+        double det = Ak[0] * Ak[3] - Ak[1] * Ak[2];
+        double s = 1.0 / det;
+        const double iT0 = Ak[3] * s, iT1 = -Ak[2] * s, iT2 = -Ak[1] * s, iT3 = Ak[0] * s;
+        const double lam0 = 0.0 + iT0 * c[0] + iT1 * c[1];
+        const double lam1 = 0.0 + iT2 * c[0] + iT3 * c[1];
+        if ((lam0 >= -10e-9) && (lam1 >= -10e-9)) break;
+        const int q = (lam0 < lam1) ? W0 : W1;
+        const double e0 = (lam0 < lam1) ? 1.0 : 0.0, e1 = (lam0 < lam1) ? 0.0 : 1.0;
+        det = Ak[0] * Ak[3] - Ak[1] * Ak[2];
+        s = 1.0 / det;
+        const double i0 = Ak[3] * s, i1 = -Ak[1] * s, i2 = -Ak[2] * s, i3 = Ak[0] * s;
+        const double pk0 = 0.0 + i0 * e0 + i1 * e1;
+        const double pk1 = 0.0 + i2 * e0 + i3 * e1;
+        int Dlen = 0;
+        for (int i = 0; i < rows; i++)
+            if (i != W0 && i != W1)
+                if (A[2 * i] * pk0 + A[2 * i + 1] * pk1 < -1e-11) D[Dlen++] = i;
+        double alpha = INFINITY;
+        for (int i = 0; i < Dlen; i++) {
+            const double aixk = A[2 * D[i]] * xk[0] + A[2 * D[i] + 1] * xk[1];
+            const double aipk = A[2 * D[i]] * pk0 + A[2 * D[i] + 1] * pk1;
+            gam[i] = (aixk - b[D[i]]) / (-aipk);
+            alpha = fmin(alpha, gam[i]);
+        }
+        int t = 0;
+        for (int i = 0; i < Dlen; i++) if (gam[i] == alpha) { t = D[i]; break; }
+        if (q == W0) W0 = t; else if (q == W1) W1 = t;
+        Ak[0] = A[2 * W0]; Ak[1] = A[2 * W0 + 1]; Ak[2] = A[2 * W1]; Ak[3] = A[2 * W1 + 1];
+        xk[0] = xk[0] + alpha * pk0;
+        xk[1] = xk[1] + alpha * pk1;
+    }
+}
+
+// ---------------------------------------------------------------- gradient + limiter
+// MODE 0: ghost states only (order 1); 1: unlimited gradient; 2: scalar limiter;
+// 3: scalar limiter followed by the LP vector limiter (lim stored as x,y pairs).
+template <int S, int MODE>
+__global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, StageArgs a)
+{
+    if (a.checkActive && !a.sc->active) return;
+    const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
+    if (bid == 0 && threadIdx.x == 0) {
+        a.sc->newDtBits = 0x7FF0000000000000ull;   // +inf
+        if (a.first) a.sc->rmaxBits = 0;
+    }
+    const uint32_t i = bid * BLK + threadIdx.x;
+    if (i >= m.N) return;
+    double qi[4];
+    ld4(a.qin, i, qi);
+    double mn[4], mx[4], du[4][S];
+    uint32_t nb[S];
+#pragma unroll
+    for (int k = 0; k < 4; k++) { mn[k] = qi[k]; mx[k] = qi[k]; }
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        nb[j] = m.nbr[i * S + j];
+        double qn[4];
+        if (nb[j] >= m.N) {
+            double old[4] = {0, 0, 0, 0};
+            const BFace& bf = m.bface[nb[j] - m.N];
+            if (bf.type == BT_TPIN) ld4(a.qin, nb[j], old);
+            ghostState(bf, qi, qn, old);
+            st4(a.qin, nb[j], qn);
+        } else {
+            ld4(a.qin, nb[j], qn);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            mn[k] = fmin(mn[k], qn[k]);
+            mx[k] = fmax(mx[k], qn[k]);
+            du[k][j] = qn[k] - qi[k];
+        }
+    }
+    if (MODE == 0) return;
+    double g[4][2];
+    const double* gm = m.gm + size_t(i) * 2 * S;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < S; j++) s += gm[r * S + j] * du[k][j];
+            g[k][r] = s;
+        }
+    double lx[4] = {1.0, 1.0, 1.0, 1.0};
+    double ly[4] = {1.0, 1.0, 1.0, 1.0};
+    if (MODE >= 2 && a.limit) {
+        const double2 ci = m.cen[i];
+        double dxs[S], dys[S];
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const double2 cn = m.cen[nb[j]];
+            const double dx = cn.x - ci.x, dy = cn.y - ci.y;
+            dxs[j] = dx; dys[j] = dy;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                double s = 1.0;
+                const double t = g[k][0] * dx + g[k][1] * dy;
+                const double qE = qi[k] + t;
+                if (qE < mn[k]) s = (mn[k] - qi[k]) / t;
+                else if (qE > mx[k]) s = (mx[k] - qi[k]) / t;
+                lx[k] = fmin(lx[k], s);
+                ly[k] = lx[k];
+            }
+        }
+        if (MODE == 3 && p.lpThresh > 0) {
+            int nn = 0;
+#pragma unroll
+            for (int j = 0; j < S; j++) nn += (nb[j] != i) ? 1 : 0;
+            for (int k = 0; k < 4; k++) {
+                if (lx[k] < p.lpThresh && (nn == 3 || nn == 4)) {
+                    double A[24], bb[12];
+                    for (int r = 0; r < 24; r++) A[r] = 0.0;
+                    for (int r = 0; r < 12; r++) bb[r] = 0.0;
+                    A[0] = 1; A[3] = 1; A[4] = -1; A[7] = -1;
+                    bb[2] = -1; bb[3] = -1;
+                    int c2 = 0;
+                    for (int j = 0; j < S; j++) {
+                        if (nb[j] == i) continue;
+                        A[2 * (c2 + 4)] = dxs[j] * g[k][0];
+                        A[2 * (c2 + 4) + 1] = dys[j] * g[k][1];
+                        bb[c2 + 4] = mn[k] - qi[k];
+                        A[2 * (c2 + 5)] = -dxs[j] * g[k][0];
+                        A[2 * (c2 + 5) + 1] = -dys[j] * g[k][1];
+                        bb[c2 + 5] = qi[k] - mx[k];
+                        c2 += 2;
+                    }
+                    const double cc[2] = {-fabs(g[k][0]), -fabs(g[k][1])};
+                    double xk[2] = {0.0, 0.0};
+                    lpSolve(4 + 2 * nn, A, bb, cc, xk);
+                    lx[k] = xk[0];
+                    ly[k] = xk[1];
+                }
+            }
+        }
+    }
+    double* gout = a.grad + size_t(i) * 8;
+    *reinterpret_cast<double4*>(gout) = make_double4(g[0][0], g[0][1], g[1][0], g[1][1]);
+    *reinterpret_cast<double4*>(gout + 4) = make_double4(g[2][0], g[2][1], g[3][0], g[3][1]);
+    if (MODE == 3) {
+        double* lo = a.lim + size_t(i) * 8;
+        *reinterpret_cast<double4*>(lo) = make_double4(lx[0], ly[0], lx[1], ly[1]);
+        *reinterpret_cast<double4*>(lo + 4) = make_double4(lx[2], ly[2], lx[3], ly[3]);
+    } else {
+        *reinterpret_cast<double4*>(a.lim + size_t(i) * 4) = make_double4(lx[0], lx[1], lx[2], lx[3]);
+    }
+}
+
+// ---------------------------------------------------------------- faces
+template <int LP>
+DEV void loadLim(const double* lim, uint32_t c, double lx[4], double ly[4])
+{
+    if (LP) {
+        const double4 a = *reinterpret_cast<const double4*>(lim + size_t(c) * 8);
+        const double4 b = *reinterpret_cast<const double4*>(lim + size_t(c) * 8 + 4);
+        lx[0] = a.x; ly[0] = a.y; lx[1] = a.z; ly[1] = a.w; lx[2] = b.x; ly[2] = b.y; lx[3] = b.z; ly[3] = b.w;
+    } else {
+        const double4 a = *reinterpret_cast<const double4*>(lim + size_t(c) * 4);
+        lx[0] = ly[0] = a.x; lx[1] = ly[1] = a.y; lx[2] = ly[2] = a.z; lx[3] = ly[3] = a.w;
+    }
+}
+
+DEV void loadGrad(const double* grad, uint32_t c, double g[8])
+{
+    const double4 a = *reinterpret_cast<const double4*>(grad + size_t(c) * 8);
+    const double4 b = *reinterpret_cast<const double4*>(grad + size_t(c) * 8 + 4);
+    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w; g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+}
+
+// solve.d:980-1010 (interior) reconstruction with the p<0 fallback
+template <int LP>
+DEV void reconstruct(const double qm[4], const double g[8], const double lx[4], const double ly[4],
+                     double dx, double dy, double out[4])
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) out[j] = qm[j] + lx[j] * g[2 * j] * dx + ly[j] * g[2 * j + 1] * dy;
+    if (pressure(out) < 0) {
+        double l0 = 1.0, l1 = 1.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) { l0 = fmin(l0, lx[j]); l1 = fmin(l1, ly[j]); }
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[j] = qm[j] + l0 * g[2 * j] * dx + l1 * g[2 * j + 1] * dy;
+    }
+}
+
+template <int FLUX, int VISC, int ORDER, int LP, int STORE>
+__global__ __launch_bounds__(BLK) void k_faces(DevMesh m, PhysParams p, StageArgs a, uint32_t nbi)
+{
+    if (a.checkActive && !a.sc->active) return;
+    double qL[4], qR[4], F[4], sMax;
+    uint32_t f;
+    double len;
+    uint64_t errKey;
+    if (blockIdx.x < nbi) {
+        f = xcdBlock(blockIdx.x, nbi) * BLK + threadIdx.x;
+        if (f >= m.Fi) return;
+        const uint2 c = m.fcell[f];
+        const double2 n = m.fn[f];
+        len = m.flen[f];
+        double gL[8], gR[8];
+        ld4(a.qin, c.x, qL);
+        ld4(a.qin, c.y, qR);
+        if (ORDER > 1 || VISC) { loadGrad(a.grad, c.x, gL); loadGrad(a.grad, c.y, gR); }
+        if (ORDER > 1) {
+            const double2 mid = m.fmid[f];
+            const double2 cL = m.cen[c.x], cR = m.cen[c.y];
+            double lx[4], ly[4], tL[4], tR[4];
+            loadLim<LP>(a.lim, c.x, lx, ly);
+            reconstruct<LP>(qL, gL, lx, ly, mid.x - cL.x, mid.y - cL.y, tL);
+            loadLim<LP>(a.lim, c.y, lx, ly);
+            reconstruct<LP>(qR, gR, lx, ly, mid.x - cR.x, mid.y - cR.y, tR);
+#pragma unroll
+            for (int k = 0; k < 4; k++) { qL[k] = tL[k]; qR[k] = tR[k]; }
+        }
+        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+        if (VISC) {
+            double qa[4], dq[8], Fv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gL[k] + gR[k]);
+            viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
+#pragma unroll
+            for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+        }
+        errKey = uint64_t(m.Fb) + f;   // boundary edges are checked first in the reference
+    } else {
+        const uint32_t b = (blockIdx.x - nbi) * BLK + threadIdx.x;
+        if (b >= m.Fb) return;
+        f = m.Fi + b;
+        const uint32_t o = m.fcell[f].x, gc = m.N + b;
+        const double2 n = m.fn[f];
+        len = m.flen[f];
+        const BFace& bf = m.bface[b];
+        double gO[8];
+        double qo[4];
+        ld4(a.qin, o, qo);
+        if (ORDER > 1 || VISC) loadGrad(a.grad, o, gO);
+        // solve.d:283-319 updateBoundaryEdge: the fallback uses lim[0] for both directions
+        if (ORDER > 1) {
+            const double2 mid = m.fmid[f], co = m.cen[o];
+            const double dx = mid.x - co.x, dy = mid.y - co.y;
+            double lx[4], ly[4];
+            loadLim<LP>(a.lim, o, lx, ly);
+#pragma unroll
+            for (int j = 0; j < 4; j++) qL[j] = qo[j] + lx[j] * gO[2 * j] * dx + ly[j] * gO[2 * j + 1] * dy;
+            if (pressure(qL) < 0) {
+                double l0 = 1.0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) l0 = fmin(l0, lx[j]);
+#pragma unroll
+                for (int j = 0; j < 4; j++) qL[j] = qo[j] + l0 * gO[2 * j] * dx + l0 * gO[2 * j + 1] * dy;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) qL[j] = qo[j];
+        }
+        const int type = bf.type;
+        if (type == BT_FULL || type == BT_DIRICHLET) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) qR[k] = bf.far[k];
+            numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+            if (VISC) {
+                double qa[4], dq[8], Fv[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+                if (type == BT_DIRICHLET) {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + bf.gg[k]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + 0.0);
+                }
+                viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
+#pragma unroll
+                for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+            }
+        } else if (type == BT_INVWALL) {
+            const double inv = 1 / qL[0];
+            const double v0 = inv * qL[1], v1 = inv * qL[2];
+            const double vn = v0 * n.x + v1 * n.y;
+            const double t0 = v0 - vn * n.x, t1 = v1 - vn * n.y;
+            const double vel = sqrt(t0 * t0 + t1 * t1);
+            double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
+            const double aw = sqrt(G_ * (pw / qL[0]));
+            if (pw < 0) pw = 1.0e-12;
+            F[0] = 0; F[1] = pw * n.x; F[2] = pw * n.y; F[3] = 0;
+            sMax = fabs(aw);
+            ld4(a.qin, gc, qR);
+        } else if (type == BT_VISCWALL) {
+            const double vel = 0.0;
+            double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
+            const double aw = sqrt(G_ * (pw / qL[0]));
+            if (pw < 0) pw = 1.0e-12;
+            qR[0] = qL[0]; qR[1] = qL[1] * -1; qR[2] = qL[2] * -1; qR[3] = qL[3];
+            double qa[4], dq[8], Fv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+            qa[1] = 0.0; qa[2] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) dq[k] = gO[k];
+            dq[6] = (qa[3] / qa[0]) * dq[0];
+            dq[7] = (qa[3] / qa[0]) * dq[1];
+            viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
+            F[0] = 0 - Fv[0]; F[1] = pw * n.x - Fv[1]; F[2] = pw * n.y - Fv[2]; F[3] = 0 - Fv[3];
+            sMax = fabs(aw);
+        } else if (type == BT_CONSTP) {
+            ld4(a.qin, gc, qR);
+            const double pg = pressure(qR);
+            const double ug = qR[1] / qR[0], vg = qR[2] / qR[0];
+            const double ag = sqrt(G_ * (pg / qR[0]));
+            convective(pg, ug, vg, qR[0], qR[3], n.x, n.y, F);
+            sMax = fabs(ag);
+        } else if (type == BT_SYM) {
+            const double v0 = qL[1] / qL[0], v1 = qL[2] / qL[0];
+            double w0, w1;
+            if (!bf.vertical) {
+                w0 = 0.0 + bf.R[0] * v0 + bf.R[1] * v1;
+                w1 = 0.0 + bf.R[2] * v0 + bf.R[3] * v1;
+            } else {
+                w0 = -v0;
+                w1 = v1;
+            }
+            qR[0] = qL[0]; qR[1] = qL[0] * w0; qR[2] = qL[0] * w1; qR[3] = qL[3];
+            numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+            if (VISC) {
+                double qa[4], bg[8], Fv[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+                const double o00 = n.x * n.x, o01 = n.x * n.y, o10 = n.y * n.x, o11 = n.y * n.y;
+                const double V00 = 1 - o00, V01 = 0 - o01, V10 = 0 - o10, V11 = 1 - o11;
+                const double T00 = 1 - 2 * o00, T01 = 0 - 2 * o01, T10 = 0 - 2 * o10, T11 = 1 - 2 * o11;
+                // tmp1 = A*gradp with A = identity except the momentum block V (4x4 * 4x2, all columns)
+                double t1[8];
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    t1[0 + c] = 0.0 + 1.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 0.0 * gO[6 + c];
+                    t1[2 + c] = 0.0 + 0.0 * gO[0 + c] + V00 * gO[2 + c] + V01 * gO[4 + c] + 0.0 * gO[6 + c];
+                    t1[4 + c] = 0.0 + 0.0 * gO[0 + c] + V10 * gO[2 + c] + V11 * gO[4 + c] + 0.0 * gO[6 + c];
+                    t1[6 + c] = 0.0 + 0.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 1.0 * gO[6 + c];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const double s30 = 0.0 + t1[2 * r] * T00 + t1[2 * r + 1] * T10;
+                    const double s31 = 0.0 + t1[2 * r] * T01 + t1[2 * r + 1] * T11;
+                    const double s10 = 0.0 + gO[2 * r] * o00 + gO[2 * r + 1] * o10;
+                    const double s11 = 0.0 + gO[2 * r] * o01 + gO[2 * r + 1] * o11;
+                    bg[2 * r] = s10 + s30;
+                    bg[2 * r + 1] = s11 + s31;
+                }
+                viscousFlux(p.Pr, p.mu, qa, bg, n.x, n.y, Fv);
+#pragma unroll
+                for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+            }
+        } else {  // TempPresInflow
+            ld4(a.qin, gc, qR);
+            numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+            if (VISC) {
+                double qa[4], Fv[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+                viscousFlux(p.Pr, p.mu, qa, gO, n.x, n.y, Fv);
+#pragma unroll
+                for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+            }
+        }
+        errKey = b;
+    }
+    if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
+        atomicMin(&a.sc->errEdge, (uint64_t(a.stageId) << 32) | errKey);
+    *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) = make_double4(len * F[0], len * F[1], len * F[2], len * F[3]);
+    a.lenS[f] = len * sMax;
+    if (STORE) {
+        st4(a.estate, 2 * f, qL);
+        st4(a.estate, 2 * f + 1, qR);
+    }
+}
+
+// ---------------------------------------------------------------- residual + update
+DEV double warpMin(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+DEV double warpMax(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int S, int KIND>
+__global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageArgs a)
+{
+    if (a.checkActive && !a.sc->active) return;
+    __shared__ double sMin[BLK / 64], sMax_[BLK / 64];
+    const uint32_t i = xcdBlock(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
+    double dtCand = INFINITY, rCand = 0.0;
+    if (i < m.N) {
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        double sAve = 0;
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const uint32_t fe = m.cface[i * S + j];
+            const uint32_t f = fe & 0x7fffffffu;
+            const double4 lf = *reinterpret_cast<const double4*>(a.lenF + size_t(f) * 4);
+            if (fe >> 31) { r[0] = r[0] - lf.x; r[1] = r[1] - lf.y; r[2] = r[2] - lf.z; r[3] = r[3] - lf.w; }
+            else { r[0] = r[0] + lf.x; r[1] = r[1] + lf.y; r[2] = r[2] + lf.z; r[3] = r[3] + lf.w; }
+            sAve += a.lenS[f];
+        }
+        sAve /= m.cper[i];
+        const double CFL = a.sc->CFL;
+        const double dtc = (CFL * m.cd[i]) / sAve;
+        if (!isnan(dtc)) dtCand = dtc;
+        double hc = 0.0;
+        if (p.localTimestep) {
+            if (a.dtUpdate) {
+                hc = CFL * m.cd[i] / sAve;
+                if (isnan(hc)) hc = 0;
+                a.celldt[i] = hc;
+            } else {
+                hc = a.celldt[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) if (fabs(r[k]) > rCand) rCand = fabs(r[k]);
+        const double s = m.nia[i];
+        double R[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) R[k] = r[k] * s;
+        if (m.src) {
+            const double4 sv = m.src[i];
+            R[0] = R[0] + sv.x; R[1] = R[1] + sv.y; R[2] = R[2] + sv.z; R[3] = R[3] + sv.w;
+        }
+        const double dt = a.sc->dt;
+        const double h = p.localTimestep ? hc : dt;
+        double qn[4], o[4];
+        if (KIND != SK_RESIDUAL) ld4(a.qn, i, qn);
+        if (KIND == SK_RESIDUAL) {
+            st4(a.R, i, R);
+        } else if (KIND == SK_EULER) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
+            st4(a.qout, i, o);
+        } else if (KIND == SK_RK2_1) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = qn[k] + (dt * R[k]);
+            st4(a.acc, i, R);
+            st4(a.qout, i, o);
+        } else if (KIND == SK_RK2_2) {
+            double k1[4];
+            ld4(a.acc, i, k1);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 2.0) * (k1[k] + R[k]);
+            st4(a.qout, i, o);
+        } else if (KIND == SK_TVD_1) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
+            st4(a.qout, i, o);
+        } else if (KIND == SK_TVD_2) {
+            double fe[4];
+            ld4(a.qin, i, fe);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = 0.5 * (qn[k] + fe[k] + h * R[k]);
+            st4(a.qout, i, o);
+        } else if (KIND == SK_RK4_1) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = qn[k] + ((dt / 2.0) * R[k]);
+            st4(a.acc, i, R);
+            st4(a.qout, i, o);
+        } else if (KIND == SK_RK4_2 || KIND == SK_RK4_3) {
+            double ac[4];
+            ld4(a.acc, i, ac);
+#pragma unroll
+            for (int k = 0; k < 4; k++) ac[k] = ac[k] + 2.0 * R[k];
+            const double c = (KIND == SK_RK4_2) ? (dt / 2.0) : dt;
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = qn[k] + (c * R[k]);
+            st4(a.acc, i, ac);
+            st4(a.qout, i, o);
+        } else if (KIND == SK_RK4_4) {
+            double ac[4];
+            ld4(a.acc, i, ac);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 6.0) * (ac[k] + R[k]);
+            st4(a.qout, i, o);
+        }
+        if (KIND != SK_RESIDUAL && a.last && a.recordErrors &&
+            (isnan(o[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3])))
+            atomicMin(&a.sc->errCell, (uint64_t(a.stepId) << 32) | m.refId[i]);
+    }
+    dtCand = warpMin(dtCand);
+    rCand = warpMax(rCand);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { sMin[w] = dtCand; sMax_[w] = rCand; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = sMin[0], b0 = sMax_[0];
+#pragma unroll
+        for (int k = 1; k < BLK / 64; k++) { a0 = fmin(a0, sMin[k]); b0 = fmax(b0, sMax_[k]); }
+        atomicMin(&a.sc->newDtBits, (unsigned long long)__double_as_longlong(a0));
+        if (b0 > 0) atomicMax(&a.sc->rmaxBits, (unsigned long long)__double_as_longlong(b0));
+    }
+}
+
+// ---------------------------------------------------------------- diagnostics
+// Phobos std.algorithm.sum over double[] (app/solver.d:153-168): 16-element
+// blocks summed by a fixed tree, blocks merged by a binary counter (perfect
+// pairwise trees over aligned power-of-two runs), remainder elements likewise,
+// and the stack folded from the top.  A workgroup reduces an aligned chunk of
+// RMS_CHUNK_BLOCKS blocks level by level; the finalize kernel replays the
+// counter over the chunk trees.  Quantities 0..3: sum of (now-last)^2 per
+// variable; 4..7: sum of now.
+DEV double tree16(const double* r)
+{
+    return (((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7])))
+         + (((r[8] + r[9]) + (r[10] + r[11])) + ((r[12] + r[13]) + (r[14] + r[15])));
+}
+
+DEV void rmsElement(const DiagArgs& d, uint64_t e, double out[8])
+{
+    if (e < d.N) {
+        const uint32_t dv = d.perm ? d.perm[e] : uint32_t(e);
+        double qa[4], qb[4];
+        ld4(d.qnew, dv, qa);
+        ld4(d.qold, dv, qb);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const double x = qa[v] - qb[v];
+            out[v] = x * x;
+            out[4 + v] = qa[v];
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 8; v++) out[v] = 0.0;   // ghost slots hold 0 in both arrays
+    }
+}
+
+constexpr int PER_T = RMS_CHUNK_BLOCKS / BLK;   // 16-blocks per thread
+
+__global__ __launch_bounds__(BLK) void k_rms_chunks(DiagArgs d)
+{
+    if (!d.sc->active) return;
+    __shared__ double v[8][RMS_CHUNK_BLOCKS];
+    const uint64_t nb = uint64_t(d.L) / 16;
+    const uint64_t c0 = uint64_t(blockIdx.x) * RMS_CHUNK_BLOCKS;
+    const uint32_t cnt = uint32_t((nb - c0) < RMS_CHUNK_BLOCKS ? (nb - c0) : RMS_CHUNK_BLOCKS);
+    for (int u = 0; u < PER_T; u++) {
+        const uint32_t lb = threadIdx.x + u * BLK;
+        if (lb < cnt) {
+            const uint64_t base = (c0 + lb) * 16;
+            for (int var = 0; var < 4; var++) {
+                double sq[16], nw[16];
+#pragma unroll
+                for (int w = 0; w < 16; w++) {
+                    const uint64_t e = base + w;
+                    if (e < d.N) {
+                        const uint32_t dv = d.perm ? d.perm[e] : uint32_t(e);
+                        const double a0 = reinterpret_cast<const double*>(d.qnew + dv)[var];
+                        const double b0 = reinterpret_cast<const double*>(d.qold + dv)[var];
+                        const double x = a0 - b0;
+                        sq[w] = x * x;
+                        nw[w] = a0;
+                    } else {
+                        sq[w] = 0.0;
+                        nw[w] = 0.0;
+                    }
+                }
+                v[var][lb] = tree16(sq);
+                v[4 + var][lb] = tree16(nw);
+            }
+        }
+    }
+    __syncthreads();
+    double* out = d.chunkTrees + size_t(blockIdx.x) * 8 * 16;
+    uint32_t c = cnt;
+    int level = 0;
+    uint32_t mask = 0;
+    while (c > 0) {
+        if (c & 1) {
+            if (threadIdx.x < 8) out[threadIdx.x * 16 + level] = v[threadIdx.x][c - 1];
+            mask |= 1u << level;
+        }
+        const uint32_t half = c / 2;
+        double tmp[PER_T * 4];
+        int n = 0;
+        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += BLK) {
+            const uint32_t q = idx / half, j = idx % half;
+            tmp[n++] = v[q][2 * j] + v[q][2 * j + 1];
+        }
+        __syncthreads();
+        n = 0;
+        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += BLK) {
+            const uint32_t q = idx / half, j = idx % half;
+            v[q][j] = tmp[n++];
+        }
+        __syncthreads();
+        c = half;
+        level++;
+    }
+    if (threadIdx.x < 8) d.chunkMask[blockIdx.x * 8 + threadIdx.x] = mask;
+}
+
+__global__ __launch_bounds__(64) void k_finalize(DiagArgs d)
+{
+    DevScalars* sc = d.sc;
+    if (!sc->active) return;
+    __shared__ double tot[8];
+    const int q = threadIdx.x;
+    if (q < 8) {
+        const uint64_t nb = uint64_t(d.L) / 16;
+        const uint64_t nfull = nb / RMS_CHUNK_BLOCKS, part = nb % RMS_CHUNK_BLOCKS;
+        double st[72];
+        int idx = 0;
+        for (uint64_t k = 0; k < nfull; k++) {
+            st[idx] = d.chunkTrees[k * 128 + q * 16 + 10];
+            const int last = idx - __builtin_ctzll(k + 1);
+            while (idx > last) { st[idx - 1] += st[idx]; --idx; }
+            ++idx;
+        }
+        if (part) {
+            const uint32_t mask = d.chunkMask[nfull * 8 + q];
+            for (int l = 15; l >= 0; l--)
+                if (mask & (1u << l)) st[idx++] = d.chunkTrees[nfull * 128 + q * 16 + l];
+        }
+        const uint64_t rem = uint64_t(d.L) - nb * 16;
+        for (uint64_t i = 0; i < rem; i++) {
+            double o[8];
+            rmsElement(d, nb * 16 + i, o);
+            st[idx] = o[q];
+            const int last = idx - __builtin_ctzll(i + 1);
+            while (idx > last) { st[idx - 1] += st[idx]; --idx; }
+            ++idx;
+        }
+        double s = 0.0;
+        if (idx > 0) {
+            s = st[idx - 1];
+            for (int j = idx - 2; j >= 0; j--) s += st[j];
+        }
+        tot[q] = 0.0 + s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double rms[4];
+    for (int v = 0; v < 4; v++) rms[v] = sqrt(double(d.N) * tot[v]) / tot[4 + v];
+    const double newDt = __longlong_as_double((long long)sc->newDtBits);
+    const double Rmax = __longlong_as_double((long long)sc->rmaxBits);
+    if (d.dynamicDt) sc->dt = newDt;
+    double f0 = 0, f1 = 0;
+    for (uint32_t k = 0; k < d.nForce; k++) {
+        const uint32_t f = d.forceFaces[k];
+        f0 += d.lenF[size_t(f) * 4 + 1];
+        f1 += d.lenF[size_t(f) * 4 + 2];
+    }
+    const double ld0 = 0.0 + d.rot[0] * f0 + d.rot[1] * f1;
+    const double ld1 = 0.0 + d.rot[2] * f0 + d.rot[3] * f1;
+    const double residMax = dmax(dmax(dmax(rms[0], rms[1]), rms[2]), rms[3]);
+    if (d.cflAdjust) {
+        if (sc->residLast < residMax) sc->incIters++;
+        else if (sc->incIters >= 1) sc->incIters--;
+        if (sc->incIters >= 2000) { sc->CFL *= 0.99; sc->incIters = 0; }
+    }
+    sc->residLast = residMax;
+    const int64_t slot = sc->steps - d.histBase;
+    if (slot >= 0 && slot < d.histCap) {
+        double* h = d.hist + slot * HIST_FIELDS;
+        h[0] = sc->t; h[1] = sc->dt; h[2] = rms[0]; h[3] = rms[1]; h[4] = rms[2]; h[5] = rms[3];
+        h[6] = Rmax; h[7] = ld0; h[8] = ld1; h[9] = sc->CFL;
+    }
+    for (int v = 0; v < 4; v++) sc->rms[v] = rms[v];
+    sc->force[0] = ld0; sc->force[1] = ld1;
+    sc->t += sc->dt;
+    sc->steps++;
+    if (!(sc->t < d.tEnd) || (d.maxSteps >= 0 && sc->steps >= d.maxSteps) ||
+        sc->errEdge != ~0ull || sc->errCell != ~0ull)
+        sc->active = 0;
+}
+
+__global__ void k_init_scalars(DevScalars* sc, double dt, double t, double CFL)
+{
+    sc->newDtBits = 0x7FF0000000000000ull;
+    sc->rmaxBits = 0;
+    sc->errEdge = ~0ull;
+    sc->errCell = ~0ull;
+    sc->dt = dt;
+    sc->t = t;
+    sc->CFL = CFL;
+    sc->residLast = INFINITY;
+    sc->incIters = 0;
+    sc->steps = 0;
+    sc->active = 1;
+    sc->pad = 0;
+    for (int v = 0; v < 4; v++) sc->rms[v] = 0;
+    sc->force[0] = sc->force[1] = 0;
+}
+
+// ---------------------------------------------------------------- launchers
+static uint32_t nblk(uint64_t n) { return uint32_t((n + BLK - 1) / BLK); }
+
+template <int S>
+static void gradS(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
+{
+    const uint32_t g = nblk(m.N);
+    if (g == 0) return;
+    int mode = p.order == 1 ? 0 : (!(p.limited && a.limit) ? 1 : (p.lpThresh > 0 ? 3 : 2));
+    // lim keeps the LP layout whenever the configuration uses LP
+    if (mode == 1 && p.lpThresh > 0 && p.limited) mode = 13;
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_gradient<S, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    case 1: hipLaunchKernelGGL((k_gradient<S, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    case 2: hipLaunchKernelGGL((k_gradient<S, 2>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    case 3: hipLaunchKernelGGL((k_gradient<S, 3>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    default: {
+        // unlimited stage of an LP configuration: write unit x,y limiter pairs
+        StageArgs b = a;
+        b.limit = 0;
+        hipLaunchKernelGGL((k_gradient<S, 3>), dim3(g), dim3(BLK), 0, st, m, p, b);
+    }
+    }
+}
+
+void launchGradient(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t st)
+{
+    if (S == 3) gradS<3>(m, p, a, st);
+    else if (S == 4) gradS<4>(m, p, a, st);
+    else gradS<6>(m, p, a, st);
+}
+
+template <int FLUX, int VISC, int ORDER, int LP>
+static void facesT(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
+{
+    const uint32_t nbi = nblk(m.Fi), nbb = nblk(m.Fb);
+    if (nbi + nbb == 0) return;
+    if (a.estate)
+        hipLaunchKernelGGL((k_faces<FLUX, VISC, ORDER, LP, 1>), dim3(nbi + nbb), dim3(BLK), 0, st, m, p, a, nbi);
+    else
+        hipLaunchKernelGGL((k_faces<FLUX, VISC, ORDER, LP, 0>), dim3(nbi + nbb), dim3(BLK), 0, st, m, p, a, nbi);
+}
+
+template <int FLUX, int VISC>
+static void facesO(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
+{
+    const bool lp = p.order > 1 && p.limited && p.lpThresh > 0;
+    if (p.order == 1) facesT<FLUX, VISC, 1, 0>(m, p, a, st);
+    else if (lp) facesT<FLUX, VISC, 2, 1>(m, p, a, st);
+    else facesT<FLUX, VISC, 2, 0>(m, p, a, st);
+}
+
+void launchFaces(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
+{
+    if (p.flux == 1) {
+        if (p.viscous) facesO<1, 1>(m, p, a, st); else facesO<1, 0>(m, p, a, st);
+    } else {
+        if (p.viscous) facesO<0, 1>(m, p, a, st); else facesO<0, 0>(m, p, a, st);
+    }
+}
+
+template <int S>
+static void updS(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
+{
+    const uint32_t g = nblk(m.N);
+    if (g == 0) return;
+#define UPD(K) case K: hipLaunchKernelGGL((k_update<S, K>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    switch (a.kind) {
+        UPD(SK_RESIDUAL) UPD(SK_EULER) UPD(SK_RK2_1) UPD(SK_RK2_2) UPD(SK_TVD_1) UPD(SK_TVD_2)
+        UPD(SK_RK4_1) UPD(SK_RK4_2) UPD(SK_RK4_3) UPD(SK_RK4_4)
+    }
+#undef UPD
+}
+
+void launchUpdate(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t st)
+{
+    if (S == 3) updS<3>(m, p, a, st);
+    else if (S == 4) updS<4>(m, p, a, st);
+    else updS<6>(m, p, a, st);
+}
+
+void launchRmsChunks(const DiagArgs& d, hipStream_t st)
+{
+    const uint64_t nb = uint64_t(d.L) / 16;
+    const uint32_t nc = uint32_t((nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS);
+    if (nc) hipLaunchKernelGGL(k_rms_chunks, dim3(nc), dim3(BLK), 0, st, d);
+}
+
+void launchFinalize(const DiagArgs& d, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, d);
+}
+
+void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, st, sc, dt, t, CFL);
+}
+
+}  // namespace ebb

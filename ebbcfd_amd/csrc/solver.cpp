@@ -1,0 +1,1158 @@
+// C ABI: mesh, config and the device-resident solver.
+//
+// Set-up follows source/solve.d:175-279 (ufvmSetup); the per-stage launch
+// sequence follows the integrators (source/integrators.d:22-270); ebb_run is
+// the driver loop of app/solver.d:119-250 with every per-step quantity (dt, t,
+// RMS residuals, forces, cflAdjust) kept on the device, so the host only syncs
+// once per batch of steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ebbcfd.h"
+#include "dev.hpp"
+#include "host.hpp"
+
+using namespace ebb;
+
+struct ebb_mesh {
+    RawMesh raw;
+    BuiltMesh b;
+};
+
+struct ebb_config {
+    Config c;
+};
+
+namespace {
+
+#define HIPCHK(x)                                                     \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) {                                       \
+            lastDeviceError = std::string(#x) + ": " + hipGetErrorString(e_); \
+            return EBB_ERR_DEVICE;                                    \
+        }                                                             \
+    } while (0)
+
+thread_local std::string lastDeviceError;
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+        n = count;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e == hipSuccess) e = hipMemset(p, 0, count * sizeof(T));
+        return e;
+    }
+    hipError_t upload(const std::vector<T>& v)
+    {
+        hipError_t e = alloc(v.size());
+        if (e == hipSuccess && !v.empty()) e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+void putBE32(std::vector<uint8_t>& b, uint32_t v)
+{
+    for (int i = 3; i >= 0; i--) b.push_back(uint8_t(v >> (8 * i)));
+}
+void putBE64(std::vector<uint8_t>& b, uint64_t v)
+{
+    for (int i = 7; i >= 0; i--) b.push_back(uint8_t(v >> (8 * i)));
+}
+void putBEd(std::vector<uint8_t>& b, double d)
+{
+    uint64_t v;
+    std::memcpy(&v, &d, 8);
+    putBE64(b, v);
+}
+uint32_t getBE32(const uint8_t* p) { return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | p[3]; }
+double getBEd(const uint8_t* p)
+{
+    uint64_t v = 0;
+    for (int i = 0; i < 8; i++) v = (v << 8) | p[i];
+    double d;
+    std::memcpy(&d, &v, 8);
+    return d;
+}
+
+}  // namespace
+
+namespace ebb {
+uint32_t crc32(const uint8_t* p, size_t n)
+{
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[i] = c;
+        }
+        init = true;
+    }
+    uint32_t c = 0xFFFFFFFFu;
+    for (size_t i = 0; i < n; i++) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
+    return c ^ 0xFFFFFFFFu;
+}
+}  // namespace ebb
+
+struct ebb_solver {
+    // host copies
+    RawMesh raw;
+    BuiltMesh b;
+    Config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t N = 0, G = 0, Fi = 0, Fb = 0, S = 3;
+    std::vector<uint32_t> faceOfEdge;   // reference edge -> device face
+    std::vector<uint32_t> edgeOfFace;   // device face -> reference edge
+    std::vector<BFace> bfaces;
+    PhysParams phys{};
+    DevMesh dm{};
+    // device buffers
+    DBuf<uint32_t> d_nbr, d_cface, d_refId, d_forceFaces;
+    DBuf<double> d_gm, d_nia, d_cd, d_cper, d_flen, d_grad, d_lim, d_lenF, d_lenS, d_celldt, d_hist, d_chunkTrees;
+    DBuf<double2> d_cen, d_fn, d_fmid;
+    DBuf<uint2> d_fcell;
+    DBuf<BFace> d_bface;
+    DBuf<double4> d_src, d_qA, d_qB, d_T, d_T2, d_acc, d_R, d_estate;
+    DBuf<uint32_t> d_chunkMask;
+    DevScalars* d_sc = nullptr;
+    DevScalars* h_sc = nullptr;  // pinned
+    double* h_hist = nullptr;    // pinned
+    int64_t histCap = 256;
+    // time-loop bookkeeping
+    int cur = 0;                 // 0: qA holds the current state, 1: qB
+    double4* lastInput = nullptr;
+    int lastLimit = 1;
+    bool haveStage = false;
+    uint32_t stageCounter = 0;
+    int64_t stepsTotal = 0;
+    double t0 = 0, dt0 = 0;
+    std::vector<double> history;     // HIST_FIELDS per completed step of ebb_run
+    // errors
+    int errKind = 0;
+    int64_t errIndex = -1, errStep = -1;
+    std::string errMsg;
+
+    double4* qcur() { return cur == 0 ? d_qA.p : d_qB.p; }
+    double4* qnext() { return cur == 0 ? d_qB.p : d_qA.p; }
+    int64_t deviceBytes = 0;
+};
+
+namespace {
+
+int fail(ebb_solver* s, int code, const std::string& msg)
+{
+    if (s) { s->errKind = code; s->errMsg = msg; }
+    return code;
+}
+
+template <typename T>
+hipError_t trackAlloc(ebb_solver* s, DBuf<T>& b, size_t n)
+{
+    hipError_t e = b.alloc(n);
+    s->deviceBytes += int64_t((n ? n : 1) * sizeof(T));
+    return e;
+}
+template <typename T>
+hipError_t trackUpload(ebb_solver* s, DBuf<T>& b, const std::vector<T>& v)
+{
+    hipError_t e = b.upload(v);
+    s->deviceBytes += int64_t((v.size() ? v.size() : 1) * sizeof(T));
+    return e;
+}
+
+// one residual evaluation + stage update
+void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qout, int limit, int dtUpdate,
+           int first, int last, int checkActive, double4* R, double4* estate, uint32_t stepId)
+{
+    StageArgs a{};
+    a.qin = qin; a.qn = qn; a.qout = qout; a.acc = s->d_acc.p; a.R = R;
+    a.grad = s->d_grad.p; a.lim = s->d_lim.p; a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p;
+    a.estate = estate; a.celldt = s->d_celldt.p; a.sc = s->d_sc;
+    a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
+    a.checkActive = checkActive; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = stepId;
+    launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+    launchFaces(s->dm, s->phys, a, s->stream);
+    launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
+    s->lastInput = qin;
+    s->lastLimit = limit;
+    s->haveStage = true;
+}
+
+// Integrator.step: all stages of one step from qcur() into qnext()
+void stepStages(ebb_solver* s, int checkActive, uint32_t stepId)
+{
+    const Config& c = s->cfg;
+    const int msl = c.multistageLimiting ? 1 : 0, lts = c.localTimestep ? 1 : 0;
+    double4 *q = s->qcur(), *nx = s->qnext(), *T = s->d_T.p, *T2 = s->d_T2.p;
+    switch (c.integratorKind()) {
+    case IntEuler:
+        stage(s, SK_EULER, q, q, nx, 1, 1, 1, 1, checkActive, nullptr, nullptr, stepId);
+        break;
+    case IntRK2:
+        stage(s, SK_RK2_1, q, q, T, 1, 1, 1, 0, checkActive, nullptr, nullptr, stepId);
+        stage(s, SK_RK2_2, T, q, nx, msl, 0, 0, 1, checkActive, nullptr, nullptr, stepId);
+        break;
+    case IntRK2TVD:
+        stage(s, SK_TVD_1, q, q, T, 1, lts, 1, 0, checkActive, nullptr, nullptr, stepId);
+        stage(s, SK_TVD_2, T, q, nx, 0, lts ? 0 : 1, 0, 1, checkActive, nullptr, nullptr, stepId);
+        break;
+    case IntRK4:
+        stage(s, SK_RK4_1, q, q, T, 1, 1, 1, 0, checkActive, nullptr, nullptr, stepId);
+        stage(s, SK_RK4_2, T, q, T2, msl, 0, 0, 0, checkActive, nullptr, nullptr, stepId);
+        stage(s, SK_RK4_3, T2, q, T, msl, 0, 0, 0, checkActive, nullptr, nullptr, stepId);
+        stage(s, SK_RK4_4, T, q, nx, msl, 0, 0, 1, checkActive, nullptr, nullptr, stepId);
+        break;
+    }
+}
+
+int stagesPerStep(const Config& c)
+{
+    switch (c.integratorKind()) {
+    case IntEuler: return 1;
+    case IntRK4: return 4;
+    default: return 2;
+    }
+}
+
+DiagArgs diagArgs(ebb_solver* s, const double4* qnew, const double4* qold, int64_t histBase, int64_t maxSteps)
+{
+    DiagArgs d{};
+    d.qnew = qnew; d.qold = qold; d.N = s->N; d.L = s->N + s->G; d.perm = nullptr;
+    d.chunkTrees = s->d_chunkTrees.p; d.chunkMask = s->d_chunkMask.p;
+    d.forceFaces = s->d_forceFaces.p; d.nForce = uint32_t(s->d_forceFaces.n);
+    d.lenF = s->d_lenF.p;
+    const long double a = (long double)s->cfg.ic[1] * (3.14159265358979323846264338327950288L / 180.0L);
+    d.rot[0] = double(cosl(a)); d.rot[1] = -double(sinl(a)); d.rot[2] = double(sinl(a)); d.rot[3] = double(cosl(a));
+    d.tEnd = s->cfg.tEnd;
+    d.dynamicDt = s->cfg.dynamicDt; d.cflAdjust = s->cfg.cflAdjust;
+    d.hist = s->d_hist.p; d.histBase = histBase; d.histCap = s->histCap;
+    d.maxSteps = maxSteps;
+    d.sc = s->d_sc;
+    return d;
+}
+
+int readScalars(ebb_solver* s)
+{
+    HIPCHK(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+int checkErrors(ebb_solver* s)
+{
+    const DevScalars& h = *s->h_sc;
+    if (h.errEdge != ~0ull) {
+        const uint64_t key = h.errEdge & 0xFFFFFFFFull;
+        int64_t edge = key < s->Fb ? int64_t(s->b.boundaryEdges[key]) : int64_t(s->b.interiorEdges[key - s->Fb]);
+        s->errKind = EBB_ERR_EDGE_NAN;
+        s->errIndex = edge;
+        s->errMsg = key < s->Fb ? "Got NaN on boundary edge" : "Got NaN on interior edge";
+        return EBB_ERR_EDGE_NAN;
+    }
+    if (h.errCell != ~0ull) {
+        s->errKind = EBB_ERR_CELL_NAN;
+        s->errIndex = int64_t(h.errCell & 0xFFFFFFFFull);
+        s->errStep = int64_t(h.errCell >> 32);
+        s->errMsg = "Got nan on cell average value";
+        return EBB_ERR_CELL_NAN;
+    }
+    return EBB_OK;
+}
+
+int clearErrors(ebb_solver* s)
+{
+    // keep t/dt/CFL/counters, clear the error words and reactivate
+    HIPCHK(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->h_sc->errEdge = ~0ull;
+    s->h_sc->errCell = ~0ull;
+    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+int buildDevice(ebb_solver* s)
+{
+    const BuiltMesh& b = s->b;
+    const Config& c = s->cfg;
+    s->N = b.nElem;
+    s->G = b.nCells - b.nElem;
+    s->Fi = uint32_t(b.interiorEdges.size());
+    s->Fb = uint32_t(b.boundaryEdges.size());
+    uint32_t maxE = 3;
+    for (uint32_t i = 0; i < s->N; i++) maxE = std::max<uint32_t>(maxE, b.nEdgesCell[i]);
+    s->S = maxE <= 3 ? 3 : (maxE == 4 ? 4 : 6);
+    const uint32_t S = s->S, N = s->N, F = s->Fi + s->Fb;
+    s->faceOfEdge.assign(b.nEdges, 0);
+    s->edgeOfFace.assign(F, 0);
+    for (uint32_t k = 0; k < s->Fi; k++) { s->faceOfEdge[b.interiorEdges[k]] = k; s->edgeOfFace[k] = b.interiorEdges[k]; }
+    for (uint32_t k = 0; k < s->Fb; k++) { s->faceOfEdge[b.boundaryEdges[k]] = s->Fi + k; s->edgeOfFace[s->Fi + k] = b.boundaryEdges[k]; }
+
+    std::vector<uint32_t> nbr(size_t(N) * S), cface(size_t(N) * S), refId(N);
+    std::vector<double> gm(size_t(N) * 2 * S), nia(N), cd(N), cper(N);
+    for (uint32_t i = 0; i < N; i++) {
+        const uint32_t ne = b.nEdgesCell[i];
+        for (uint32_t j = 0; j < S; j++) {
+            if (j < ne) {
+                nbr[size_t(i) * S + j] = b.nbr[size_t(i) * MAXE + j];
+                const uint32_t f = s->faceOfEdge[b.cellEdges[size_t(i) * MAXE + j]];
+                cface[size_t(i) * S + j] = f | (b.fluxMult[size_t(i) * MAXE + j] < 0 ? 0x80000000u : 0u);
+            } else {
+                nbr[size_t(i) * S + j] = i;   // self: du = 0, dx = dy = 0
+                cface[size_t(i) * S + j] = F; // all-zero padding face
+            }
+            for (int r = 0; r < 2; r++)
+                gm[size_t(i) * 2 * S + r * S + j] = j < uint32_t(MAXE) ? b.gradMat[size_t(i) * 2 * MAXE + r * MAXE + j] : 0.0;
+        }
+        nia[i] = -(1.0 / b.area[i]);
+        cd[i] = b.d[i];
+        cper[i] = b.perim[i];
+        refId[i] = i;
+    }
+    std::vector<double2> cen(b.nCells), fn(F), fmid(F);
+    std::vector<uint2> fcell(F);
+    std::vector<double> flen(F);
+    for (uint32_t k = 0; k < b.nCells; k++) cen[k] = make_double2(b.cx[k], b.cy[k]);
+    for (uint32_t f = 0; f < F; f++) {
+        const uint32_t e = s->edgeOfFace[f];
+        fcell[f] = make_uint2(b.eCell0[e], b.eCell1[e]);
+        fn[f] = make_double2(b.nx[e], b.ny[e]);
+        fmid[f] = make_double2(b.mx[e], b.my[e]);
+        flen[f] = b.len[e];
+    }
+    // boundary binding (solve.d:222-278)
+    s->bfaces.assign(s->Fb, BFace{});
+    std::vector<int> bcOfTag(s->raw.btags.size(), -1);
+    for (size_t t = 0; t < s->raw.btags.size(); t++) {
+        for (size_t k = 0; k < c.bcs.size(); k++)
+            if (c.bcs[k].tag == s->raw.btags[t]) { bcOfTag[t] = int(k); break; }
+        if (bcOfTag[t] < 0 && !b.bGroups[t].empty()) {
+            return fail(s, EBB_ERR_BC, "Could not find matching boundary condition tag " + s->raw.btags[t]);
+        }
+    }
+    std::vector<uint32_t> posOfBoundary(b.nEdges, 0);
+    for (uint32_t k = 0; k < s->Fb; k++) posOfBoundary[b.boundaryEdges[k]] = k;
+    const double* nd = s->raw.nodes.data();
+    for (uint32_t k = 0; k < s->Fb; k++) {
+        const uint32_t e = b.boundaryEdges[k];
+        BFace& bf = s->bfaces[k];
+        const int32_t g = b.eBGroup[e];
+        const BCConfig& bc = c.bcs[bcOfTag[g]];
+        bf.type = bc.type;
+        bf.nrm[0] = b.nx[e];
+        bf.nrm[1] = b.ny[e];
+        bf.Rgas = c.R;
+        const double x1 = nd[2 * b.eNode0[e]], y1 = nd[2 * b.eNode0[e] + 1];
+        const double x2 = nd[2 * b.eNode1[e]], y2 = nd[2 * b.eNode1[e] + 1];
+        bf.vertical = (x1 != x2) ? 0 : 1;
+        if (x1 != x2) {
+            const double m = (y2 - y1) / (x2 - x1);
+            const double den = 1 + m * m;
+            bf.R[0] = (1 - m * m) * 1.0 / den;
+            bf.R[1] = (2.0 * m) * 1.0 / den;
+            bf.R[2] = (2.0 * m) * 1.0 / den;
+            bf.R[3] = (m * m - 1) * 1.0 / den;
+        }
+        if (bc.type == FullState) {
+            if (bc.data.size() < 4) return fail(s, EBB_ERR_BC, "fullState boundary data needs 4 values");
+            freestreamState(bc.data[0], bc.data[1], bc.data[3], bf.far);
+        } else if (bc.type == ConstPressure) {
+            if (bc.data.size() != 1) return fail(s, EBB_ERR_BC, "Constant pressure boundary data should only have one element; the constant pressure");
+            bf.data[0] = bc.data[0];
+        } else if (bc.type == TempPresInflow) {
+            if (bc.data.size() != 3) return fail(s, EBB_ERR_BC, "Constant temperature/pressure/aoa boundary data should have three elements");
+            const double aoa = aoaRadians(bc.data[2]);
+            bf.data[0] = bc.data[0];
+            bf.data[1] = bc.data[1];
+            bf.data[2] = double(cosl((long double)aoa));
+            bf.data[3] = double(sinl((long double)aoa));
+        } else if (bc.type == Dirichlet) {
+            if (!c.mms) return fail(s, EBB_ERR_BC, "dirichlet boundary needs the manufactured solution (extensions.mms)");
+            mmsSolution(b.mx[e], b.my[e], bf.far);
+            const uint32_t gc = b.eCell1[e];
+            mmsSolution(b.cx[gc], b.cy[gc], bf.gq);
+            mmsSolutionGradient(b.cx[gc], b.cy[gc], bf.gg);
+        }
+    }
+    // physics parameters
+    PhysParams& p = s->phys;
+    p.flux = c.fluxKind();
+    p.viscous = c.viscosity ? 1 : 0;
+    p.order = int(c.order);
+    p.limited = c.limited ? 1 : 0;
+    p.Pr = c.Pr;
+    // the reference sets mu only for viscous runs (solve.d:186-190); otherwise it keeps D's NaN default
+    p.mu = c.viscosity ? (c.ic[3] * 1.0 * c.L) / c.Re : std::nan("");
+    s->cfg.mu = p.mu;
+    p.Rgas = c.R;
+    p.lpThresh = c.lpThresh;
+    p.localTimestep = c.localTimestep ? 1 : 0;
+    p.mixed = c.precision;
+    // MMS source at centroids
+    std::vector<double4> src;
+    if (c.mms) {
+        src.resize(N);
+        for (uint32_t i = 0; i < N; i++) {
+            double S4[4];
+            mmsSourceTerm(b.cx[i], b.cy[i], c.viscosity, c.R, p.mu, c.Pr, S4);
+            src[i] = make_double4(S4[0], S4[1], S4[2], S4[3]);
+        }
+    }
+    // force faces in bGroups order of the forceBoundary tags (mesh.d:645-671)
+    std::vector<uint32_t> ff;
+    for (auto& tag : c.forceBoundary) {
+        auto it = std::find(s->raw.btags.begin(), s->raw.btags.end(), tag);
+        if (it == s->raw.btags.end()) continue;
+        for (uint32_t e : b.bGroups[size_t(it - s->raw.btags.begin())]) ff.push_back(s->faceOfEdge[e]);
+    }
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPCHK(trackUpload(s, s->d_nbr, nbr));
+    HIPCHK(trackUpload(s, s->d_cface, cface));
+    HIPCHK(trackUpload(s, s->d_refId, refId));
+    HIPCHK(trackUpload(s, s->d_gm, gm));
+    HIPCHK(trackUpload(s, s->d_nia, nia));
+    HIPCHK(trackUpload(s, s->d_cd, cd));
+    HIPCHK(trackUpload(s, s->d_cper, cper));
+    HIPCHK(trackUpload(s, s->d_cen, cen));
+    HIPCHK(trackUpload(s, s->d_fcell, fcell));
+    HIPCHK(trackUpload(s, s->d_fn, fn));
+    HIPCHK(trackUpload(s, s->d_fmid, fmid));
+    HIPCHK(trackUpload(s, s->d_flen, flen));
+    HIPCHK(trackUpload(s, s->d_bface, s->bfaces));
+    HIPCHK(trackUpload(s, s->d_forceFaces, ff));
+    s->d_forceFaces.n = ff.size();
+    if (c.mms) HIPCHK(trackUpload(s, s->d_src, src));
+    const size_t C = b.nCells;
+    HIPCHK((hipError_t)trackAlloc(s, s->d_qA, C));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_qB, C));
+    const int nst = stagesPerStep(c);
+    if (nst > 1) HIPCHK((hipError_t)trackAlloc(s, s->d_T, C));
+    if (c.integratorKind() == IntRK4) HIPCHK((hipError_t)trackAlloc(s, s->d_T2, C));
+    if (c.integratorKind() == IntRK2 || c.integratorKind() == IntRK4) HIPCHK((hipError_t)trackAlloc(s, s->d_acc, N));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_grad, size_t(N) * 8));
+    const bool lp = c.order > 1 && c.limited && c.lpThresh > 0;
+    HIPCHK((hipError_t)trackAlloc(s, s->d_lim, size_t(N) * (lp ? 8 : 4)));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_lenF, size_t(F + 1) * 4));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_lenS, size_t(F + 1)));
+    if (c.localTimestep) HIPCHK((hipError_t)trackAlloc(s, s->d_celldt, N));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_hist, size_t(s->histCap) * HIST_FIELDS));
+    const uint64_t nb = uint64_t(C) / 16;
+    const uint64_t nchunks = (nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
+    HIPCHK((hipError_t)trackAlloc(s, s->d_chunkTrees, size_t(nchunks) * 128));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_chunkMask, size_t(nchunks) * 8));
+    HIPCHK(hipMalloc((void**)&s->d_sc, sizeof(DevScalars)));
+    HIPCHK(hipHostMalloc((void**)&s->h_sc, sizeof(DevScalars), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&s->h_hist, sizeof(double) * size_t(s->histCap) * HIST_FIELDS, hipHostMallocDefault));
+
+    DevMesh& dm = s->dm;
+    dm.N = N; dm.G = s->G; dm.Fi = s->Fi; dm.Fb = s->Fb;
+    dm.nbr = s->d_nbr.p; dm.cface = s->d_cface.p; dm.gm = s->d_gm.p; dm.cen = s->d_cen.p;
+    dm.nia = s->d_nia.p; dm.cd = s->d_cd.p; dm.cper = s->d_cper.p;
+    dm.fcell = s->d_fcell.p; dm.fn = s->d_fn.p; dm.fmid = s->d_fmid.p; dm.flen = s->d_flen.p;
+    dm.bface = s->d_bface.p; dm.src = c.mms ? s->d_src.p : nullptr; dm.refId = s->d_refId.p;
+
+    // initial state (solve.d:194-207; app/ms.d:295-298)
+    std::vector<double4> q0(C, make_double4(0, 0, 0, 0));
+    double fs[4];
+    freestreamState(c.ic[0], c.ic[1], c.ic[3], fs);
+    for (uint32_t i = 0; i < N; i++) {
+        if (c.mms) {
+            double qq[4];
+            mmsSolution(b.cx[i], b.cy[i], qq);
+            q0[i] = make_double4(qq[0], qq[1], qq[2], qq[3]);
+        } else {
+            q0[i] = make_double4(fs[0], fs[1], fs[2], fs[3]);
+        }
+    }
+    HIPCHK(hipMemcpy(s->d_qA.p, q0.data(), C * sizeof(double4), hipMemcpyHostToDevice));
+    s->cur = 0;
+    s->t0 = 0;
+    s->dt0 = c.dt;
+    launchInitScalars(s->d_sc, c.dt, 0.0, c.CFL, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+const char* ebb_error_string(int code)
+{
+    switch (code) {
+    case EBB_OK: return "ok";
+    case EBB_ERR_ARG: return "bad argument";
+    case EBB_ERR_IO: return "i/o error";
+    case EBB_ERR_PARSE: return "parse error";
+    case EBB_ERR_MESH: return "mesh error";
+    case EBB_ERR_BC: return "boundary condition error";
+    case EBB_ERR_DEVICE: return "device error";
+    case EBB_ERR_EDGE_NAN: return "NaN flux on edge";
+    case EBB_ERR_CELL_NAN: return "NaN state in cell";
+    case EBB_ERR_STATE: return "invalid solver state";
+    }
+    return "unknown";
+}
+
+int ebb_mesh_load(const char* path, ebb_mesh** out)
+{
+    if (!path || !out) return EBB_ERR_ARG;
+    std::string p(path);
+    if (p.find(".msh") == std::string::npos && p.find(".gmsh") == std::string::npos) return EBB_ERR_PARSE;
+    auto m = std::make_unique<ebb_mesh>();
+    std::string err;
+    {
+        std::ifstream f(p);
+        if (!f) return EBB_ERR_IO;
+    }
+    if (!parseGmsh(p, m->raw, err)) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_PARSE; }
+    if (!buildMesh(m->raw, m->b, err)) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_MESH; }
+    *out = m.release();
+    return EBB_OK;
+}
+
+int ebb_mesh_create(uint32_t n_nodes, const double* xy, uint32_t n_elem, const uint32_t* eoff, const uint32_t* enodes,
+                    uint32_t n_bnodes, const uint32_t* bnodes, uint32_t n_tags, const char* const* tags,
+                    const uint64_t* bgroup_start, ebb_mesh** out)
+{
+    if (!out || !xy || !eoff || !enodes || (n_bnodes && !bnodes) || (n_tags && (!tags || !bgroup_start))) return EBB_ERR_ARG;
+    auto m = std::make_unique<ebb_mesh>();
+    m->raw.nodes.assign(xy, xy + 2 * size_t(n_nodes));
+    m->raw.eoff.assign(eoff, eoff + n_elem + 1);
+    m->raw.enodes.assign(enodes, enodes + eoff[n_elem]);
+    m->raw.bnodes.assign(bnodes, bnodes + 2 * size_t(n_bnodes));
+    for (uint32_t t = 0; t < n_tags; t++) m->raw.btags.emplace_back(tags[t]);
+    m->raw.bgroupStart.assign(bgroup_start, bgroup_start + n_tags);
+    std::string err;
+    if (!buildMesh(m->raw, m->b, err)) { std::fprintf(stderr, "ebb_mesh_create: %s\n", err.c_str()); return EBB_ERR_MESH; }
+    *out = m.release();
+    return EBB_OK;
+}
+
+int ebb_mesh_info(const ebb_mesh* m, int64_t info[8])
+{
+    if (!m || !info) return EBB_ERR_ARG;
+    const BuiltMesh& b = m->b;
+    uint32_t maxE = 0;
+    for (auto v : b.nEdgesCell) maxE = std::max<uint32_t>(maxE, v);
+    info[0] = b.nCells; info[1] = b.nElem; info[2] = b.nCells - b.nElem; info[3] = b.nEdges;
+    info[4] = int64_t(b.interiorEdges.size()); info[5] = int64_t(b.boundaryEdges.size());
+    info[6] = maxE; info[7] = int64_t(m->raw.btags.size());
+    return EBB_OK;
+}
+
+int ebb_mesh_cell_geometry(const ebb_mesh* m, double* area, double* d, double* perim, double* centroid,
+                           double* grad_mat, uint32_t* edges, int8_t* flux_mult, uint32_t* neighbors, uint8_t* n_edges)
+{
+    if (!m) return EBB_ERR_ARG;
+    const BuiltMesh& b = m->b;
+    for (uint32_t i = 0; i < b.nCells; i++) {
+        if (area) area[i] = b.area[i];
+        if (d) d[i] = b.d[i];
+        if (perim) perim[i] = b.perim[i];
+        if (centroid) { centroid[2 * i] = b.cx[i]; centroid[2 * i + 1] = b.cy[i]; }
+    }
+    if (grad_mat) std::memcpy(grad_mat, b.gradMat.data(), b.gradMat.size() * sizeof(double));
+    if (edges) std::memcpy(edges, b.cellEdges.data(), b.cellEdges.size() * sizeof(uint32_t));
+    if (flux_mult) std::memcpy(flux_mult, b.fluxMult.data(), b.fluxMult.size());
+    if (neighbors) std::memcpy(neighbors, b.nbr.data(), b.nbr.size() * sizeof(uint32_t));
+    if (n_edges) std::memcpy(n_edges, b.nEdgesCell.data(), b.nEdgesCell.size());
+    return EBB_OK;
+}
+
+int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, double* len, double* normal,
+                           double* mid, int32_t* bgroup)
+{
+    if (!m) return EBB_ERR_ARG;
+    const BuiltMesh& b = m->b;
+    for (uint32_t e = 0; e < b.nEdges; e++) {
+        if (nodes) { nodes[2 * e] = b.eNode0[e]; nodes[2 * e + 1] = b.eNode1[e]; }
+        if (cells) { cells[2 * e] = b.eCell0[e]; cells[2 * e + 1] = b.eCell1[e]; }
+        if (len) len[e] = b.len[e];
+        if (normal) { normal[2 * e] = b.nx[e]; normal[2 * e + 1] = b.ny[e]; }
+        if (mid) { mid[2 * e] = b.mx[e]; mid[2 * e + 1] = b.my[e]; }
+        if (bgroup) bgroup[e] = b.eBGroup[e];
+    }
+    return EBB_OK;
+}
+
+void ebb_mesh_destroy(ebb_mesh* m) { delete m; }
+
+int ebb_config_parse(const char* json, ebb_config** out)
+{
+    if (!json || !out) return EBB_ERR_ARG;
+    auto c = std::make_unique<ebb_config>();
+    std::string err;
+    if (!parseConfig(json, c->c, err)) { std::fprintf(stderr, "ebb_config_parse: %s\n", err.c_str()); return EBB_ERR_PARSE; }
+    *out = c.release();
+    return EBB_OK;
+}
+
+static double* cfgField(Config& c, const std::string& k, bool& isInt, long long*& ip, bool*& bp)
+{
+    isInt = false; ip = nullptr; bp = nullptr;
+    if (k == "dt") return &c.dt;
+    if (k == "tEnd") return &c.tEnd;
+    if (k == "CFL") return &c.CFL;
+    if (k == "lpThresh") return &c.lpThresh;
+    if (k == "Pr") return &c.Pr;
+    if (k == "R") return &c.R;
+    if (k == "Re") return &c.Re;
+    if (k == "L") return &c.L;
+    if (k.size() == 3 && k[0] == 'i' && k[1] == 'c' && k[2] >= '0' && k[2] <= '3') return &c.ic[k[2] - '0'];
+    isInt = true;
+    if (k == "saveIter") ip = &c.saveIter;
+    else if (k == "plotIter") ip = &c.plotIter;
+    else if (k == "order") ip = &c.order;
+    else if (k == "maxSteps") ip = &c.maxSteps;
+    else if (k == "limited") bp = &c.limited;
+    else if (k == "dynamicDt") bp = &c.dynamicDt;
+    else if (k == "localTimestep") bp = &c.localTimestep;
+    else if (k == "multistageLimiting") bp = &c.multistageLimiting;
+    else if (k == "cflAdjust") bp = &c.cflAdjust;
+    else if (k == "viscosity") bp = &c.viscosity;
+    else if (k == "mms") bp = &c.mms;
+    return nullptr;
+}
+
+int ebb_config_get(const ebb_config* c, const char* key, double* value)
+{
+    if (!c || !key || !value) return EBB_ERR_ARG;
+    bool isInt; long long* ip; bool* bp;
+    double* dp = cfgField(const_cast<Config&>(c->c), key, isInt, ip, bp);
+    if (dp) *value = *dp;
+    else if (ip) *value = double(*ip);
+    else if (bp) *value = *bp ? 1.0 : 0.0;
+    else if (std::string(key) == "precision") *value = c->c.precision;
+    else if (std::string(key) == "integrator") *value = c->c.integratorKind();
+    else if (std::string(key) == "flux") *value = c->c.fluxKind();
+    else return EBB_ERR_ARG;
+    return EBB_OK;
+}
+
+int ebb_config_set(ebb_config* c, const char* key, double value)
+{
+    if (!c || !key) return EBB_ERR_ARG;
+    bool isInt; long long* ip; bool* bp;
+    double* dp = cfgField(c->c, key, isInt, ip, bp);
+    if (dp) *dp = value;
+    else if (ip) *ip = (long long)value;
+    else if (bp) *bp = value != 0.0;
+    else if (std::string(key) == "precision") c->c.precision = int(value);
+    else return EBB_ERR_ARG;
+    return EBB_OK;
+}
+
+int ebb_config_mesh_file(const ebb_config* c, char* buf, size_t len)
+{
+    if (!c || !buf || len == 0) return EBB_ERR_ARG;
+    std::snprintf(buf, len, "%s", c->c.meshFile.c_str());
+    return c->c.meshFile.size() < len ? EBB_OK : EBB_ERR_ARG;
+}
+
+void ebb_config_destroy(ebb_config* c) { delete c; }
+
+void ebb_solver_destroy(ebb_solver* s);
+
+int ebb_solver_create(const ebb_mesh* m, const ebb_config* c, int device, ebb_solver** out)
+{
+    if (!m || !c || !out) return EBB_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        std::fprintf(stderr, "ebb_solver_create: no HIP device %d\n", device);
+        return EBB_ERR_DEVICE;
+    }
+    auto s = new ebb_solver();
+    s->raw = m->raw;
+    s->b = m->b;
+    s->cfg = c->c;
+    s->device = device;
+    if (s->cfg.precision != 0) {
+        delete s;
+        std::fprintf(stderr, "ebb_solver_create: mixed precision is not available in this library build\n");
+        return EBB_ERR_ARG;
+    }
+    int rc = buildDevice(s);
+    if (rc != EBB_OK) {
+        std::fprintf(stderr, "ebb_solver_create: %s %s\n", s->errMsg.c_str(), lastDeviceError.c_str());
+        ebb_solver_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return EBB_OK;
+}
+
+int ebb_state_upload(ebb_solver* s, const double* q)
+{
+    if (!s || !q) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpyAsync(s->qcur(), q, size_t(s->N) * sizeof(double4), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+int ebb_state_download(ebb_solver* s, double* q)
+{
+    if (!s || !q) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpyAsync(q, s->qcur(), size_t(s->N) * sizeof(double4), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+int ebb_state_download_all(ebb_solver* s, double* q)
+{
+    if (!s || !q) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpyAsync(q, s->qcur(), size_t(s->N + s->G) * sizeof(double4), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+int ebb_residual(ebb_solver* s, int limit, int dt_update, double* R, double* new_dt, double* rmax)
+{
+    if (!s) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    if (!s->d_R.p) HIPCHK((hipError_t)trackAlloc(s, s->d_R, s->N));
+    if (!s->d_estate.p) HIPCHK((hipError_t)trackAlloc(s, s->d_estate, size_t(s->Fi + s->Fb) * 2));
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    s->h_sc->rmaxBits = 0;
+    if (rmax && *rmax > 0) std::memcpy(&s->h_sc->rmaxBits, rmax, 8);
+    s->h_sc->errEdge = ~0ull;
+    s->h_sc->errCell = ~0ull;
+    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+    stage(s, SK_RESIDUAL, s->qcur(), s->qcur(), nullptr, limit, dt_update, 0, 0, 0, s->d_R.p, s->d_estate.p, 0);
+    HIPCHK(hipGetLastError());
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    if (R) HIPCHK(hipMemcpy(R, s->d_R.p, size_t(s->N) * sizeof(double4), hipMemcpyDeviceToHost));
+    double nd, rm;
+    std::memcpy(&nd, &s->h_sc->newDtBits, 8);
+    std::memcpy(&rm, &s->h_sc->rmaxBits, 8);
+    if (new_dt) *new_dt = nd;
+    if (rmax) *rmax = rm;
+    return checkErrors(s);
+}
+
+int ebb_step(ebb_solver* s, double* dt, double* rmax)
+{
+    if (!s || !dt) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    DevScalars saved = *s->h_sc;
+    s->h_sc->dt = *dt;
+    s->h_sc->errEdge = ~0ull;
+    s->h_sc->errCell = ~0ull;
+    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+    stepStages(s, 0, uint32_t(s->stepsTotal));
+    HIPCHK(hipGetLastError());
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    s->cur ^= 1;
+    double nd, rm;
+    std::memcpy(&nd, &s->h_sc->newDtBits, 8);
+    std::memcpy(&rm, &s->h_sc->rmaxBits, 8);
+    *dt = nd;
+    if (rmax) *rmax = rm;
+    int rc = checkErrors(s);
+    // the run loop's time/dt are not advanced by a bare integrator step
+    s->h_sc->dt = saved.dt;
+    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return rc;
+}
+
+int ebb_run(ebb_solver* s, int64_t max_steps, double* t, double* dt, int64_t* steps_done)
+{
+    if (!s) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    int64_t done = 0;
+    int rc = EBB_OK;
+    if (s->h_sc->errEdge != ~0ull || s->h_sc->errCell != ~0ull) return fail(s, EBB_ERR_STATE, "solver stopped on an error");
+    if (!(s->h_sc->t < s->cfg.tEnd)) goto out;
+    {
+        // (re)activate: the device stops itself at tEnd, after maxSteps or on an error
+        const int64_t cfgMax = s->cfg.maxSteps;
+        int64_t limit = max_steps;
+        if (cfgMax >= 0) {
+            const int64_t left = std::max<int64_t>(0, cfgMax - s->stepsTotal);
+            limit = (limit < 0) ? left : std::min(limit, left);
+        }
+        if (limit == 0) goto out;
+        s->h_sc->active = 1;
+        HIPCHK(hipMemcpyAsync(&s->d_sc->active, &s->h_sc->active, sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+        while (limit < 0 || done < limit) {
+            const int64_t batch = limit < 0 ? s->histCap : std::min<int64_t>(s->histCap, limit - done);
+            const int64_t base = s->h_sc->steps;
+            const int64_t stopAt = base + batch;
+            int c0 = s->cur;
+            for (int64_t k = 0; k < batch; k++) {
+                const double4* qold = s->qcur();
+                stepStages(s, 1, uint32_t(base + k));
+                const double4* qnew = s->qnext();
+                DiagArgs d = diagArgs(s, qnew, qold, base, stopAt);
+                launchRmsChunks(d, s->stream);
+                launchFinalize(d, s->stream);
+                s->cur ^= 1;
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(s->h_hist, s->d_hist.p, sizeof(double) * size_t(batch) * HIST_FIELDS,
+                                  hipMemcpyDeviceToHost, s->stream));
+            { int rc_ = readScalars(s); if (rc_) return rc_; }
+            const int64_t ran = s->h_sc->steps - base;
+            s->cur = (c0 ^ int(ran & 1));
+            s->history.insert(s->history.end(), s->h_hist, s->h_hist + ran * HIST_FIELDS);
+            s->stepsTotal += ran;
+            done += ran;
+            rc = checkErrors(s);
+            if (rc != EBB_OK) break;
+            if (ran < batch) break;              // t >= tEnd
+            if (!(s->h_sc->t < s->cfg.tEnd)) break;
+            s->h_sc->active = 1;
+            HIPCHK(hipMemcpyAsync(&s->d_sc->active, &s->h_sc->active, sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+        }
+    }
+out:
+    if (t) *t = s->h_sc->t;
+    if (dt) *dt = s->h_sc->dt;
+    if (steps_done) *steps_done = done;
+    return rc;
+}
+
+int ebb_run_state(ebb_solver* s, double* t, double* dt, double* cfl, int64_t* steps)
+{
+    if (!s) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    if (t) *t = s->h_sc->t;
+    if (dt) *dt = s->h_sc->dt;
+    if (cfl) *cfl = s->h_sc->CFL;
+    if (steps) *steps = s->stepsTotal;
+    return EBB_OK;
+}
+
+int ebb_run_reset(ebb_solver* s, double t, double dt)
+{
+    if (!s) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    launchInitScalars(s->d_sc, dt, t, s->cfg.CFL, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->history.clear();
+    s->stepsTotal = 0;
+    s->errKind = 0;
+    return EBB_OK;
+}
+
+int64_t ebb_history_length(ebb_solver* s) { return s ? int64_t(s->history.size() / HIST_FIELDS) : 0; }
+
+int ebb_history(ebb_solver* s, int64_t first, int64_t count, double* out)
+{
+    if (!s || !out || first < 0 || count < 0) return EBB_ERR_ARG;
+    if ((first + count) * HIST_FIELDS > int64_t(s->history.size())) return EBB_ERR_ARG;
+    std::memcpy(out, s->history.data() + first * HIST_FIELDS, sizeof(double) * size_t(count) * HIST_FIELDS);
+    return EBB_OK;
+}
+
+// Recompute the edge states of the last residual evaluation into d_estate.
+static int refreshEdgeStates(ebb_solver* s)
+{
+    if (!s->d_estate.p) HIPCHK((hipError_t)trackAlloc(s, s->d_estate, size_t(s->Fi + s->Fb) * 2));
+    if (!s->haveStage) return EBB_OK;
+    // re-run the last stage's gradient + face kernels on its (unchanged) input;
+    // the update kernel is not re-run, so no state or scalar changes
+    StageArgs a{};
+    a.qin = s->lastInput; a.qn = s->lastInput; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
+    a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.estate = s->d_estate.p; a.celldt = s->d_celldt.p;
+    a.sc = s->d_sc; a.kind = SK_RESIDUAL; a.limit = s->lastLimit; a.first = 0; a.checkActive = 0; a.recordErrors = 0;
+    // keep the device accumulators: the gradient kernel resets newDt, restore it afterwards
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    DevScalars saved = *s->h_sc;
+    launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+    launchFaces(s->dm, s->phys, a, s->stream);
+    HIPCHK(hipGetLastError());
+    *s->h_sc = saved;
+    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+int ebb_edge_states(ebb_solver* s, double* eq, double* flux, double* smax)
+{
+    if (!s) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    int rc = refreshEdgeStates(s);
+    if (rc) return rc;
+    const uint32_t F = s->Fi + s->Fb;
+    std::vector<double4> es(size_t(F) * 2);
+    std::vector<double> lf(size_t(F) * 4), ls(F);
+    HIPCHK(hipMemcpy(es.data(), s->d_estate.p, es.size() * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lf.data(), s->d_lenF.p, lf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ls.data(), s->d_lenS.p, ls.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t f = 0; f < F; f++) {
+        const uint32_t e = s->edgeOfFace[f];
+        if (eq) {
+            const double4 a = es[2 * f], b = es[2 * f + 1];
+            double* o = eq + size_t(e) * 8;
+            o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+        }
+        const double len = s->b.len[e];
+        // flux = (len*flux)/len is exact only up to rounding; report len*flux/len
+        if (flux) for (int k = 0; k < 4; k++) flux[size_t(e) * 4 + k] = lf[size_t(f) * 4 + k] / len;
+        if (smax) smax[e] = ls[f] / len;
+    }
+    return EBB_OK;
+}
+
+int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
+{
+    if (!s) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    if (grad) HIPCHK(hipMemcpy(grad, s->d_grad.p, size_t(s->N) * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    if (lim) {
+        const bool lp = s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0;
+        std::vector<double> l(size_t(s->N) * (lp ? 8 : 4));
+        HIPCHK(hipMemcpy(l.data(), s->d_lim.p, l.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < s->N; i++)
+            for (int k = 0; k < 4; k++) {
+                lim[size_t(i) * 8 + 2 * k] = lp ? l[size_t(i) * 8 + 2 * k] : l[size_t(i) * 4 + k];
+                lim[size_t(i) * 8 + 2 * k + 1] = lp ? l[size_t(i) * 8 + 2 * k + 1] : l[size_t(i) * 4 + k];
+            }
+    }
+    return EBB_OK;
+}
+
+// source/io.d:68-161
+int ebb_save_solution(ebb_solver* s, const char* path)
+{
+    if (!s || !path) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    int rc = refreshEdgeStates(s);
+    if (rc) return rc;
+    const uint32_t F = s->Fi + s->Fb, N = s->N;
+    std::vector<double4> es(size_t(F) * 2), q(N);
+    HIPCHK(hipMemcpy(es.data(), s->d_estate.p, es.size() * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(q.data(), s->qcur(), size_t(N) * sizeof(double4), hipMemcpyDeviceToHost));
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    if (!s->haveStage) {
+        // no residual evaluated yet: edge states are the set-up values (far states)
+        for (uint32_t k = 0; k < s->Fb; k++) {
+            const BFace& bf = s->bfaces[k];
+            es[2 * (s->Fi + k) + 1] = make_double4(bf.far[0], bf.far[1], bf.far[2], bf.far[3]);
+        }
+    }
+    std::vector<uint8_t> buf;
+    buf.reserve(64 + size_t(N) * 150);
+    putBE32(buf, 0xEA98E1F5u);
+    putBE32(buf, 2);
+    putBE32(buf, N);
+    putBEd(buf, s->h_sc->t);
+    putBEd(buf, s->h_sc->dt);
+    putBE32(buf, 4);
+    putBE32(buf, uint32_t(s->cfg.order));
+    for (uint32_t i = 0; i < N; i++) {
+        const uint32_t ne = s->b.nEdgesCell[i];
+        putBE32(buf, i);
+        buf.push_back(uint8_t(ne));
+        putBEd(buf, q[i].x); putBEd(buf, q[i].y); putBEd(buf, q[i].z); putBEd(buf, q[i].w);
+        for (uint32_t j = 0; j < ne; j++) {
+            const uint32_t e = s->b.cellEdges[size_t(i) * MAXE + j];
+            const uint32_t f = s->faceOfEdge[e];
+            const double4 v = (s->b.eCell0[e] == i) ? es[2 * f] : es[2 * f + 1];
+            putBEd(buf, v.x); putBEd(buf, v.y); putBEd(buf, v.z); putBEd(buf, v.w);
+        }
+    }
+    // the reference computes the CRC over its buffer including the 4 not yet written CRC bytes (zero here)
+    const size_t body = buf.size();
+    buf.resize(body + 4, 0);
+    const uint32_t crc = crc32(buf.data(), buf.size());
+    buf[body + 0] = uint8_t(crc);
+    buf[body + 1] = uint8_t(crc >> 8);
+    buf[body + 2] = uint8_t(crc >> 16);
+    buf[body + 3] = uint8_t(crc >> 24);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return EBB_ERR_IO;
+    const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+    std::fclose(f);
+    return w == buf.size() ? EBB_OK : EBB_ERR_IO;
+}
+
+// source/io.d:218-366 (single rank: data points must equal the interior cell count)
+int ebb_load_solution(ebb_solver* s, const char* path)
+{
+    if (!s || !path) return EBB_ERR_ARG;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return EBB_ERR_IO;
+    std::vector<uint8_t> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if (buf.size() < 28 || getBE32(buf.data()) != 0xEA98E1F5u) return EBB_ERR_PARSE;
+    const uint32_t ver = getBE32(buf.data() + 4), np = getBE32(buf.data() + 8);
+    if (np != s->N) return EBB_ERR_PARSE;
+    const double t = getBEd(buf.data() + 12), dt = getBEd(buf.data() + 20);
+    size_t off = 28;
+    std::vector<double4> q(s->N);
+    if (ver == 1) {
+        if (buf.size() < off + size_t(np) * 32) return EBB_ERR_PARSE;
+        for (uint32_t i = 0; i < np; i++, off += 32)
+            q[i] = make_double4(getBEd(&buf[off]), getBEd(&buf[off + 8]), getBEd(&buf[off + 16]), getBEd(&buf[off + 24]));
+    } else if (ver == 2) {
+        off += 8;  // M, order
+        for (uint32_t i = 0; i < np; i++) {
+            if (buf.size() < off + 5 + 32) return EBB_ERR_PARSE;
+            const uint8_t ne = buf[off + 4];
+            off += 5;
+            q[i] = make_double4(getBEd(&buf[off]), getBEd(&buf[off + 8]), getBEd(&buf[off + 16]), getBEd(&buf[off + 24]));
+            off += 32 + size_t(ne) * 32;
+            if (off > buf.size()) return EBB_ERR_PARSE;
+        }
+    } else {
+        return EBB_ERR_PARSE;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpy(s->qcur(), q.data(), size_t(s->N) * sizeof(double4), hipMemcpyHostToDevice));
+    return ebb_run_reset(s, t, dt);
+}
+
+int ebb_write_forces(ebb_solver* s, const char* path, int64_t first, int64_t count, int append)
+{
+    if (!s || !path || first < 0 || count < 0) return EBB_ERR_ARG;
+    if ((first + count) * HIST_FIELDS > int64_t(s->history.size())) return EBB_ERR_ARG;
+    std::vector<uint8_t> buf;
+    buf.reserve(size_t(count) * 24);
+    for (int64_t k = first; k < first + count; k++) {
+        const double* h = s->history.data() + k * HIST_FIELDS;
+        putBEd(buf, h[0]);
+        putBEd(buf, h[7]);
+        putBEd(buf, h[8]);
+    }
+    FILE* f = std::fopen(path, append ? "ab" : "wb");
+    if (!f) return EBB_ERR_IO;
+    const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+    std::fclose(f);
+    return w == buf.size() ? EBB_OK : EBB_ERR_IO;
+}
+
+int ebb_last_error(ebb_solver* s, int* kind, int64_t* index, int64_t* step, char* msg, size_t len)
+{
+    if (!s) return EBB_ERR_ARG;
+    if (kind) *kind = s->errKind;
+    if (index) *index = s->errIndex;
+    if (step) *step = s->errStep;
+    if (msg && len) std::snprintf(msg, len, "%s%s%s", s->errMsg.c_str(), lastDeviceError.empty() ? "" : " ", lastDeviceError.c_str());
+    return EBB_OK;
+}
+
+int ebb_time_steps(ebb_solver* s, int steps, double out[6])
+{
+    if (!s || !out || steps <= 0) return EBB_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    { int rc_ = readScalars(s); if (rc_) return rc_; }
+    DevScalars saved = *s->h_sc;
+    const int nst = stagesPerStep(s->cfg);
+    std::vector<hipEvent_t> ev(size_t(steps) * nst * 3 + 1);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    const double4* q0 = s->qcur();
+    (void)q0;
+    // same launch sequence as stepStages, with an event after each kernel
+    HIPCHK(hipEventRecord(ev[0], s->stream));
+    size_t k = 1;
+    const Config& c = s->cfg;
+    const int msl = c.multistageLimiting ? 1 : 0, lts = c.localTimestep ? 1 : 0;
+    for (int st = 0; st < steps; st++) {
+        double4 *q = s->qcur(), *nx = s->qnext(), *T = s->d_T.p, *T2 = s->d_T2.p;
+        struct St { int kind; double4 *in, *out; int limit, dtu, first, last; };
+        std::vector<St> seq;
+        switch (c.integratorKind()) {
+        case IntEuler: seq = {{SK_EULER, q, nx, 1, 1, 1, 1}}; break;
+        case IntRK2: seq = {{SK_RK2_1, q, T, 1, 1, 1, 0}, {SK_RK2_2, T, nx, msl, 0, 0, 1}}; break;
+        case IntRK2TVD: seq = {{SK_TVD_1, q, T, 1, lts, 1, 0}, {SK_TVD_2, T, nx, 0, lts ? 0 : 1, 0, 1}}; break;
+        default: seq = {{SK_RK4_1, q, T, 1, 1, 1, 0}, {SK_RK4_2, T, T2, msl, 0, 0, 0}, {SK_RK4_3, T2, T, msl, 0, 0, 0}, {SK_RK4_4, T, nx, msl, 0, 0, 1}};
+        }
+        for (auto& x : seq) {
+            StageArgs a{};
+            a.qin = x.in; a.qn = q; a.qout = x.out; a.acc = s->d_acc.p; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
+            a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.sc = s->d_sc;
+            a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
+            a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
+            launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+            HIPCHK(hipEventRecord(ev[k++], s->stream));
+            launchFaces(s->dm, s->phys, a, s->stream);
+            HIPCHK(hipEventRecord(ev[k++], s->stream));
+            launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
+            HIPCHK(hipEventRecord(ev[k++], s->stream));
+            s->lastInput = x.in;
+            s->lastLimit = x.limit;
+            s->haveStage = true;
+        }
+        s->cur ^= 1;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(ev[k - 1]));
+    double tg = 0, tf = 0, tu = 0;
+    for (size_t j = 1; j < k; j += 3) {
+        float a, b, cc;
+        HIPCHK(hipEventElapsedTime(&a, ev[j - 1], ev[j]));
+        HIPCHK(hipEventElapsedTime(&b, ev[j], ev[j + 1]));
+        HIPCHK(hipEventElapsedTime(&cc, ev[j + 1], ev[j + 2]));
+        tg += a; tf += b; tu += cc;
+    }
+    float total;
+    HIPCHK(hipEventElapsedTime(&total, ev[0], ev[k - 1]));
+    const double n = double(steps) * nst;
+    out[0] = tg / n; out[1] = tf / n; out[2] = tu / n; out[3] = 0;
+    out[4] = double(total) / steps;
+    out[5] = nst;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    // timing steps advance the state but not the run loop's clock
+    *s->h_sc = saved;
+    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+int64_t ebb_solver_device_bytes(const ebb_solver* s) { return s ? s->deviceBytes : 0; }
+
+void ebb_solver_destroy(ebb_solver* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->d_nbr.release(); s->d_cface.release(); s->d_refId.release(); s->d_forceFaces.release();
+    s->d_gm.release(); s->d_nia.release(); s->d_cd.release(); s->d_cper.release(); s->d_flen.release();
+    s->d_grad.release(); s->d_lim.release(); s->d_lenF.release(); s->d_lenS.release(); s->d_celldt.release();
+    s->d_hist.release(); s->d_chunkTrees.release(); s->d_chunkMask.release();
+    s->d_cen.release(); s->d_fn.release(); s->d_fmid.release(); s->d_fcell.release(); s->d_bface.release();
+    s->d_src.release(); s->d_qA.release(); s->d_qB.release(); s->d_T.release(); s->d_T2.release();
+    s->d_acc.release(); s->d_R.release(); s->d_estate.release();
+    if (s->d_sc) (void)hipFree(s->d_sc);
+    if (s->h_sc) (void)hipHostFree(s->h_sc);
+    if (s->h_hist) (void)hipHostFree(s->h_hist);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // extern "C"

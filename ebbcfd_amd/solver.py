@@ -1,0 +1,240 @@
+"""Python mirror of the reference's solver interface over the C ABI.
+
+Names follow the reference:
+  * ``load_config``   -- source/config.d:112 loadConfig (JSON text or dict)
+  * ``import_mesh``   -- source/io.d:669 importMesh (+ buildMesh, source/mesh.d:217)
+  * ``UMesh2``        -- source/mesh.d:101 (built mesh; geometry accessors)
+  * ``Solver``        -- ufvmSetup (source/solve.d:175) on a HIP device, with
+                         ``ufvm_solver`` (solve.d:322), ``step`` (integrators.d),
+                         ``run`` (app/solver.d:38 runIntegrator), ``save_solution``
+                         / ``load_solution`` (io.d:68, 218).
+Everything runs through libebbcfd.so on the GPU; nothing here computes.
+"""
+import ctypes as C
+import json
+
+import numpy as np
+
+from ._lib import EbbError, lib
+
+HIST_FIELDS = ("t", "dt", "rho_rms", "u_rms", "v_rms", "E_rms", "Rmax", "drag", "lift", "CFL")
+
+
+def _p(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _check(rc, what, solver=None):
+    if rc:
+        detail = ""
+        if solver is not None:
+            kind, idx, step = C.c_int(0), C.c_int64(0), C.c_int64(0)
+            buf = C.create_string_buffer(512)
+            lib().ebb_last_error(solver, C.byref(kind), C.byref(idx), C.byref(step), buf, 512)
+            detail = "%s; index %d step %d" % (buf.value.decode(errors="replace"), idx.value, step.value)
+            err = EbbError(rc, what, detail)
+            err.index, err.step = idx.value, step.value
+            raise err
+        raise EbbError(rc, what, detail)
+
+
+class Config:
+    """loadConfig result (source/config.d:72-94)."""
+
+    def __init__(self, text):
+        if isinstance(text, dict):
+            text = json.dumps(text)
+        self.text = text
+        self.h = C.c_void_p()
+        _check(lib().ebb_config_parse(text.encode(), C.byref(self.h)), "loadConfig")
+
+    def __getitem__(self, key):
+        v = C.c_double(0)
+        _check(lib().ebb_config_get(self.h, key.encode(), C.byref(v)), "config get %s" % key)
+        return v.value
+
+    def __setitem__(self, key, value):
+        _check(lib().ebb_config_set(self.h, key.encode(), float(value)), "config set %s" % key)
+
+    @property
+    def mesh_file(self):
+        buf = C.create_string_buffer(4096)
+        lib().ebb_config_mesh_file(self.h, buf, 4096)
+        return buf.value.decode()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ebb_config_destroy(self.h)
+            self.h = None
+
+
+def load_config(text):
+    return Config(text)
+
+
+class UMesh2:
+    """Built mesh (source/mesh.d:101-672)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        info = np.zeros(8, np.int64)
+        _check(lib().ebb_mesh_info(self.h, _p(info, C.c_int64)), "mesh info")
+        (self.n_cells, self.n_interior, self.n_ghost, self.n_edges, self.n_interior_edges,
+         self.n_boundary_edges, self.max_edges, self.n_tags) = map(int, info)
+
+    @classmethod
+    def from_raw(cls, raw):
+        nodes = np.ascontiguousarray(raw.nodes, np.float64).reshape(-1, 2)
+        eoff = np.ascontiguousarray(raw.elem_offsets, np.uint32)
+        enodes = np.ascontiguousarray(raw.elem_nodes, np.uint32)
+        bn = np.ascontiguousarray(raw.bnodes, np.uint32).reshape(-1, 2)
+        bgs = np.ascontiguousarray(raw.bgroup_start, np.uint64)
+        tags = (C.c_char_p * max(1, len(raw.btags)))(*[t.encode() for t in raw.btags])
+        h = C.c_void_p()
+        _check(lib().ebb_mesh_create(nodes.shape[0], _p(nodes), eoff.size - 1, _p(eoff, C.c_uint32),
+                                     _p(enodes, C.c_uint32), bn.shape[0], _p(bn, C.c_uint32), len(raw.btags),
+                                     tags, _p(bgs, C.c_uint64), C.byref(h)), "buildMesh")
+        m = cls(h)
+        m.btags = list(raw.btags)
+        return m
+
+    def cell_geometry(self):
+        Cn, N = self.n_cells, self.n_interior
+        out = dict(area=np.zeros(Cn), d=np.zeros(Cn), perim=np.zeros(Cn), centroid=np.zeros((Cn, 2)),
+                   gradMat=np.zeros((N, 2, 6)), edges=np.zeros((N, 6), np.uint32), fm=np.zeros((N, 6), np.int8),
+                   nbrs=np.zeros((N, 6), np.uint32), nEdges=np.zeros(N, np.uint8))
+        _check(lib().ebb_mesh_cell_geometry(self.h, _p(out["area"]), _p(out["d"]), _p(out["perim"]),
+                                            _p(out["centroid"]), _p(out["gradMat"]), _p(out["edges"], C.c_uint32),
+                                            _p(out["fm"], C.c_int8), _p(out["nbrs"], C.c_uint32),
+                                            _p(out["nEdges"], C.c_uint8)), "cell geometry")
+        return out
+
+    def edge_geometry(self):
+        E = self.n_edges
+        out = dict(nodeIdx=np.zeros((E, 2), np.uint32), cellIdx=np.zeros((E, 2), np.uint32), len=np.zeros(E),
+                   normal=np.zeros((E, 2)), mid=np.zeros((E, 2)), bGroup=np.zeros(E, np.int32))
+        _check(lib().ebb_mesh_edge_geometry(self.h, _p(out["nodeIdx"], C.c_uint32), _p(out["cellIdx"], C.c_uint32),
+                                            _p(out["len"]), _p(out["normal"]), _p(out["mid"]),
+                                            _p(out["bGroup"], C.c_int32)), "edge geometry")
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ebb_mesh_destroy(self.h)
+            self.h = None
+
+
+def import_mesh(path):
+    h = C.c_void_p()
+    _check(lib().ebb_mesh_load(str(path).encode(), C.byref(h)), "importMesh %s" % path)
+    return UMesh2(h)
+
+
+def build_mesh(raw):
+    return UMesh2.from_raw(raw)
+
+
+class Solver:
+    """ufvmSetup + device-resident state for one mesh (source/solve.d:175-279)."""
+
+    def __init__(self, mesh, config, device=0):
+        if not isinstance(config, Config):
+            config = Config(config)
+        self.mesh, self.config = mesh, config
+        self.h = C.c_void_p()
+        _check(lib().ebb_solver_create(mesh.h, config.h, int(device), C.byref(self.h)), "ufvmSetup")
+        self.n_interior, self.n_cells, self.n_edges = mesh.n_interior, mesh.n_cells, mesh.n_edges
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ebb_solver_destroy(self.h)
+            self.h = None
+
+    # state
+    @property
+    def q(self):
+        out = np.zeros((self.n_interior, 4))
+        _check(lib().ebb_state_download(self.h, _p(out)), "state download", self.h)
+        return out
+
+    @q.setter
+    def q(self, value):
+        v = np.ascontiguousarray(value, np.float64).reshape(self.n_interior, 4)
+        _check(lib().ebb_state_upload(self.h, _p(v)), "state upload", self.h)
+
+    def q_all(self):
+        out = np.zeros((self.n_cells, 4))
+        _check(lib().ebb_state_download_all(self.h, _p(out)), "state download", self.h)
+        return out
+
+    # operator
+    def ufvm_solver(self, limit=True, dt_update=True, rmax=0.0):
+        """One residual evaluation; returns (R, newDt, Rmax)."""
+        R = np.zeros((self.n_interior, 4))
+        nd, rm = C.c_double(0), C.c_double(rmax)
+        _check(lib().ebb_residual(self.h, int(limit), int(dt_update), _p(R), C.byref(nd), C.byref(rm)),
+               "ufvmSolver", self.h)
+        return R, nd.value, rm.value
+
+    def step(self, dt):
+        """Integrator.step: returns (new dt, Rmax)."""
+        d, rm = C.c_double(dt), C.c_double(0)
+        _check(lib().ebb_step(self.h, C.byref(d), C.byref(rm)), "step", self.h)
+        return d.value, rm.value
+
+    def run(self, max_steps=-1):
+        """runIntegrator on the device; returns (history rows of this call, t, dt)."""
+        start = lib().ebb_history_length(self.h)
+        t, dt, n = C.c_double(0), C.c_double(0), C.c_int64(0)
+        rc = lib().ebb_run(self.h, int(max_steps), C.byref(t), C.byref(dt), C.byref(n))
+        _check(rc, "runIntegrator", self.h)
+        return self.history(start, n.value), t.value, dt.value
+
+    def run_state(self):
+        t, dt, cfl, n = C.c_double(0), C.c_double(0), C.c_double(0), C.c_int64(0)
+        _check(lib().ebb_run_state(self.h, C.byref(t), C.byref(dt), C.byref(cfl), C.byref(n)), "run state", self.h)
+        return dict(t=t.value, dt=dt.value, CFL=cfl.value, steps=n.value)
+
+    def run_reset(self, t=0.0, dt=None):
+        _check(lib().ebb_run_reset(self.h, float(t), float(self.config["dt"] if dt is None else dt)), "reset", self.h)
+
+    def history(self, first=0, count=None):
+        n = lib().ebb_history_length(self.h)
+        if count is None:
+            count = n - first
+        out = np.zeros((count, len(HIST_FIELDS)))
+        if count:
+            _check(lib().ebb_history(self.h, int(first), int(count), _p(out)), "history", self.h)
+        return out
+
+    def edge_states(self):
+        E = self.n_edges
+        eq, fl, sm = np.zeros((E, 2, 4)), np.zeros((E, 4)), np.zeros(E)
+        _check(lib().ebb_edge_states(self.h, _p(eq), _p(fl), _p(sm)), "edge states", self.h)
+        return eq, fl, sm
+
+    def cell_gradients(self):
+        g, l = np.zeros((self.n_interior, 4, 2)), np.zeros((self.n_interior, 4, 2))
+        _check(lib().ebb_cell_gradients(self.h, _p(g), _p(l)), "cell gradients", self.h)
+        return g, l
+
+    def save_solution(self, path):
+        _check(lib().ebb_save_solution(self.h, str(path).encode()), "saveSolution", self.h)
+
+    def load_solution(self, path):
+        _check(lib().ebb_load_solution(self.h, str(path).encode()), "loadSolution", self.h)
+
+    def write_forces(self, path, first=0, count=None, append=False):
+        if count is None:
+            count = lib().ebb_history_length(self.h) - first
+        _check(lib().ebb_write_forces(self.h, str(path).encode(), int(first), int(count), int(append)), "forces")
+
+    def time_steps(self, steps):
+        """Device time per kernel launch (ms) over `steps` integrator steps, HIP events on the solver stream."""
+        out = np.zeros(6)
+        _check(lib().ebb_time_steps(self.h, int(steps), _p(out)), "time steps", self.h)
+        return dict(gradient_ms=out[0], faces_ms=out[1], update_ms=out[2], step_ms=out[4], stages=int(out[5]))
+
+    @property
+    def device_bytes(self):
+        return lib().ebb_solver_device_bytes(self.h)

@@ -1,0 +1,167 @@
+/*
+ * ebbcfd.h -- C ABI of the MI355X explicit residual-and-update solver.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (EBB_OK = 0) instead of the reference's exceptions.  Every call on one
+ * solver handle is serialised by the caller; several handles may coexist.
+ * Cell arrays use the reference's numbering (elements in file order, ghosts
+ * appended in boundary-edge order); states are 4 doubles per cell
+ * (rho, rho u, rho v, rho E).  Each entry point names the reference
+ * interface it replaces.
+ */
+#ifndef EBBCFD_H
+#define EBBCFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ebb_mesh ebb_mesh;
+typedef struct ebb_config ebb_config;
+typedef struct ebb_solver ebb_solver;
+
+enum {
+    EBB_OK = 0,
+    EBB_ERR_ARG = 1,        /* bad argument / handle */
+    EBB_ERR_IO = 2,         /* file could not be read or written */
+    EBB_ERR_PARSE = 3,      /* malformed mesh or config */
+    EBB_ERR_MESH = 4,       /* mesh topology/geometry rejected (e.g. negative area) */
+    EBB_ERR_BC = 5,         /* missing or inconsistent boundary condition */
+    EBB_ERR_DEVICE = 6,     /* HIP runtime error or no device */
+    EBB_ERR_EDGE_NAN = 7,   /* NaN flux on an edge (EdgeException, source/solve.d:806) */
+    EBB_ERR_CELL_NAN = 8,   /* NaN state in a cell (CellException, app/solver.d:148) */
+    EBB_ERR_STATE = 9       /* call not valid in the solver's current state */
+};
+
+/* Text for a status code. */
+const char* ebb_error_string(int code);
+
+/* ---------------------------------------------------------------- mesh */
+
+/* importMesh + buildMesh: gmsh 2.2 ASCII reader (source/io.d:669-990) followed
+ * by the geometry precompute (source/mesh.d:217-643). */
+int ebb_mesh_load(const char* path, ebb_mesh** out);
+
+/* buildMesh from arrays (source/mesh.d:217): xy[2*n_nodes]; elements as CSR
+ * (elem_offsets[n_elem+1], elem_nodes 1-based, counter-clockwise);
+ * bnodes[2*n_bnodes] 0-based boundary node pairs grouped by tag;
+ * tags[n_tags] names; bgroup_start[n_tags] first bnode of each tag. */
+int ebb_mesh_create(uint32_t n_nodes, const double* xy, uint32_t n_elem, const uint32_t* elem_offsets,
+                    const uint32_t* elem_nodes, uint32_t n_bnodes, const uint32_t* bnodes, uint32_t n_tags,
+                    const char* const* tags, const uint64_t* bgroup_start, ebb_mesh** out);
+
+/* info[0..7]: cells (incl. ghosts), interior cells, ghost cells, edges,
+ * interior edges, boundary edges, max edges per cell, boundary tags. */
+int ebb_mesh_info(const ebb_mesh* m, int64_t info[8]);
+
+/* UCell2 geometry (source/mesh.d:73-94); every output may be NULL.
+ * area/d/perim/centroid over all cells (centroid 2 per cell); per interior
+ * cell 6 slots each of edges, flux multipliers, neighbours and 12 gradMat
+ * entries (2 x 6, row-major); n_edges per interior cell. */
+int ebb_mesh_cell_geometry(const ebb_mesh* m, double* area, double* d, double* perim, double* centroid,
+                           double* grad_mat, uint32_t* edges, int8_t* flux_mult, uint32_t* neighbors,
+                           uint8_t* n_edges);
+
+/* Edge geometry (source/mesh.d:38-65): node pair, cell pair (owner first),
+ * length, unit normal, midpoint, boundary group (-1 for interior edges). */
+int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, double* len, double* normal,
+                           double* mid, int32_t* bgroup);
+
+void ebb_mesh_destroy(ebb_mesh* m);
+
+/* ---------------------------------------------------------------- config */
+
+/* loadConfig (source/config.d:112-313): the reference JSON keys and defaults
+ * ("boudaryConditions", "dt" as a JSON float, ...).  Extra options live in an
+ * optional "extensions" object: {"mms": bool, "maxSteps": int,
+ * "precision": "fp64"|"mixed"}. */
+int ebb_config_parse(const char* json, ebb_config** out);
+
+/* Read / override a numeric field by its JSON key: "dt", "tEnd", "CFL",
+ * "lpThresh", "saveIter", "plotIter", "order", "maxSteps", "limited",
+ * "dynamicDt", "localTimestep", "multistageLimiting", "cflAdjust",
+ * "viscosity", "mms", "Pr", "R", "Re", "L", "ic0".."ic3". */
+int ebb_config_get(const ebb_config* c, const char* key, double* value);
+int ebb_config_set(ebb_config* c, const char* key, double value);
+/* Mesh file named by the config ("mesh"), copied into buf. */
+int ebb_config_mesh_file(const ebb_config* c, char* buf, size_t len);
+void ebb_config_destroy(ebb_config* c);
+
+/* ---------------------------------------------------------------- solver */
+
+/* ufvmSetup (source/solve.d:175-279): binds boundary conditions, sets the
+ * freestream (or manufactured-solution) initial state, and uploads the mesh
+ * to HIP device `device`.  The solver keeps its own copy of mesh and config. */
+int ebb_solver_create(const ebb_mesh* m, const ebb_config* c, int device, ebb_solver** out);
+
+/* Interior states in reference cell order, n_interior x 4. */
+int ebb_state_upload(ebb_solver* s, const double* q);
+int ebb_state_download(ebb_solver* s, double* q);
+/* All cells (interior + ghost slots of the current state buffer), n_cells x 4. */
+int ebb_state_download_all(ebb_solver* s, double* q);
+
+/* ufvmSolver (source/solve.d:322-1122) on the current state: writes the ghost
+ * states, R (n_interior x 4, already scaled by -1/area, MMS source added when
+ * configured), new_dt (min CFL*d/sAve) and max-accumulates *rmax. */
+int ebb_residual(ebb_solver* s, int limit, int dt_update, double* R, double* new_dt, double* rmax);
+
+/* Integrator.step (source/integrators.d:22-270): one step with *dt; on return
+ * *dt is the last stage's new time step and *rmax the step's max |R|. */
+int ebb_step(ebb_solver* s, double* dt, double* rmax);
+
+/* runIntegrator (app/solver.d:38-277) on the device: steps while t < tEnd,
+ * at most max_steps (< 0: no limit) more steps, keeping t, dt, CFL and the
+ * RMS/cflAdjust state between calls.  Per-step diagnostics are appended to the
+ * history (ebb_history).  Returns EBB_ERR_EDGE_NAN/EBB_ERR_CELL_NAN when the
+ * reference would have thrown (details in ebb_last_error). */
+int ebb_run(ebb_solver* s, int64_t max_steps, double* t, double* dt, int64_t* steps_done);
+
+/* Time-loop state: t, dt, CFL, steps done (total over all ebb_run calls). */
+int ebb_run_state(ebb_solver* s, double* t, double* dt, double* cfl, int64_t* steps);
+/* Reset the time loop to (t, dt) with the configured CFL (as after ufvmSetup). */
+int ebb_run_reset(ebb_solver* s, double t, double dt);
+
+/* History rows of steps [first, first+count): t, dt, rho/u/v/E RMS, Flux_R
+ * (Rmax), drag, lift, CFL -- the printed line of app/solver.d:220-230 and the
+ * .frc record of app/solver.d:178-193. */
+int ebb_history(ebb_solver* s, int64_t first, int64_t count, double* out);
+int64_t ebb_history_length(ebb_solver* s);
+
+/* Edge data of the last residual evaluation (Edge.q[2], flux, sMax): eq is
+ * n_edges x 2 x 4, flux n_edges x 4, smax n_edges; any output may be NULL. */
+int ebb_edge_states(ebb_solver* s, double* eq, double* flux, double* smax);
+
+/* Per interior cell gradient (4 x 2) and limiter (4 x 2) of the last residual evaluation. */
+int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim);
+
+/* saveSolution / loadSolution (source/io.d:68-161, 218-366): .esln v2,
+ * big-endian, edge states of the last residual evaluation.  Loading restores
+ * the interior state and the time loop's t and dt. */
+int ebb_save_solution(ebb_solver* s, const char* path);
+int ebb_load_solution(ebb_solver* s, const char* path);
+
+/* Append the boundary-force records (t, drag, lift; big-endian f64) of history
+ * rows [first, first+count) to a .frc file (app/solver.d:178-193, 252-260). */
+int ebb_write_forces(ebb_solver* s, const char* path, int64_t first, int64_t count, int append);
+
+/* Error details: kind (EBB_ERR_*), reference edge or cell index, step. */
+int ebb_last_error(ebb_solver* s, int* kind, int64_t* index, int64_t* step, char* msg, size_t len);
+
+/* Device timing of `steps` integrator steps with HIP events on the solver's
+ * stream.  out[0..3]: average ms per launch of the gradient, face and update
+ * kernels; out[4]: ms per step; out[5]: launches per step of each kernel kind. */
+int ebb_time_steps(ebb_solver* s, int steps, double out[6]);
+
+/* Device bytes held by the solver. */
+int64_t ebb_solver_device_bytes(const ebb_solver* s);
+
+void ebb_solver_destroy(ebb_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EBBCFD_H */

@@ -1,0 +1,74 @@
+"""Shared test inputs: matching meshes/configs for the HIP path and the oracle."""
+import copy
+import json
+import os
+
+import numpy as np
+
+from ebbcfd_amd import meshgen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MMS_DIR = os.path.join(ROOT, "tests", "golden", "ms_refinement")
+
+
+def base_config(**over):
+    cfg = {
+        "mesh": "synthetic.msh", "dt": 1.0e-4, "tEnd": 1.0e9, "flux": "roeFlux", "plotIter": 100, "saveIter": -1,
+        "CFL": 0.5, "integrator": "Euler", "forceBoundary": "Wall", "limited": True, "order": 2,
+        "initialConditions": [0.5, 0.0, 1.0, 1.0], "viscosity": False, "cflAdjust": True,
+        "physicalConfig": {"Pr": 0.7, "R": 8.3145, "gamma": 1.4, "Re": 100.0, "L": 1.0},
+        "boudaryConditions": [],
+    }
+    cfg.update(over)
+    return cfg
+
+
+def channel_case(nx=24, ny=8, height=0.06, seed=3, walls="inviscidWall", M=0.5, aoa=2.0, **over):
+    raw = meshgen.channel_bump(nx, ny, height, seed=seed)
+    q = [M, aoa, 1.0, 1.0]
+    bcs = [{"tag": "Inflow", "type": "fullState", "q": q}, {"tag": "Outflow", "type": "fullState", "q": q},
+           {"tag": "Lower", "type": walls, "q": q}, {"tag": "Upper", "type": walls, "q": q}]
+    cfg = base_config(initialConditions=q, boudaryConditions=bcs, forceBoundary="Lower", **over)
+    return raw, cfg
+
+
+def cylinder_case(n_theta=64, n_r=32, M=0.2, viscous=True, flux="rusanovFlux", integrator="RK2_TVD", r_far=30.0,
+                  **over):
+    raw = meshgen.cylinder_ogrid(n_theta, n_r, r_far=r_far, stretch=1.006 if n_r >= 512 else 1.08)
+    q = [M, 0.0, 1.0, 1.0]
+    bcs = [{"tag": "Wall", "type": "viscidWall" if viscous else "inviscidWall", "q": q},
+           {"tag": "Farfield", "type": "fullState", "q": q}]
+    cfg = base_config(initialConditions=q, boudaryConditions=bcs, forceBoundary="Wall", flux=flux,
+                      integrator=integrator, viscosity=viscous,
+                      physicalConfig={"Pr": 0.7, "R": 8.3145, "gamma": 1.4, "Re": 100.0, "L": 1.0}, **over)
+    return raw, cfg
+
+
+def mms_case(n=16, jitter=0.1, seed=1, integrator="RK4", **over):
+    raw = meshgen.mms_square(n, jitter, seed)
+    q = [0.1, 0.0, 1.1, 1.0]
+    cfg = base_config(initialConditions=q, boudaryConditions=[{"tag": "Freestream", "type": "dirichlet", "q": q}],
+                      forceBoundary="Wall", integrator=integrator, dt=1.0e-5, CFL=0.5,
+                      physicalConfig={"Pr": 0.71, "R": 1.0, "gamma": 1.4, "Re": 1000.0, "L": 1.0},
+                      extensions={"mms": True}, **over)
+    return raw, cfg
+
+
+def oracle_cfg(cfg):
+    """The oracle reads the MMS switch at the top level."""
+    c = copy.deepcopy(cfg)
+    c["mms"] = bool(cfg.get("extensions", {}).get("mms", False))
+    return c
+
+
+def reference_fixture(name):
+    from oracle.gmsh import parse_gmsh
+    raw = parse_gmsh(os.path.join(MMS_DIR, name + ".msh"))
+    cfg = json.load(open(os.path.join(MMS_DIR, name + ".json")))
+    cfg["extensions"] = {"mms": True}
+    return raw, cfg
+
+
+def perturb(q, scale=0.02, seed=0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return q * (1.0 + scale * rng.uniform(-1, 1, size=q.shape))

@@ -1,0 +1,156 @@
+"""HIP path vs the CPU oracle, called through the C ABI on the GPU.
+
+Bar: the FP64 kernels keep the reference's operation order with no FMA
+contraction, so per-stage results are compared bit for bit (assert_array_equal
+treats +0 and -0 as equal); multi-step trajectories must stay within 1e-11
+relative (they are expected to stay bit-identical as well).  The ConstP and
+TempPresInflow ghost states use the device pow(), so those cases carry a
+1e-13 relative tolerance.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ebbcfd_amd import Solver, build_mesh
+
+from helpers import channel_case, cylinder_case, mms_case, oracle_cfg, perturb, reference_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(raw, cfg, q=None):
+    om = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    s = Solver(build_mesh(raw), cfg)
+    if q is not None:
+        qq = om.q
+        qq[: om.n_interior] = q
+        om.q = qq
+        s.q = q
+    else:
+        np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
+    return om, s
+
+
+RESIDUAL_CASES = {
+    "roe_inviscid_wall": lambda: channel_case(flux="roeFlux"),
+    "rusanov_inviscid_wall": lambda: channel_case(flux="rusanovFlux"),
+    "roe_symmetry": lambda: channel_case(walls="symmetry", flux="roeFlux"),
+    "roe_symmetry_viscous": lambda: channel_case(walls="symmetry", flux="roeFlux", viscosity=True),
+    "roe_unlimited": lambda: channel_case(flux="roeFlux", limited=False),
+    "roe_order1": lambda: channel_case(flux="roeFlux", order=1),
+    "cyl_ns_rusanov": lambda: cylinder_case(),
+    "cyl_ns_roe": lambda: cylinder_case(flux="roeFlux"),
+    "cyl_euler_wall": lambda: cylinder_case(viscous=False, flux="roeFlux"),
+    "mms_dirichlet": lambda: mms_case(),
+    "mms_dirichlet_viscous": lambda: mms_case(viscosity=True),
+    "mms_fixture_lp": lambda: reference_fixture("ms_refine_tri_pert_1"),
+    "mms_fixture_quads_lp": lambda: reference_fixture("ms_refine_1"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RESIDUAL_CASES))
+@pytest.mark.parametrize("limit", [True, False])
+def test_residual_bitwise(name, limit):
+    raw, cfg = RESIDUAL_CASES[name]()
+    om0 = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    q = perturb(om0.q[: om0.n_interior], 0.03, seed=7)
+    om, s = _pair(raw, cfg, q)
+    R_o, nd_o, rm_o = om.residual(limit=limit, dt_update=True)
+    R_g, nd_g, rm_g = s.ufvm_solver(limit=limit, dt_update=True)
+    np.testing.assert_array_equal(R_g, R_o)
+    assert nd_g == nd_o
+    assert rm_g == rm_o
+    # ghost states, gradients/limiters and edge data of the same evaluation
+    np.testing.assert_array_equal(s.q_all(), om.q)
+    g_g, l_g = s.cell_gradients()
+    g_o, l_o, _ = om.cell_data()
+    if cfg.get("order", 2) > 1:
+        np.testing.assert_array_equal(g_g, g_o[: om.n_interior])
+        np.testing.assert_array_equal(l_g, l_o[: om.n_interior])
+    eq_g, fl_g, sm_g = s.edge_states()
+    eq_o, fl_o, sm_o = om.edge_data()
+    np.testing.assert_array_equal(eq_g, eq_o)
+    # the device stores len*flux; dividing back by len is exact to 1 ulp
+    np.testing.assert_allclose(fl_g, fl_o, rtol=4e-16, atol=1e-300)
+    np.testing.assert_allclose(sm_g, sm_o, rtol=4e-16, atol=1e-300)
+
+
+STEP_CASES = {
+    "euler_roe": lambda: channel_case(integrator="Euler"),
+    "rk2_roe": lambda: channel_case(integrator="RK2"),
+    "rk2tvd_rusanov": lambda: channel_case(integrator="RK2_TVD", flux="rusanovFlux"),
+    "rk4_roe": lambda: channel_case(integrator="RK4"),
+    "rk4_nomultistage": lambda: channel_case(integrator="RK4", multistageLimiting=False),
+    "local_dt_rk2tvd": lambda: channel_case(integrator="RK2_TVD", localTimestep=True),
+    "local_dt_rk4": lambda: channel_case(integrator="RK4", localTimestep=True),
+    "local_dt_euler": lambda: channel_case(integrator="Euler", localTimestep=True),
+    "cyl_ns_rk2tvd": lambda: cylinder_case(),
+    "mms_rk4": lambda: mms_case(),
+    "mms_fixture_euler_lp": lambda: reference_fixture("ms_refine_tri_pert_0"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(STEP_CASES))
+def test_run_matches_oracle(name):
+    raw, cfg = STEP_CASES[name]()
+    nsteps = 40
+    om, s = _pair(raw, cfg)
+    hist_o, t_o, dt_o = om.run(nsteps)
+    hist_g, t_g, dt_g = s.run(nsteps)
+    assert hist_g.shape == hist_o.shape
+    qo = om.q[: om.n_interior]
+    qg = s.q
+    scale = np.abs(qo).max(axis=0)
+    assert np.all(np.abs(qg - qo) <= 1e-11 * scale), np.abs(qg - qo).max()
+    np.testing.assert_array_equal(qg, qo)
+    np.testing.assert_array_equal(hist_g, hist_o)
+    assert t_g == t_o and dt_g == dt_o
+
+
+def test_step_api_matches_oracle():
+    raw, cfg = channel_case(integrator="RK4")
+    om, s = _pair(raw, cfg)
+    dt = cfg["dt"]
+    for _ in range(5):
+        dt_o, rm_o = om.step(dt)
+        dt_g, rm_g = s.step(dt)
+        assert dt_g == dt_o and rm_g == rm_o
+        dt = dt_o
+    np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
+
+
+def test_run_in_batches_equals_one_run():
+    raw, cfg = channel_case(integrator="RK2_TVD")
+    s1 = Solver(build_mesh(raw), cfg)
+    s2 = Solver(build_mesh(raw), cfg)
+    h1, _, _ = s1.run(30)
+    parts = [s2.run(7)[0], s2.run(11)[0], s2.run(12)[0]]
+    np.testing.assert_array_equal(np.concatenate(parts), h1)
+    np.testing.assert_array_equal(s1.q, s2.q)
+
+
+def test_stops_at_tend():
+    raw, cfg = channel_case(integrator="Euler", dynamicDt=False, dt=1.0e-3, tEnd=0.0105)
+    om, s = _pair(raw, cfg)
+    hist_o, t_o, _ = om.run(10 ** 6)
+    hist_g, t_g, _ = s.run(-1)
+    assert len(hist_g) == len(hist_o) == 11
+    assert t_g == t_o
+    np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
+
+
+def test_nan_edge_reported_like_reference():
+    raw, cfg = channel_case(integrator="Euler")
+    om, s = _pair(raw, cfg)
+    q = s.q
+    q[5, 3] = -5.0          # negative energy: NaN sound speed on that cell's faces
+    s.q = q
+    qq = om.q
+    qq[5, 3] = -5.0
+    om.q = qq
+    with pytest.raises(oracle.OracleError) as eo:
+        om.residual()
+    with pytest.raises(Exception) as eg:
+        s.ufvm_solver()
+    assert eg.value.code == 7
+    assert eg.value.index == eo.value.index
