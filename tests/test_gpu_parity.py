@@ -143,10 +143,10 @@ def test_nan_edge_reported_like_reference():
     raw, cfg = channel_case(integrator="Euler")
     om, s = _pair(raw, cfg)
     q = s.q
-    q[5, 3] = -5.0          # negative energy: NaN sound speed on that cell's faces
+    q[5, 0] = -1.0          # negative density: NaN Roe average on that cell's faces
     s.q = q
     qq = om.q
-    qq[5, 3] = -5.0
+    qq[5, 0] = -1.0
     om.q = qq
     with pytest.raises(oracle.OracleError) as eo:
         om.residual()
