@@ -5,7 +5,10 @@ Workload (default): FP64 Navier-Stokes flow past a cylinder, O-grid of
 2048 azimuthal x 1024 radial cells (geometric stretching 1.006, R=0.5 to 30),
 each quad split into 2 triangles along a seeded diagonal -> 4,194,304 cells;
 M=0.2, Re=100, Rusanov flux, scalar limiter, TVD-RK2 (2 stages per step),
-CFL 0.5.  A step is one full pass of the reference's time loop
+CFL 0.05.  (The reference's CFL time step, solve.d:1092-1094, has no viscous
+limit; with 3.9e-4 wall spacing and nu = 0.01 it diverges within three steps
+at CFL 0.5 (the CPU restatement shows the same) and within a hundred steps
+at CFL 0.1 -- so the case runs at CFL 0.05.)  A step is one full pass of the reference's time loop
 (app/solver.d:119-250): both stages plus RMS residuals, forces, dt/CFL update.
 
 Metric: Mcell-RK-stage updates/s = interior cells * stages * steps / wall time.
@@ -34,11 +37,11 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cylinder_config(M=0.2):
+def cylinder_config(M=0.2, CFL=0.05):
     q = [M, 0.0, 1.0, 1.0]
     return {
         "mesh": "cylinder_ogrid.msh", "dt": 1.0e-6, "tEnd": 1.0e9, "flux": "rusanovFlux", "plotIter": -1,
-        "saveIter": -1, "CFL": 0.5, "integrator": "RK2_TVD", "forceBoundary": "Wall", "limited": True,
+        "saveIter": -1, "CFL": CFL, "integrator": "RK2_TVD", "forceBoundary": "Wall", "limited": True,
         "order": 2, "initialConditions": q, "viscosity": True, "cflAdjust": True,
         "physicalConfig": {"Pr": 0.7, "R": 8.3145, "gamma": 1.4, "Re": 100.0, "L": 1.0},
         "boudaryConditions": [{"tag": "Wall", "type": "viscidWall", "q": q},
@@ -131,7 +134,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r), "cells": N,
                    "edges": F, "flux": "rusanovFlux", "limiter": "scalar (solve.d:360-403)",
-                   "integrator": "RK2_TVD", "stages_per_step": stages, "M": 0.2, "Re": 100, "CFL": 0.5},
+                   "integrator": "RK2_TVD", "stages_per_step": stages, "M": 0.2, "Re": 100, "CFL": cfg["CFL"]},
         "roofline": {"bound": "hbm", "kernel": "k_faces", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_ms": round(face_ms, 4), "algorithmic_bytes_per_launch": int(face_bytes)},

@@ -98,6 +98,7 @@ struct StageArgs {
     double* lenS;          // (F+1): len*sMax per face
     double4* estate;       // optional (F)*2 edge states for .esln, null otherwise
     double* celldt;        // N local time step (localTimestep)
+    double* partial;       // 2 per update workgroup: min dt candidate, max |R|
     DevScalars* sc;
     int kind;              // StageKind
     int limit;             // limiter on for this stage

@@ -314,10 +314,7 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
 {
     if (a.checkActive && !a.sc->active) return;
     const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
-    if (bid == 0 && threadIdx.x == 0) {
-        a.sc->newDtBits = 0x7FF0000000000000ull;   // +inf
-        if (a.first) a.sc->rmaxBits = 0;
-    }
+    if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
     const uint32_t i = bid * BLK + threadIdx.x;
     if (i >= m.N) return;
     double qi[4];
@@ -775,8 +772,36 @@ __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageAr
         double a0 = sMin[0], b0 = sMax_[0];
 #pragma unroll
         for (int k = 1; k < BLK / 64; k++) { a0 = fmin(a0, sMin[k]); b0 = fmax(b0, sMax_[k]); }
-        atomicMin(&a.sc->newDtBits, (unsigned long long)__double_as_longlong(a0));
-        if (b0 > 0) atomicMax(&a.sc->rmaxBits, (unsigned long long)__double_as_longlong(b0));
+        // one partial per workgroup; k_reduce_stage combines them (device-scope
+        // atomics from every workgroup on one address serialise across the XCDs)
+        a.partial[2 * blockIdx.x] = a0;
+        a.partial[2 * blockIdx.x + 1] = b0;
+    }
+}
+
+// newDt = min over the stage's workgroup partials; Rmax = max with the step's running value
+__global__ __launch_bounds__(1024) void k_reduce_stage(const double* __restrict__ partial, uint32_t n,
+                                                       DevScalars* sc, int checkActive)
+{
+    if (checkActive && !sc->active) return;
+    __shared__ double sMin[16], sMax_[16];
+    double a0 = INFINITY, b0 = 0.0;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        a0 = fmin(a0, partial[2 * k]);
+        b0 = fmax(b0, partial[2 * k + 1]);
+    }
+    a0 = warpMin(a0);
+    b0 = warpMax(b0);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { sMin[w] = a0; sMax_[w] = b0; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t k = 1; k < blockDim.x / 64; k++) { a0 = fmin(a0, sMin[k]); b0 = fmax(b0, sMax_[k]); }
+        a0 = fmin(a0, sMin[0]);
+        b0 = fmax(b0, sMax_[0]);
+        sc->newDtBits = (unsigned long long)__double_as_longlong(a0);
+        const double prev = __longlong_as_double((long long)sc->rmaxBits);
+        if (b0 > prev) sc->rmaxBits = (unsigned long long)__double_as_longlong(b0);
     }
 }
 
@@ -788,12 +813,6 @@ __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageAr
 // RMS_CHUNK_BLOCKS blocks level by level; the finalize kernel replays the
 // counter over the chunk trees.  Quantities 0..3: sum of (now-last)^2 per
 // variable; 4..7: sum of now.
-DEV double tree16(const double* r)
-{
-    return (((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7])))
-         + (((r[8] + r[9]) + (r[10] + r[11])) + ((r[12] + r[13]) + (r[14] + r[15])));
-}
-
 DEV void rmsElement(const DiagArgs& d, uint64_t e, double out[8])
 {
     if (e < d.N) {
@@ -822,31 +841,20 @@ __global__ __launch_bounds__(BLK) void k_rms_chunks(DiagArgs d)
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint64_t c0 = uint64_t(blockIdx.x) * RMS_CHUNK_BLOCKS;
     const uint32_t cnt = uint32_t((nb - c0) < RMS_CHUNK_BLOCKS ? (nb - c0) : RMS_CHUNK_BLOCKS);
-    for (int u = 0; u < PER_T; u++) {
-        const uint32_t lb = threadIdx.x + u * BLK;
-        if (lb < cnt) {
-            const uint64_t base = (c0 + lb) * 16;
-            for (int var = 0; var < 4; var++) {
-                double sq[16], nw[16];
+    // 16 consecutive lanes own one 16-element block: the xor butterfly over
+    // 16 lanes adds exactly like the fixed tree (((r0+r1)+(r2+r3))+...)
+    constexpr uint32_t BPI = BLK / 16;
+    for (uint32_t it = 0; it < RMS_CHUNK_BLOCKS / BPI; it++) {
+        const uint32_t lb = it * BPI + threadIdx.x / 16;
+        double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (lb < cnt) rmsElement(d, (c0 + lb) * 16 + (threadIdx.x % 16), o);
 #pragma unroll
-                for (int w = 0; w < 16; w++) {
-                    const uint64_t e = base + w;
-                    if (e < d.N) {
-                        const uint32_t dv = d.perm ? d.perm[e] : uint32_t(e);
-                        const double a0 = reinterpret_cast<const double*>(d.qnew + dv)[var];
-                        const double b0 = reinterpret_cast<const double*>(d.qold + dv)[var];
-                        const double x = a0 - b0;
-                        sq[w] = x * x;
-                        nw[w] = a0;
-                    } else {
-                        sq[w] = 0.0;
-                        nw[w] = 0.0;
-                    }
-                }
-                v[var][lb] = tree16(sq);
-                v[4 + var][lb] = tree16(nw);
-            }
-        }
+        for (int m = 1; m < 16; m <<= 1)
+#pragma unroll
+            for (int q = 0; q < 8; q++) o[q] = o[q] + __shfl_xor(o[q], m, 64);
+        if ((threadIdx.x % 16) == 0 && lb < cnt)
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q][lb] = o[q];
     }
     __syncthreads();
     double* out = d.chunkTrees + size_t(blockIdx.x) * 8 * 16;
@@ -878,23 +886,84 @@ __global__ __launch_bounds__(BLK) void k_rms_chunks(DiagArgs d)
     if (threadIdx.x < 8) d.chunkMask[blockIdx.x * 8 + threadIdx.x] = mask;
 }
 
-__global__ __launch_bounds__(64) void k_finalize(DiagArgs d)
+// Step epilogue (app/solver.d:122-247), one workgroup.  The chunk trees are
+// merged level by level in LDS (a perfect tree over each aligned power-of-two
+// run of chunks is exactly the binary counter's merge), the boundary force is
+// summed in the reference's edge order by one lane from LDS-staged values.
+constexpr int FIN_T = 256;
+constexpr int FIN_CHUNKS = 1024;     // chunk trees merged in LDS per quantity
+constexpr int FIN_FTILE = 2048;      // force faces staged per tile
+
+__global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
 {
     DevScalars* sc = d.sc;
     if (!sc->active) return;
+    __shared__ double v[8][FIN_CHUNKS];
+    __shared__ double trees[8][48];
+    __shared__ int ntree[8];
     __shared__ double tot[8];
-    const int q = threadIdx.x;
-    if (q < 8) {
-        const uint64_t nb = uint64_t(d.L) / 16;
-        const uint64_t nfull = nb / RMS_CHUNK_BLOCKS, part = nb % RMS_CHUNK_BLOCKS;
-        double st[72];
+    const int tid = threadIdx.x;
+    const uint64_t nb = uint64_t(d.L) / 16;
+    const uint64_t nfull = nb / RMS_CHUNK_BLOCKS, part = nb % RMS_CHUNK_BLOCKS;
+    if (tid < 8) ntree[tid] = 0;
+    __syncthreads();
+    if (nfull <= FIN_CHUNKS) {
+        const uint32_t n = uint32_t(nfull);
+        for (uint32_t x = tid; x < 8 * n; x += FIN_T) {
+            const uint32_t q = x / n, k = x % n;
+            v[q][k] = d.chunkTrees[size_t(k) * 128 + q * 16 + 10];
+        }
+        __syncthreads();
+        uint32_t c = n;
+        int level = 0;
+        uint32_t lmask = 0;
+        while (c > 0) {
+            if (c & 1) {
+                if (tid < 8) trees[tid][47 - level] = v[tid][c - 1];
+                lmask |= 1u << level;
+            }
+            const uint32_t half = c / 2;
+            double tmp[(8 * FIN_CHUNKS / 2 + FIN_T - 1) / FIN_T];
+            int m = 0;
+            for (uint32_t x = tid; x < 8 * half; x += FIN_T) {
+                const uint32_t q = x / half, jj = x % half;
+                tmp[m++] = v[q][2 * jj] + v[q][2 * jj + 1];
+            }
+            __syncthreads();
+            m = 0;
+            for (uint32_t x = tid; x < 8 * half; x += FIN_T) {
+                const uint32_t q = x / half, jj = x % half;
+                v[q][jj] = tmp[m++];
+            }
+            __syncthreads();
+            c = half;
+            level++;
+        }
+        // chunk-level trees, largest first
+        if (tid < 8) {
+            int t = 0;
+            for (int l = 31; l >= 0; l--)
+                if (lmask & (1u << l)) trees[tid][t++] = trees[tid][47 - l];
+            ntree[tid] = t;
+        }
+    } else if (tid < 8) {
+        // very large meshes: replay the binary counter over the chunk trees directly
+        double st[48];
         int idx = 0;
         for (uint64_t k = 0; k < nfull; k++) {
-            st[idx] = d.chunkTrees[k * 128 + q * 16 + 10];
+            st[idx] = d.chunkTrees[k * 128 + tid * 16 + 10];
             const int last = idx - __builtin_ctzll(k + 1);
             while (idx > last) { st[idx - 1] += st[idx]; --idx; }
             ++idx;
         }
+        for (int t = 0; t < idx; t++) trees[tid][t] = st[t];
+        ntree[tid] = idx;
+    }
+    __syncthreads();
+    if (tid < 8) {
+        const int q = tid;
+        int idx = ntree[q];
+        double* st = trees[q];
         if (part) {
             const uint32_t mask = d.chunkMask[nfull * 8 + q];
             for (int l = 15; l >= 0; l--)
@@ -912,23 +981,37 @@ __global__ __launch_bounds__(64) void k_finalize(DiagArgs d)
         double s = 0.0;
         if (idx > 0) {
             s = st[idx - 1];
-            for (int j = idx - 2; j >= 0; j--) s += st[j];
+            for (int j2 = idx - 2; j2 >= 0; j2--) s += st[j2];
         }
         tot[q] = 0.0 + s;
     }
+    // boundary force, in bGroups order (mesh.d:645-671)
+    double f0 = 0, f1 = 0;
+    double* fb = &v[0][0];
+    for (uint32_t k0 = 0; k0 < d.nForce; k0 += FIN_FTILE) {
+        const uint32_t cnt = (d.nForce - k0) < uint32_t(FIN_FTILE) ? (d.nForce - k0) : uint32_t(FIN_FTILE);
+        __syncthreads();
+        for (uint32_t x = tid; x < cnt; x += FIN_T) {
+            const uint32_t f = d.forceFaces[k0 + x];
+            fb[x] = d.lenF[size_t(f) * 4 + 1];
+            fb[FIN_FTILE + x] = d.lenF[size_t(f) * 4 + 2];
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll 8
+            for (uint32_t x = 0; x < cnt; x++) {
+                f0 += fb[x];
+                f1 += fb[FIN_FTILE + x];
+            }
+        }
+    }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (tid != 0) return;
     double rms[4];
-    for (int v = 0; v < 4; v++) rms[v] = sqrt(double(d.N) * tot[v]) / tot[4 + v];
+    for (int q = 0; q < 4; q++) rms[q] = sqrt(double(d.N) * tot[q]) / tot[4 + q];
     const double newDt = __longlong_as_double((long long)sc->newDtBits);
     const double Rmax = __longlong_as_double((long long)sc->rmaxBits);
     if (d.dynamicDt) sc->dt = newDt;
-    double f0 = 0, f1 = 0;
-    for (uint32_t k = 0; k < d.nForce; k++) {
-        const uint32_t f = d.forceFaces[k];
-        f0 += d.lenF[size_t(f) * 4 + 1];
-        f1 += d.lenF[size_t(f) * 4 + 2];
-    }
     const double ld0 = 0.0 + d.rot[0] * f0 + d.rot[1] * f1;
     const double ld1 = 0.0 + d.rot[2] * f0 + d.rot[3] * f1;
     const double residMax = dmax(dmax(dmax(rms[0], rms[1]), rms[2]), rms[3]);
@@ -944,7 +1027,7 @@ __global__ __launch_bounds__(64) void k_finalize(DiagArgs d)
         h[0] = sc->t; h[1] = sc->dt; h[2] = rms[0]; h[3] = rms[1]; h[4] = rms[2]; h[5] = rms[3];
         h[6] = Rmax; h[7] = ld0; h[8] = ld1; h[9] = sc->CFL;
     }
-    for (int v = 0; v < 4; v++) sc->rms[v] = rms[v];
+    for (int q = 0; q < 4; q++) sc->rms[q] = rms[q];
     sc->force[0] = ld0; sc->force[1] = ld1;
     sc->t += sc->dt;
     sc->steps++;
@@ -1050,6 +1133,8 @@ void launchUpdate(const DevMesh& m, const PhysParams& p, const StageArgs& a, int
     if (S == 3) updS<3>(m, p, a, st);
     else if (S == 4) updS<4>(m, p, a, st);
     else updS<6>(m, p, a, st);
+    const uint32_t g = nblk(m.N);
+    if (g) hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, g, a.sc, a.checkActive);
 }
 
 void launchRmsChunks(const DiagArgs& d, hipStream_t st)
@@ -1061,7 +1146,7 @@ void launchRmsChunks(const DiagArgs& d, hipStream_t st)
 
 void launchFinalize(const DiagArgs& d, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_T), 0, st, d);
 }
 
 void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStream_t st)

@@ -132,6 +132,7 @@ struct ebb_solver {
     // device buffers
     DBuf<uint32_t> d_nbr, d_cface, d_refId, d_forceFaces;
     DBuf<double> d_gm, d_nia, d_cd, d_cper, d_flen, d_grad, d_lim, d_lenF, d_lenS, d_celldt, d_hist, d_chunkTrees;
+    DBuf<double> d_partial;
     DBuf<double2> d_cen, d_fn, d_fmid;
     DBuf<uint2> d_fcell;
     DBuf<BFace> d_bface;
@@ -190,7 +191,7 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
     StageArgs a{};
     a.qin = qin; a.qn = qn; a.qout = qout; a.acc = s->d_acc.p; a.R = R;
     a.grad = s->d_grad.p; a.lim = s->d_lim.p; a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p;
-    a.estate = estate; a.celldt = s->d_celldt.p; a.sc = s->d_sc;
+    a.estate = estate; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
     a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
     a.checkActive = checkActive; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = stepId;
     launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
@@ -459,6 +460,7 @@ int buildDevice(ebb_solver* s)
     HIPCHK((hipError_t)trackAlloc(s, s->d_lenF, size_t(F + 1) * 4));
     HIPCHK((hipError_t)trackAlloc(s, s->d_lenS, size_t(F + 1)));
     if (c.localTimestep) HIPCHK((hipError_t)trackAlloc(s, s->d_celldt, N));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_partial, 2 * size_t((N + 255) / 256) + 2));
     HIPCHK((hipError_t)trackAlloc(s, s->d_hist, size_t(s->histCap) * HIST_FIELDS));
     const uint64_t nb = uint64_t(C) / 16;
     const uint64_t nchunks = (nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
@@ -1095,7 +1097,7 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
         for (auto& x : seq) {
             StageArgs a{};
             a.qin = x.in; a.qn = q; a.qout = x.out; a.acc = s->d_acc.p; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
-            a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.sc = s->d_sc;
+            a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
             a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
             a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
             launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
@@ -1144,7 +1146,7 @@ void ebb_solver_destroy(ebb_solver* s)
     s->d_nbr.release(); s->d_cface.release(); s->d_refId.release(); s->d_forceFaces.release();
     s->d_gm.release(); s->d_nia.release(); s->d_cd.release(); s->d_cper.release(); s->d_flen.release();
     s->d_grad.release(); s->d_lim.release(); s->d_lenF.release(); s->d_lenS.release(); s->d_celldt.release();
-    s->d_hist.release(); s->d_chunkTrees.release(); s->d_chunkMask.release();
+    s->d_hist.release(); s->d_chunkTrees.release(); s->d_chunkMask.release(); s->d_partial.release();
     s->d_cen.release(); s->d_fn.release(); s->d_fmid.release(); s->d_fcell.release(); s->d_bface.release();
     s->d_src.release(); s->d_qA.release(); s->d_qB.release(); s->d_T.release(); s->d_T2.release();
     s->d_acc.release(); s->d_R.release(); s->d_estate.release();

@@ -1,0 +1,89 @@
+"""Manufactured-solution refinement study on the GPU against the reference's
+published results.
+
+The reference's study (app/ms.d:214-361) runs forward Euler with the MMS source
+to steady state (max_i,k (R+S) <= 1e-10) on the ms_refinement fixtures and
+reports L2 errors of rho at the centroids (app/ms.d:162-188); the numbers were
+pasted into ms_refinement/plot_conv.py:12-20 and the observed orders printed in
+Documentation/writeup2/TheoryGuide2.tex:177-192.  Each family below reruns
+that study through the HIP path (same meshes, configs and stop criterion) and
+compares the L2 values and the quoted order.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ebbcfd_amd import Solver, import_mesh
+
+from helpers import MMS_DIR
+
+pytestmark = pytest.mark.gpu
+
+# plot_conv.py:12-20 (L2 of rho per refinement level) and the level pair each order is quoted on
+PUBLISHED = {
+    "euler_quads": ([0.00059068, 0.000113051, 2.26156e-05, 5.11884e-06], (2, 3), 2.1434),
+    "euler_tris": ([0.00101603, 0.000277795, 6.37774e-05, 1.4947e-05], (2, 3), 2.0932),
+    "euler_tris_pert": ([0.00053851, 0.000121805, 2.67446e-05, 4.97857e-05], (1, 2), 2.2193),
+    "euler_tris_limited": ([0.00234712, 0.000548344, 0.000139978, 3.56414e-05], (2, 3), 1.9736),
+    "euler_tris_lp": ([0.00242227, 0.000586065, 0.000140727], (1, 2), 2.0582),
+    "euler_quads_limited": ([0.00187271, 0.000435096, 0.000120633, 2.90233e-05], (2, 3), 2.0553),
+    "euler_quads_lp": ([0.00180033, 0.000422567, 0.000120477, 2.90644e-05], (2, 3), 2.0514),
+}
+# mesh family, config base, overrides
+FAMILY = {
+    "euler_quads": ("ms_refine_%d", "ms_refine_%d", {"limited": False, "lpThresh": -1}),
+    "euler_tris": ("ms_refine_tri_%d", "ms_refine_tri_%d", {"limited": False, "lpThresh": -1}),
+    "euler_tris_pert": ("ms_refine_tri_pert_%d", "ms_refine_tri_pert_%d", {}),
+    "euler_tris_limited": ("ms_refine_tri_%d", "ms_refine_tri_%d", {"limited": True, "lpThresh": -1}),
+    "euler_tris_lp": ("ms_refine_tri_%d", "ms_refine_tri_%d", {}),
+    "euler_quads_limited": ("ms_refine_%d", "ms_refine_%d", {"limited": True, "lpThresh": -1}),
+    "euler_quads_lp": ("ms_refine_%d", "ms_refine_%d", {}),
+}
+DX = {"ms_refine_%d": [1 / 4, 1 / 8, 1 / 16, 1 / 32],
+      "ms_refine_tri_%d": [1 / np.sqrt(32), 1 / np.sqrt(128), 1 / np.sqrt(512), 1 / np.sqrt(2048)],
+      "ms_refine_tri_pert_%d": [0.0270466, 0.0134827, 0.00680914, 0.00367054]}   # plot_conv.py:6-8
+
+
+def steady_l2(mesh_name, cfg_name, over, max_steps=600000, chunk=20000):
+    cfg = json.load(open(os.path.join(MMS_DIR, cfg_name + ".json")))
+    cfg.update(over)
+    cfg["extensions"] = {"mms": True}
+    cfg["plotIter"] = -1
+    cfg["saveIter"] = -1
+    mesh = import_mesh(os.path.join(MMS_DIR, mesh_name + ".msh"))
+    s = Solver(mesh, cfg)
+    steps, rmax = 0, np.inf
+    while steps < max_steps:
+        h, _, _ = s.run(chunk)
+        steps += len(h)
+        R, _, _ = s.ufvm_solver(limit=bool(cfg["limited"]), dt_update=False)
+        rmax = max(0.0, float(R.max()))          # app/ms.d:345-352: signed max from 0
+        if rmax <= 1e-10:
+            break
+    g = mesh.cell_geometry()
+    import oracle
+    exact = np.array([oracle.solution(x, y) for x, y in g["centroid"][: mesh.n_interior]])
+    err = np.abs(exact - s.q)
+    return np.sqrt((err ** 2).sum(axis=0) / mesh.n_interior), rmax, steps
+
+
+@pytest.mark.parametrize("family", sorted(PUBLISHED))
+def test_mms_refinement_matches_published(family):
+    l2_pub, (a, b), order_pub = PUBLISHED[family]
+    mname, cname, over = FAMILY[family]
+    res = [steady_l2(mname % k, cname % k, over) for k in range(len(l2_pub))]
+    l2 = np.array([r[0][0] for r in res])
+    print(family, "L2", l2, "published", l2_pub, "rmax", [r[1] for r in res], "steps", [r[2] for r in res])
+    dx = DX[mname]
+    order = np.log(l2[a] / l2[b]) / np.log(dx[a] / dx[b])
+    assert abs(order - order_pub) < 0.15, (order, order_pub)
+    # magnitudes: the published study ran an earlier revision of the code (app/ms.d no longer
+    # compiles against source/solve.d); the restatement lands within 0.8-1.45x of every published
+    # value except plot_conv.py's pert level 3, which grows under refinement (an unconverged run)
+    ratio = l2 / np.array(l2_pub)
+    keep = np.ones(len(l2), bool)
+    if family == "euler_tris_pert":
+        keep[3] = False
+    assert np.all((ratio[keep] > 0.8) & (ratio[keep] < 1.45)), ratio

@@ -1,0 +1,134 @@
+"""Host side of the C ABI, no GPU needed: the library loads and exports every
+symbol include/ebbcfd.h declares; the C++ gmsh reader and geometry precompute
+agree bit for bit with the oracle; the config reader keeps the reference's
+keys, types and defaults (source/config.d:112-313)."""
+import glob
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle.gmsh import parse_gmsh
+from ebbcfd_amd import _lib, build_mesh, import_mesh, load_config, meshgen
+from ebbcfd_amd.solver import Config
+
+from helpers import MMS_DIR, ROOT
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "ebbcfd.h")).read()
+    declared = set(re.findall(r"\b(ebb_\w+)\s*\(", header))
+    assert declared == set(_lib.SYMBOLS), declared ^ set(_lib.SYMBOLS)
+    L = _lib.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def _compare_geometry(mesh, om):
+    g, og = mesh.cell_geometry(), om.cells()
+    N = om.n_interior
+    for k in ("area", "d", "perim", "centroid"):
+        np.testing.assert_array_equal(g[k], og[k], err_msg=k)
+    np.testing.assert_array_equal(g["gradMat"], og["gradMat"][:N])
+    np.testing.assert_array_equal(g["edges"], og["edges"][:N])
+    np.testing.assert_array_equal(g["fm"], og["fm"][:N].astype(np.int8))
+    np.testing.assert_array_equal(g["nbrs"], og["nbrs"][:N])
+    e, oe = mesh.edge_geometry(), om.edges()
+    for k in e:
+        np.testing.assert_array_equal(e[k], oe[k], err_msg=k)
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(MMS_DIR, "*.msh"))), ids=os.path.basename)
+def test_gmsh_reader_and_geometry_match_oracle(path):
+    m = import_mesh(path)
+    om = oracle.Mesh(parse_gmsh(path))
+    assert (m.n_cells, m.n_interior, m.n_edges) == (om.n_cells, om.n_interior, om.n_edges)
+    _compare_geometry(m, om)
+
+
+@pytest.mark.parametrize("maker", ["mms", "cyl", "bump"])
+def test_generated_mesh_roundtrip_through_gmsh_file(maker, tmp_path):
+    raw = {"mms": lambda: meshgen.mms_square(10, 0.1, 4), "cyl": lambda: meshgen.cylinder_ogrid(32, 12, stretch=1.1),
+           "bump": lambda: meshgen.channel_bump(18, 6, 0.05, 2)}[maker]()
+    path = str(tmp_path / "m.msh")
+    meshgen.write_gmsh(raw, path)
+    a, b = import_mesh(path), build_mesh(raw)
+    om = oracle.Mesh(raw)
+    _compare_geometry(a, om)
+    _compare_geometry(b, om)
+
+
+def test_negative_area_rejected():
+    raw = meshgen.mms_square(4, 0.0, 1)
+    raw.elem_nodes = raw.elem_nodes.reshape(-1, 3)[:, ::-1].reshape(-1).copy()   # clockwise
+    with pytest.raises(Exception):
+        build_mesh(raw)
+
+
+REF_CFG = json.load(open(os.path.join(MMS_DIR, "ms_refine_tri_0.json")))
+
+
+def test_config_reference_fixture():
+    c = load_config(json.dumps(REF_CFG))
+    assert c["dt"] == 1e-5 and c["tEnd"] == 10000 and c["CFL"] == 0.2 and c["lpThresh"] == 1
+    assert c["limited"] == 1 and c["order"] == 2 and c["viscosity"] == 0 and c["cflAdjust"] == 0
+    assert c["integrator"] == 0 and c["flux"] == 1 and c["dynamicDt"] == 1 and c["localTimestep"] == 0
+    assert c["Pr"] == 0.71 and c["R"] == 1 and c["Re"] == 1000
+    assert c.mesh_file == "ms_refine_tri_0.msh"
+
+
+def test_config_defaults():
+    minimal = {k: REF_CFG[k] for k in ("mesh", "dt", "tEnd", "flux", "plotIter", "saveIter", "forceBoundary",
+                                       "initialConditions", "boudaryConditions")}
+    c = load_config(json.dumps(minimal))
+    # config.d:137-260
+    assert c["CFL"] == 0.5 and c["dynamicDt"] == 1 and c["integrator"] == 0 and c["limited"] == 1
+    assert c["localTimestep"] == 0 and c["lpThresh"] == -1 and c["order"] == 2 and c["multistageLimiting"] == 1
+    assert c["cflAdjust"] == 1 and c["viscosity"] == 0
+    assert c["Pr"] == 0.7 and c["R"] == 8.3145 and c["Re"] == 1000 and c["L"] == 1
+
+
+def test_config_type_rules():
+    bad = dict(REF_CFG)
+    bad["dt"] = 1          # dt must be a JSON float (config.d:119)
+    with pytest.raises(Exception):
+        load_config(json.dumps(bad))
+    bad = dict(REF_CFG)
+    bad["saveIter"] = 10.0
+    with pytest.raises(Exception):
+        load_config(json.dumps(bad))
+    bad = dict(REF_CFG)
+    del bad["boudaryConditions"]
+    bad["boundaryConditions"] = REF_CFG["boudaryConditions"]     # the reference's key is misspelt
+    with pytest.raises(Exception):
+        load_config(json.dumps(bad))
+    odd = dict(REF_CFG)
+    odd["order"] = 3       # invalid order falls back to 2 (config.d:200-204)
+    assert load_config(json.dumps(odd))["order"] == 2
+
+
+def test_config_set_get_roundtrip():
+    c = Config(REF_CFG)
+    c["CFL"] = 0.3
+    c["maxSteps"] = 17
+    assert c["CFL"] == 0.3 and c["maxSteps"] == 17
+
+
+def test_solver_needs_a_device_or_fails_loudly():
+    """No silent CPU fallback: without a HIP device the solver refuses to start."""
+    import ctypes
+    if ctypes.CDLL("libamdhip64.so").hipGetDeviceCount is None:  # pragma: no cover
+        pytest.skip("no HIP runtime")
+    from ebbcfd_amd import Solver
+    n = ctypes.c_int(0)
+    ctypes.CDLL("libamdhip64.so").hipGetDeviceCount(ctypes.byref(n))
+    if n.value > 0:
+        pytest.skip("a device is present")
+    raw = meshgen.mms_square(4, 0.0, 1)
+    cfg = dict(REF_CFG, extensions={"mms": True})
+    with pytest.raises(Exception) as e:
+        Solver(build_mesh(raw), cfg)
+    assert "device" in str(e.value)
