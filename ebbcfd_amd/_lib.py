@@ -17,6 +17,7 @@ SYMBOLS = (
     "ebb_config_mesh_file", "ebb_config_destroy", "ebb_solver_create", "ebb_state_upload", "ebb_state_download",
     "ebb_state_download_all", "ebb_residual", "ebb_step", "ebb_run", "ebb_run_state", "ebb_run_reset",
     "ebb_history", "ebb_history_length", "ebb_edge_states", "ebb_cell_gradients", "ebb_save_solution",
+    "ebb_save_solution_as",
     "ebb_load_solution", "ebb_write_forces", "ebb_last_error", "ebb_time_steps", "ebb_solver_device_bytes",
     "ebb_solver_destroy",
 )
@@ -72,6 +73,7 @@ def lib():
     L.ebb_edge_states.argtypes = [vp, P(d), P(d), P(d)]
     L.ebb_cell_gradients.argtypes = [vp, P(d), P(d)]
     L.ebb_save_solution.argtypes = [vp, C.c_char_p]
+    L.ebb_save_solution_as.argtypes = [vp, C.c_char_p, d, d]
     L.ebb_load_solution.argtypes = [vp, C.c_char_p]
     L.ebb_write_forces.argtypes = [vp, C.c_char_p, i64, i64, C.c_int]
     L.ebb_last_error.argtypes = [vp, P(C.c_int), P(i64), P(i64), C.c_char_p, C.c_size_t]

@@ -106,7 +106,8 @@ int main(int argc, char** argv)
         if (save > 0 && ran > 0 && (it - 1) % save == 0) {
             char name[512];
             std::snprintf(name, sizeof(name), "save_%lld_0.esln", (long long)saveItr++);
-            ebb_save_solution(s, name);
+            // saved before t += dt in the reference loop (app/solver.d:232-247)
+            ebb_save_solution_as(s, name, h[0], h[1]);
         }
         if (ran < want) break;   // t >= tEnd or maxSteps reached
     }

@@ -75,6 +75,7 @@ struct Config {
 
 // mesh.cpp
 bool parseGmsh(const std::string& path, RawMesh& out, std::string& err);
+bool parseGri(const std::string& path, RawMesh& out, std::string& err);
 bool buildMesh(const RawMesh& raw, BuiltMesh& out, std::string& err);
 
 // config.cpp

@@ -218,6 +218,78 @@ bool parseGmsh(const std::string& path, RawMesh& out, std::string& err)
     return true;
 }
 
+// XFlow .gri reader (source/io.d:992-1121): "nNodes nElems 2", node lines,
+// boundary groups "nFaces nodesPerFace [tag]" whose faces keep their first and
+// last node, element groups "count q TriLagrange|QuadLagrange" with q = 1..3
+// (corner nodes picked from the higher-order node lists).  No orientation fix is
+// applied (the reference applies it to gmsh meshes only).  The reference gives
+// every cell the edge count of the last element group, so it cannot read a mesh
+// that mixes triangles and quads; such a file is rejected here.
+bool parseGri(const std::string& path, RawMesh& out, std::string& err)
+{
+    std::ifstream f(path);
+    if (!f) { err = "cannot open mesh file " + path; return false; }
+    std::string line;
+    std::vector<std::string> tok;
+    auto next = [&]() -> bool {
+        while (std::getline(f, line)) {
+            tok = cleanLine(line);
+            if (!tok.empty()) return true;
+        }
+        return false;
+    };
+    out = RawMesh();
+    if (!next() || tok.size() < 3) { err = "bad .gri header"; return false; }
+    const long nNodes = std::stol(tok[0]), nElems = std::stol(tok[1]), nDims = std::stol(tok[2]);
+    if (nDims != 2) { err = std::to_string(nDims) + " dimensional meshes not supported"; return false; }
+    for (long i = 0; i < nNodes; i++) {
+        if (!next() || tok.size() < 2) { err = "Unexpected end of file in nodes"; return false; }
+        out.nodes.push_back(std::strtod(tok[0].c_str(), nullptr));
+        out.nodes.push_back(std::strtod(tok[1].c_str(), nullptr));
+    }
+    if (!next()) { err = "missing boundary group count"; return false; }
+    const long nGroups = std::stol(tok[0]);
+    for (long g = 0; g < nGroups; g++) {
+        if (!next() || tok.size() < 2) { err = "bad boundary group header"; return false; }
+        const long nf = std::stol(tok[0]);
+        if (tok.size() == 3) out.btags.push_back(tok[2]);
+        out.bgroupStart.push_back(out.bnodes.size() / 2);
+        for (long k = 0; k < nf; k++) {
+            if (!next() || tok.size() < 2) { err = "Unexpected end of file in boundary faces"; return false; }
+            out.bnodes.push_back(uint32_t(std::stol(tok.front()) - 1));
+            out.bnodes.push_back(uint32_t(std::stol(tok.back()) - 1));
+        }
+    }
+    out.eoff.assign(1, 0);
+    long found = 0;
+    int faces = 0, firstFaces = 0;
+    while (found < nElems) {
+        if (!next() || tok.size() < 3) { err = "bad element group header"; return false; }
+        const long sub = std::stol(tok[0]);
+        const long q = std::stol(tok[1]);
+        if (q >= 4 || q == 0) { err = "Unsuported q"; return false; }
+        if (tok[2].find("Tri") != std::string::npos) faces = 3;
+        else if (tok[2].find("Quad") != std::string::npos) faces = 4;
+        else { err = "Unsuported cell type"; return false; }
+        if (firstFaces == 0) firstFaces = faces;
+        if (faces != firstFaces) { err = "mixed triangle/quad .gri meshes are not supported"; return false; }
+        for (long i = 0; i < sub; i++) {
+            if (!next()) { err = "Unexpected end of file in elements"; return false; }
+            std::vector<uint32_t> e;
+            for (auto& t : tok) e.push_back(uint32_t(std::stol(t)));
+            std::vector<uint32_t> c;
+            if (q == 1) c = e;
+            else if (q == 2) c = faces == 3 ? std::vector<uint32_t>{e[0], e[2], e[5]} : std::vector<uint32_t>{e[0], e[2], e[8], e[6]};
+            else c = faces == 3 ? std::vector<uint32_t>{e[0], e[3], e[9]} : std::vector<uint32_t>{e[0], e[3], e[15], e[12]};
+            if (int(c.size()) != faces) { err = "element has the wrong number of nodes"; return false; }
+            out.enodes.insert(out.enodes.end(), c.begin(), c.end());
+            out.eoff.push_back(uint32_t(out.enodes.size()));
+        }
+        found += sub;
+    }
+    return true;
+}
+
 bool buildMesh(const RawMesh& raw, BuiltMesh& m, std::string& err)
 {
     m = BuiltMesh();

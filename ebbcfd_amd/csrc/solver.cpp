@@ -526,14 +526,16 @@ int ebb_mesh_load(const char* path, ebb_mesh** out)
 {
     if (!path || !out) return EBB_ERR_ARG;
     std::string p(path);
-    if (p.find(".msh") == std::string::npos && p.find(".gmsh") == std::string::npos) return EBB_ERR_PARSE;
+    const bool gri = p.find(".gri") != std::string::npos;   // io.d:673-684 dispatch order
+    if (!gri && p.find(".msh") == std::string::npos && p.find(".gmsh") == std::string::npos) return EBB_ERR_PARSE;
     auto m = std::make_unique<ebb_mesh>();
     std::string err;
     {
         std::ifstream f(p);
         if (!f) return EBB_ERR_IO;
     }
-    if (!parseGmsh(p, m->raw, err)) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_PARSE; }
+    const bool ok = gri ? parseGri(p, m->raw, err) : parseGmsh(p, m->raw, err);
+    if (!ok) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_PARSE; }
     if (!buildMesh(m->raw, m->b, err)) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_MESH; }
     *out = m.release();
     return EBB_OK;
@@ -950,7 +952,17 @@ int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
 }
 
 // source/io.d:68-161
-int ebb_save_solution(ebb_solver* s, const char* path)
+static int saveSolutionImpl(ebb_solver* s, const char* path, const double* tdt);
+
+int ebb_save_solution(ebb_solver* s, const char* path) { return saveSolutionImpl(s, path, nullptr); }
+
+int ebb_save_solution_as(ebb_solver* s, const char* path, double t, double dt)
+{
+    const double tdt[2] = {t, dt};
+    return saveSolutionImpl(s, path, tdt);
+}
+
+static int saveSolutionImpl(ebb_solver* s, const char* path, const double* tdt)
 {
     if (!s || !path) return EBB_ERR_ARG;
     HIPCHK(hipSetDevice(s->device));
@@ -973,8 +985,8 @@ int ebb_save_solution(ebb_solver* s, const char* path)
     putBE32(buf, 0xEA98E1F5u);
     putBE32(buf, 2);
     putBE32(buf, N);
-    putBEd(buf, s->h_sc->t);
-    putBEd(buf, s->h_sc->dt);
+    putBEd(buf, tdt ? tdt[0] : s->h_sc->t);
+    putBEd(buf, tdt ? tdt[1] : s->h_sc->dt);
     putBE32(buf, 4);
     putBE32(buf, uint32_t(s->cfg.order));
     for (uint32_t i = 0; i < N; i++) {

@@ -170,3 +170,43 @@ def write_gmsh(raw, path):
             k += 1
         f.write("".join(out))
         f.write("$EndElements\n")
+
+
+def write_gri(raw, path, q=1):
+    """Write a single-element-type RawMesh as an XFlow .gri file (the layout read by
+    source/io.d:992-1121).  q=2 writes quadratic triangles with edge-midpoint nodes."""
+    nodes = np.asarray(raw.nodes)
+    offs = np.asarray(raw.elem_offsets)
+    nv = np.diff(offs)
+    assert np.all(nv == nv[0]), ".gri needs one element type"
+    el = np.asarray(raw.elem_nodes).reshape(-1, int(nv[0]))
+    extra = []
+    if q == 2:
+        assert nv[0] == 3
+        mids = {}
+        base = nodes.shape[0]
+        hi = []
+        for t in el:
+            a, b, c = (int(v) for v in t)
+            def mid(u, w):
+                key = (min(u, w), max(u, w))
+                if key not in mids:
+                    mids[key] = base + len(extra) + 1
+                    extra.append(0.5 * (nodes[u - 1] + nodes[w - 1]))
+                return mids[key]
+            # q=2 triangle node order: corners at positions 0, 2, 5
+            hi.append([a, mid(a, b), b, mid(a, c), mid(b, c), c])
+        el = np.array(hi)
+    allnodes = np.vstack([nodes] + ([np.array(extra)] if extra else []))
+    starts = list(np.asarray(raw.bgroup_start, dtype=np.int64)) + [raw.bnodes.shape[0]]
+    kind = "TriLagrange" if nv[0] == 3 else "QuadLagrange"
+    with open(path, "w") as f:
+        f.write("%d %d 2\n" % (allnodes.shape[0], el.shape[0]))
+        f.write("".join("%.17g %.17g\n" % (x, y) for x, y in allnodes))
+        f.write("%d\n" % len(raw.btags))
+        for g, tag in enumerate(raw.btags):
+            f.write("%d 2 %s\n" % (starts[g + 1] - starts[g], tag))
+            for b in range(starts[g], starts[g + 1]):
+                f.write("%d %d\n" % (raw.bnodes[b, 0] + 1, raw.bnodes[b, 1] + 1))
+        f.write("%d %d %s\n" % (el.shape[0], q, kind))
+        f.write("".join(" ".join(str(int(v)) for v in row) + "\n" for row in el))

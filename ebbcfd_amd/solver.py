@@ -218,8 +218,11 @@ class Solver:
         _check(lib().ebb_cell_gradients(self.h, _p(g), _p(l)), "cell gradients", self.h)
         return g, l
 
-    def save_solution(self, path):
-        _check(lib().ebb_save_solution(self.h, str(path).encode()), "saveSolution", self.h)
+    def save_solution(self, path, t=None, dt=None):
+        if t is None:
+            _check(lib().ebb_save_solution(self.h, str(path).encode()), "saveSolution", self.h)
+        else:
+            _check(lib().ebb_save_solution_as(self.h, str(path).encode(), float(t), float(dt)), "saveSolution", self.h)
 
     def load_solution(self, path):
         _check(lib().ebb_load_solution(self.h, str(path).encode()), "loadSolution", self.h)

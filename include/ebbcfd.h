@@ -41,7 +41,8 @@ const char* ebb_error_string(int code);
 
 /* ---------------------------------------------------------------- mesh */
 
-/* importMesh + buildMesh: gmsh 2.2 ASCII reader (source/io.d:669-990) followed
+/* importMesh + buildMesh: gmsh 2.2 ASCII (source/io.d:689-990) or XFlow .gri
+ * (source/io.d:992-1121) reader chosen by file name (io.d:669-687), followed
  * by the geometry precompute (source/mesh.d:217-643). */
 int ebb_mesh_load(const char* path, ebb_mesh** out);
 
@@ -141,6 +142,9 @@ int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim);
  * big-endian, edge states of the last residual evaluation.  Loading restores
  * the interior state and the time loop's t and dt. */
 int ebb_save_solution(ebb_solver* s, const char* path);
+/* The same with an explicit header time and time step: the reference's periodic
+ * saves run before t += dt (app/solver.d:232-247), its final save after. */
+int ebb_save_solution_as(ebb_solver* s, const char* path, double t, double dt);
 int ebb_load_solution(ebb_solver* s, const char* path);
 
 /* Append the boundary-force records (t, drag, lift; big-endian f64) of history

@@ -133,3 +133,47 @@ def parse_gmsh(path):
         btags=btags,
         bgroup_start=np.array(starts, dtype=np.uint64),
     )
+
+
+def parse_gri(path):
+    """Restatement of the XFlow .gri reader (source/io.d:992-1121)."""
+    with open(path) as f:
+        lines = [ln.split() for ln in f if ln.strip()]
+    pos = 0
+    nn, ne, nd = (int(v) for v in lines[pos][:3]); pos += 1
+    assert nd == 2
+    nodes = [[float(v) for v in lines[pos + i][:2]] for i in range(nn)]
+    pos += nn
+    ng = int(lines[pos][0]); pos += 1
+    bnodes, starts, tags = [], [], []
+    for _ in range(ng):
+        hdr = lines[pos]; pos += 1
+        nf = int(hdr[0])
+        if len(hdr) == 3:
+            tags.append(hdr[2])
+        starts.append(len(bnodes))
+        for k in range(nf):
+            bn = [int(v) for v in lines[pos + k]]
+            bnodes.append([bn[0] - 1, bn[-1] - 1])
+        pos += nf
+    elements, found = [], 0
+    while found < ne:
+        hdr = lines[pos]; pos += 1
+        sub, q = int(hdr[0]), int(hdr[1])
+        faces = 3 if "Tri" in hdr[2] else 4
+        for k in range(sub):
+            e = [int(v) for v in lines[pos + k]]
+            if q == 1:
+                elements.append(e)
+            elif q == 2:
+                elements.append([e[0], e[2], e[5]] if faces == 3 else [e[0], e[2], e[8], e[6]])
+            else:
+                elements.append([e[0], e[3], e[9]] if faces == 3 else [e[0], e[3], e[15], e[12]])
+        pos += sub
+        found += sub
+    offs = np.zeros(len(elements) + 1, dtype=np.uint32)
+    offs[1:] = np.cumsum([len(e) for e in elements])
+    return SimpleNamespace(nodes=np.array(nodes, np.float64).reshape(-1, 2), elem_offsets=offs,
+                           elem_nodes=np.array([v for e in elements for v in e], np.uint32),
+                           bnodes=np.array(bnodes, np.uint32).reshape(-1, 2), btags=tags,
+                           bgroup_start=np.array(starts, np.uint64))

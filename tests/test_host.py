@@ -132,3 +132,18 @@ def test_solver_needs_a_device_or_fails_loudly():
     with pytest.raises(Exception) as e:
         Solver(build_mesh(raw), cfg)
     assert "device" in str(e.value)
+
+
+@pytest.mark.parametrize("q", [1, 2])
+def test_gri_reader_matches_oracle(q, tmp_path):
+    from oracle.gmsh import parse_gri
+    raw = meshgen.channel_bump(14, 5, 0.05, 3)
+    path = str(tmp_path / "m.gri")
+    meshgen.write_gri(raw, path, q=q)
+    m = import_mesh(path)
+    ro = parse_gri(path)
+    om = oracle.Mesh(ro)
+    _compare_geometry(m, om)
+    # same geometry as the mesh it was written from (q=2 only adds edge midpoints)
+    ob = oracle.Mesh(raw)
+    np.testing.assert_array_equal(om.cells()["area"], ob.cells()["area"])
