@@ -27,7 +27,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes (FP64, triangles): per interior cell-stage for the whole
-# residual+update, and per launch unit for the face kernel (dominant)
+# residual+update (the fused tile kernel's unit), and per launch unit for the
+# split path's face kernel
 STAGE_BYTES_PER_CELL = 268.0
 FACE_BYTES_PER_FACE = 88.0      # owner/other ids 8 + normal 16 + midpoint 16 + length 8 + len*flux,len*sMax 40
 FACE_BYTES_PER_CELL = 144.0     # state 32 + gradient 64 + limiter 32 + centroid 16, each cell read once
@@ -61,6 +62,17 @@ def cpu_baseline(raw, cfg, steps, n_cells):
     return n_cells * 2 * steps / el / 1e6, el
 
 
+def copy_calibration():
+    """A plain device-to-device copy of 2^27 doubles (1 GiB read + 1 GiB written), five times."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copied %d bytes 5 times" % (x.numel() * 8))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,7 +82,17 @@ def main():
     ap.add_argument("--n-r", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--time-steps", type=int, default=20, help="steps timed per kernel with HIP events")
+    ap.add_argument("--tile-cells", type=int, default=0,
+                    help="0: split gradient/face/update kernels; n: fused stage kernel with tiles of up to n cells; "
+                         "-1: fused kernel, automatic tile size")
+    ap.add_argument("--tile-order", default="grow", choices=("grow", "morton"),
+                    help="how the fused kernel's tiles are cut: greedily grown regions or Morton order")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver keeps the whole mesh on a single MI355X "
               "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
@@ -81,6 +103,7 @@ def main():
     log("generating O-grid %d x %d" % (args.n_theta, args.n_r))
     raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1)
     cfg = cylinder_config()
+    cfg["extensions"] = {"tileCells": args.tile_cells, "tileOrder": args.tile_order}
     mesh = build_mesh(raw)
     N = mesh.n_interior
     log("mesh built: %d cells, %d edges" % (N, mesh.n_edges))
@@ -101,14 +124,18 @@ def main():
 
     # per-kernel device time with HIP events on the solver stream
     tk = s.time_steps(args.time_steps)
-    face_ms = tk["faces_ms"]
     F = mesh.n_edges
-    face_bytes = F * FACE_BYTES_PER_FACE + N * FACE_BYTES_PER_CELL
-    achieved = face_bytes / (face_ms * 1e-3) / 1e9
     stage_ms = tk["step_ms"] / stages
     stage_gbs = N * STAGE_BYTES_PER_CELL / (stage_ms * 1e-3) / 1e9
-    log("kernels: gradient %.3f ms, faces %.3f ms, update %.3f ms, step %.3f ms"
-        % (tk["gradient_ms"], face_ms, tk["update_ms"], tk["step_ms"]))
+    log("kernels: fused %.3f ms | gradient %.3f ms, faces %.3f ms, update %.3f ms | stages of a step %.3f ms"
+        % (tk["fused_ms"], tk["gradient_ms"], tk["faces_ms"], tk["update_ms"], tk["step_ms"]))
+    if tk["fused_ms"] > 0:
+        kernel, launch_ms = "k_stage_tile", tk["fused_ms"]
+        launch_bytes = N * STAGE_BYTES_PER_CELL
+    else:
+        kernel, launch_ms = "k_faces", tk["faces_ms"]
+        launch_bytes = F * FACE_BYTES_PER_FACE + N * FACE_BYTES_PER_CELL
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
 
     cpu = None
     if args.cpu_steps > 0:
@@ -135,9 +162,9 @@ def main():
         "config": {"workload": "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r), "cells": N,
                    "edges": F, "flux": "rusanovFlux", "limiter": "scalar (solve.d:360-403)",
                    "integrator": "RK2_TVD", "stages_per_step": stages, "M": 0.2, "Re": 100, "CFL": cfg["CFL"]},
-        "roofline": {"bound": "hbm", "kernel": "k_faces", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_ms": round(face_ms, 4), "algorithmic_bytes_per_launch": int(face_bytes)},
+                     "avg_launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(launch_bytes)},
         "stage_roofline": {"bytes_per_cell_stage": STAGE_BYTES_PER_CELL, "stage_ms": round(stage_ms, 4),
                            "achieved": round(stage_gbs, 1), "frac": round(stage_gbs / HBM_PEAK_GBS, 4),
                            "unit": "GB/s"},

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libebbcfd.so")
 # every symbol include/ebbcfd.h declares
 SYMBOLS = (
     "ebb_error_string", "ebb_mesh_load", "ebb_mesh_create", "ebb_mesh_info", "ebb_mesh_cell_geometry",
-    "ebb_mesh_edge_geometry", "ebb_mesh_destroy", "ebb_config_parse", "ebb_config_get", "ebb_config_set",
+    "ebb_mesh_edge_geometry", "ebb_mesh_tile_stats", "ebb_mesh_destroy", "ebb_config_parse", "ebb_config_get", "ebb_config_set",
     "ebb_config_mesh_file", "ebb_config_destroy", "ebb_solver_create", "ebb_state_upload", "ebb_state_download",
     "ebb_state_download_all", "ebb_residual", "ebb_step", "ebb_run", "ebb_run_state", "ebb_run_reset",
     "ebb_history", "ebb_history_length", "ebb_edge_states", "ebb_cell_gradients", "ebb_save_solution",
@@ -50,6 +50,7 @@ def lib():
     L.ebb_mesh_info.argtypes = [vp, P(i64)]
     L.ebb_mesh_cell_geometry.argtypes = [vp, P(d), P(d), P(d), P(d), P(d), P(u32), P(C.c_int8), P(u32), P(C.c_uint8)]
     L.ebb_mesh_edge_geometry.argtypes = [vp, P(u32), P(u32), P(d), P(d), P(d), P(C.c_int32)]
+    L.ebb_mesh_tile_stats.argtypes = [vp, C.c_int, C.c_int, P(d)]
     L.ebb_mesh_destroy.argtypes = [vp]
     L.ebb_mesh_destroy.restype = None
     L.ebb_config_parse.argtypes = [C.c_char_p, P(vp)]

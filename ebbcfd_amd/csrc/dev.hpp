@@ -111,7 +111,52 @@ struct StageArgs {
     uint32_t stepId;
 };
 
+// Tiles for the fused stage kernel: each tile is a compact group of interior
+// cells plus the halo it needs: local cells [own | ring 1 | ring 2 | ghosts of
+// own + ring 1].  Gradients are computed for own + ring 1, fluxes for every face
+// of an own cell, updates for own cells; faces shared by two tiles are computed
+// by both, identically.  Everything a tile reads besides the states is packed
+// into one contiguous blob that is copied into LDS as is (layout: tileImage).
+struct TileArgs {
+    uint32_t nTiles;
+    uint32_t nBlocks;              // persistent workgroups, each walking a contiguous run of tiles
+    uint32_t ldsBytes;             // LDS per workgroup (largest tile image)
+    const uint4* desc;             // 2 per tile: (nC, nG, nQ, nL), (nF, blob offset, blob length, 0) in 16 B;
+                                   // each blob starts with the descriptor of the tile after it
+    const uint4* blob;
+};
+
+// Byte offsets of a tile's LDS image; [0, blobEnd) is the packed blob.
+struct TileImg {
+    uint32_t hdr, cs, gm, fs, met, fc, ids, nb, cf, gown, blobEnd, qs, gs, ls, end;
+};
+__host__ __device__ inline TileImg tileImage(uint32_t nC, uint32_t nG, uint32_t nQ, uint32_t nL, uint32_t nF, int S,
+                                             int LW)
+{
+    auto al = [](uint32_t x) { return (x + 15u) & ~15u; };
+    TileImg I;
+    uint32_t o = 0;
+    I.hdr = o; o += 32;                             // the next tile's descriptor (2 x uint4)
+    I.cs = o; o = al(o + 16 * nL);                  // centroids (double2)
+    I.gm = o; o = al(o + 8 * 2 * S * nG);           // LSQ weights, (dir, nbr)-major over own + ring 1
+    I.fs = o; o = al(o + 40 * (nF + 1));            // faces: normal, midpoint, length; then len*flux, len*sMax
+    I.met = o; o = al(o + 24 * nC);                 // own cells: perimeter, d, -1/area
+    I.fc = o; o = al(o + 8 * nF);                   // faces: (global face id, local a | local b << 16)
+    I.ids = o; o = al(o + 4 * nL);                  // global cell ids
+    I.nb = o; o = al(o + 2 * S * nG);               // local neighbours of own + ring 1
+    I.cf = o; o = al(o + 2 * S * nC);               // own cells' local faces, bit 15: not the owner
+    I.gown = o; o = al(o + 2 * (nL - nQ));          // ghosts: local index of the owner cell
+    I.blobEnd = o;
+    I.qs = o; o = al(o + 32 * nL);                  // states
+    I.gs = o; o = al(o + 64 * nG);                  // gradients (var, dir)
+    I.ls = o; o = al(o + 8 * LW * nG);              // limiters
+    I.end = o;
+    return I;
+}
+
 // Launchers (kernels.hip).  All run on `stream`.
+void launchStageTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const TileArgs& t, int S,
+                      hipStream_t stream);
 void launchGradient(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t stream);
 void launchFaces(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t stream);
 void launchUpdate(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t stream);
@@ -135,6 +180,7 @@ struct DiagArgs {
     DevScalars* sc;
 };
 constexpr int HIST_FIELDS = 10;
+constexpr uint32_t TILE_BLOB_MAX = 8 * 256 * 16;   // largest tile blob the fused kernel prefetches
 constexpr uint32_t RMS_CHUNK_BLOCKS = 1024;     // 16-element blocks per workgroup
 void launchRmsChunks(const DiagArgs& d, hipStream_t stream);
 void launchFinalize(const DiagArgs& d, hipStream_t stream);

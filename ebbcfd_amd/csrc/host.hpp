@@ -63,6 +63,8 @@ struct Config {
     bool mms = false;                 // manufactured solution: dirichlet = exact, source per stage
     long long maxSteps = -1;          // stop after this many steps (-1: only tEnd)
     int precision = 0;                // 0 = fp64, 1 = mixed fp32 flux/reconstruction/limiter
+    long long tileCells = 0;          // cells per tile of the fused stage kernel (0: split kernels, -1: automatic)
+    int tileOrder = 0;                // 0: greedily grown regions, 1: Morton order of centroids
     std::vector<std::string> warnings;
     int fluxKind() const { return flux == "roeFlux" ? RoeFlux : RusanovFlux; }
     int integratorKind() const {

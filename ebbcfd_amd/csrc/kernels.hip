@@ -307,63 +307,42 @@ DEV void lpSolve(int rows, const double* A, const double* b, const double c[2], 
 }
 
 // ---------------------------------------------------------------- gradient + limiter
-// MODE 0: ghost states only (order 1); 1: unlimited gradient; 2: scalar limiter;
-// 3: scalar limiter followed by the LP vector limiter (lim stored as x,y pairs).
+// solve.d:324-497 for one interior cell, given its own state, its S neighbour
+// states (ghosts already filled), LSQ weights, centroids and which slots are real
+// neighbours (padding slots repeat the cell).  MODE 1: unlimited; 2: scalar
+// limiter; 3: scalar limiter then the LP vector limiter.
 template <int S, int MODE>
-__global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, StageArgs a)
+DEV void gradLimit(const double qi[4], const double qn[S][4], const double* gm, uint32_t gstride, double2 ci,
+                   const double2 cn[S], const bool real[S], int limit, double lpThresh, double g[4][2], double lx[4],
+                   double ly[4])
 {
-    if (a.checkActive && !a.sc->active) return;
-    const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
-    if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
-    const uint32_t i = bid * BLK + threadIdx.x;
-    if (i >= m.N) return;
-    double qi[4];
-    ld4(a.qin, i, qi);
     double mn[4], mx[4], du[4][S];
-    uint32_t nb[S];
 #pragma unroll
     for (int k = 0; k < 4; k++) { mn[k] = qi[k]; mx[k] = qi[k]; }
 #pragma unroll
-    for (int j = 0; j < S; j++) {
-        nb[j] = m.nbr[i * S + j];
-        double qn[4];
-        if (nb[j] >= m.N) {
-            double old[4] = {0, 0, 0, 0};
-            const BFace& bf = m.bface[nb[j] - m.N];
-            if (bf.type == BT_TPIN) ld4(a.qin, nb[j], old);
-            ghostState(bf, qi, qn, old);
-            st4(a.qin, nb[j], qn);
-        } else {
-            ld4(a.qin, nb[j], qn);
-        }
+    for (int j = 0; j < S; j++)
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            mn[k] = fmin(mn[k], qn[k]);
-            mx[k] = fmax(mx[k], qn[k]);
-            du[k][j] = qn[k] - qi[k];
+            mn[k] = fmin(mn[k], qn[j][k]);
+            mx[k] = fmax(mx[k], qn[j][k]);
+            du[k][j] = qn[j][k] - qi[k];
         }
-    }
-    if (MODE == 0) return;
-    double g[4][2];
-    const double* gm = m.gm + size_t(i) * 2 * S;
 #pragma unroll
     for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int r = 0; r < 2; r++) {
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < S; j++) s += gm[r * S + j] * du[k][j];
+            for (int j = 0; j < S; j++) s += gm[(r * S + j) * gstride] * du[k][j];
             g[k][r] = s;
         }
-    double lx[4] = {1.0, 1.0, 1.0, 1.0};
-    double ly[4] = {1.0, 1.0, 1.0, 1.0};
-    if (MODE >= 2 && a.limit) {
-        const double2 ci = m.cen[i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) { lx[k] = 1.0; ly[k] = 1.0; }
+    if (MODE >= 2 && limit) {
         double dxs[S], dys[S];
 #pragma unroll
         for (int j = 0; j < S; j++) {
-            const double2 cn = m.cen[nb[j]];
-            const double dx = cn.x - ci.x, dy = cn.y - ci.y;
+            const double dx = cn[j].x - ci.x, dy = cn[j].y - ci.y;
             dxs[j] = dx; dys[j] = dy;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -376,12 +355,12 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
                 ly[k] = lx[k];
             }
         }
-        if (MODE == 3 && p.lpThresh > 0) {
+        if (MODE == 3 && lpThresh > 0) {
             int nn = 0;
 #pragma unroll
-            for (int j = 0; j < S; j++) nn += (nb[j] != i) ? 1 : 0;
+            for (int j = 0; j < S; j++) nn += real[j] ? 1 : 0;
             for (int k = 0; k < 4; k++) {
-                if (lx[k] < p.lpThresh && (nn == 3 || nn == 4)) {
+                if (lx[k] < lpThresh && (nn == 3 || nn == 4)) {
                     double A[24], bb[12];
                     for (int r = 0; r < 24; r++) A[r] = 0.0;
                     for (int r = 0; r < 12; r++) bb[r] = 0.0;
@@ -389,7 +368,7 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
                     bb[2] = -1; bb[3] = -1;
                     int c2 = 0;
                     for (int j = 0; j < S; j++) {
-                        if (nb[j] == i) continue;
+                        if (!real[j]) continue;
                         A[2 * (c2 + 4)] = dxs[j] * g[k][0];
                         A[2 * (c2 + 4) + 1] = dys[j] * g[k][1];
                         bb[c2 + 4] = mn[k] - qi[k];
@@ -407,6 +386,47 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
             }
         }
     }
+}
+
+// the owner's ghost state for boundary record b, with TempPresInflow's untouched components
+DEV void ghostFromOwner(const BFace& bf, const double qo[4], const double4* qbuf, uint32_t gIdx, double qg[4])
+{
+    double old[4] = {0, 0, 0, 0};
+    if (bf.type == BT_TPIN) ld4(qbuf, gIdx, old);
+    ghostState(bf, qo, qg, old);
+}
+
+// MODE 0: ghost states only (order 1); 1: unlimited gradient; 2: scalar limiter;
+// 3: scalar limiter followed by the LP vector limiter (lim stored as x,y pairs).
+template <int S, int MODE>
+__global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, StageArgs a)
+{
+    if (a.checkActive && !a.sc->active) return;
+    const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
+    if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
+    const uint32_t i = bid * BLK + threadIdx.x;
+    if (i >= m.N) return;
+    double qi[4];
+    ld4(a.qin, i, qi);
+    double qn[S][4];
+    uint32_t nb[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        nb[j] = m.nbr[i * S + j];
+        if (nb[j] >= m.N) {
+            ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qn[j]);
+            st4(a.qin, nb[j], qn[j]);
+        } else {
+            ld4(a.qin, nb[j], qn[j]);
+        }
+    }
+    if (MODE == 0) return;
+    double2 cn[S];
+    bool real[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) { cn[j] = m.cen[nb[j]]; real[j] = nb[j] != i; }
+    double g[4][2], lx[4], ly[4];
+    gradLimit<S, MODE>(qi, qn, m.gm + size_t(i) * 2 * S, 1, m.cen[i], cn, real, a.limit, p.lpThresh, g, lx, ly);
     double* gout = a.grad + size_t(i) * 8;
     *reinterpret_cast<double4*>(gout) = make_double4(g[0][0], g[0][1], g[1][0], g[1][1]);
     *reinterpret_cast<double4*>(gout + 4) = make_double4(g[2][0], g[2][1], g[3][0], g[3][1]);
@@ -456,6 +476,166 @@ DEV void reconstruct(const double qm[4], const double g[8], const double lx[4], 
     }
 }
 
+// solve.d:970-1033: one interior face from both cells' raw data
+template <int FLUX, int VISC, int ORDER, int LP>
+DEV void interiorFace(const double qa[4], const double ga[8], const double lxa[4], const double lya[4], double2 ca,
+                      const double qb[4], const double gb[8], const double lxb[4], const double lyb[4], double2 cb,
+                      double2 n, double2 mid, const PhysParams& p, double qL[4], double qR[4], double F[4],
+                      double& sMax)
+{
+    if (ORDER > 1) {
+        reconstruct<LP>(qa, ga, lxa, lya, mid.x - ca.x, mid.y - ca.y, qL);
+        reconstruct<LP>(qb, gb, lxb, lyb, mid.x - cb.x, mid.y - cb.y, qR);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { qL[k] = qa[k]; qR[k] = qb[k]; }
+    }
+    numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+    if (VISC) {
+        double qa2[4], dq[8], Fv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) qa2[k] = 0.5 * (qL[k] + qR[k]);
+#pragma unroll
+        for (int k = 0; k < 8; k++) dq[k] = 0.5 * (ga[k] + gb[k]);
+        viscousFlux(p.Pr, p.mu, qa2, dq, n.x, n.y, Fv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+    }
+}
+
+// solve.d:283-319 + 769-968: one boundary face from its owner's data and ghost state
+template <int FLUX, int VISC, int ORDER, int LP>
+DEV void boundaryFace(const double qo[4], const double gO[8], const double lx[4], const double ly[4], double2 co,
+                      const double qg[4], const BFace& bf, double2 n, double2 mid, const PhysParams& p,
+                      double qL[4], double qR[4], double F[4], double& sMax)
+{
+    // updateBoundaryEdge: the fallback uses lim[0] for both directions
+    if (ORDER > 1) {
+        const double dx = mid.x - co.x, dy = mid.y - co.y;
+#pragma unroll
+        for (int j = 0; j < 4; j++) qL[j] = qo[j] + lx[j] * gO[2 * j] * dx + ly[j] * gO[2 * j + 1] * dy;
+        if (pressure(qL) < 0) {
+            double l0 = 1.0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) l0 = fmin(l0, lx[j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) qL[j] = qo[j] + l0 * gO[2 * j] * dx + l0 * gO[2 * j + 1] * dy;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) qL[j] = qo[j];
+    }
+    const int type = bf.type;
+    if (type == BT_FULL || type == BT_DIRICHLET) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) qR[k] = bf.far[k];
+        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+        if (VISC) {
+            double qa[4], dq[8], Fv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+            if (type == BT_DIRICHLET) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + bf.gg[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + 0.0);
+            }
+            viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
+#pragma unroll
+            for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+        }
+    } else if (type == BT_INVWALL) {
+        const double inv = 1 / qL[0];
+        const double v0 = inv * qL[1], v1 = inv * qL[2];
+        const double vn = v0 * n.x + v1 * n.y;
+        const double t0 = v0 - vn * n.x, t1 = v1 - vn * n.y;
+        const double vel = sqrt(t0 * t0 + t1 * t1);
+        double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
+        const double aw = sqrt(G_ * (pw / qL[0]));
+        if (pw < 0) pw = 1.0e-12;
+        F[0] = 0; F[1] = pw * n.x; F[2] = pw * n.y; F[3] = 0;
+        sMax = fabs(aw);
+        for (int k = 0; k < 4; k++) qR[k] = qg[k];
+    } else if (type == BT_VISCWALL) {
+        const double vel = 0.0;
+        double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
+        const double aw = sqrt(G_ * (pw / qL[0]));
+        if (pw < 0) pw = 1.0e-12;
+        qR[0] = qL[0]; qR[1] = qL[1] * -1; qR[2] = qL[2] * -1; qR[3] = qL[3];
+        double qa[4], dq[8], Fv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+        qa[1] = 0.0; qa[2] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) dq[k] = gO[k];
+        dq[6] = (qa[3] / qa[0]) * dq[0];
+        dq[7] = (qa[3] / qa[0]) * dq[1];
+        viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
+        F[0] = 0 - Fv[0]; F[1] = pw * n.x - Fv[1]; F[2] = pw * n.y - Fv[2]; F[3] = 0 - Fv[3];
+        sMax = fabs(aw);
+    } else if (type == BT_CONSTP) {
+        for (int k = 0; k < 4; k++) qR[k] = qg[k];
+        const double pg = pressure(qR);
+        const double ug = qR[1] / qR[0], vg = qR[2] / qR[0];
+        const double ag = sqrt(G_ * (pg / qR[0]));
+        convective(pg, ug, vg, qR[0], qR[3], n.x, n.y, F);
+        sMax = fabs(ag);
+    } else if (type == BT_SYM) {
+        const double v0 = qL[1] / qL[0], v1 = qL[2] / qL[0];
+        double w0, w1;
+        if (!bf.vertical) {
+            w0 = 0.0 + bf.R[0] * v0 + bf.R[1] * v1;
+            w1 = 0.0 + bf.R[2] * v0 + bf.R[3] * v1;
+        } else {
+            w0 = -v0;
+            w1 = v1;
+        }
+        qR[0] = qL[0]; qR[1] = qL[0] * w0; qR[2] = qL[0] * w1; qR[3] = qL[3];
+        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+        if (VISC) {
+            double qa[4], bg[8], Fv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+            const double o00 = n.x * n.x, o01 = n.x * n.y, o10 = n.y * n.x, o11 = n.y * n.y;
+            const double V00 = 1 - o00, V01 = 0 - o01, V10 = 0 - o10, V11 = 1 - o11;
+            const double T00 = 1 - 2 * o00, T01 = 0 - 2 * o01, T10 = 0 - 2 * o10, T11 = 1 - 2 * o11;
+            // tmp1 = A*gradp with A = identity except the momentum block V (4x4 * 4x2, all columns)
+            double t1[8];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                t1[0 + c] = 0.0 + 1.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 0.0 * gO[6 + c];
+                t1[2 + c] = 0.0 + 0.0 * gO[0 + c] + V00 * gO[2 + c] + V01 * gO[4 + c] + 0.0 * gO[6 + c];
+                t1[4 + c] = 0.0 + 0.0 * gO[0 + c] + V10 * gO[2 + c] + V11 * gO[4 + c] + 0.0 * gO[6 + c];
+                t1[6 + c] = 0.0 + 0.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 1.0 * gO[6 + c];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const double s30 = 0.0 + t1[2 * r] * T00 + t1[2 * r + 1] * T10;
+                const double s31 = 0.0 + t1[2 * r] * T01 + t1[2 * r + 1] * T11;
+                const double s10 = 0.0 + gO[2 * r] * o00 + gO[2 * r + 1] * o10;
+                const double s11 = 0.0 + gO[2 * r] * o01 + gO[2 * r + 1] * o11;
+                bg[2 * r] = s10 + s30;
+                bg[2 * r + 1] = s11 + s31;
+            }
+            viscousFlux(p.Pr, p.mu, qa, bg, n.x, n.y, Fv);
+#pragma unroll
+            for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+        }
+    } else {  // TempPresInflow
+        for (int k = 0; k < 4; k++) qR[k] = qg[k];
+        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+        if (VISC) {
+            double qa[4], Fv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+            viscousFlux(p.Pr, p.mu, qa, gO, n.x, n.y, Fv);
+#pragma unroll
+            for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
+        }
+    }
+}
+
 template <int FLUX, int VISC, int ORDER, int LP, int STORE>
 __global__ __launch_bounds__(BLK) void k_faces(DevMesh m, PhysParams p, StageArgs a, uint32_t nbi)
 {
@@ -468,175 +648,35 @@ __global__ __launch_bounds__(BLK) void k_faces(DevMesh m, PhysParams p, StageArg
         f = xcdBlock(blockIdx.x, nbi) * BLK + threadIdx.x;
         if (f >= m.Fi) return;
         const uint2 c = m.fcell[f];
-        const double2 n = m.fn[f];
         len = m.flen[f];
-        double gL[8], gR[8];
-        ld4(a.qin, c.x, qL);
-        ld4(a.qin, c.y, qR);
-        if (ORDER > 1 || VISC) { loadGrad(a.grad, c.x, gL); loadGrad(a.grad, c.y, gR); }
+        double qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
+        ld4(a.qin, c.x, qa);
+        ld4(a.qin, c.y, qb);
+        if (ORDER > 1 || VISC) { loadGrad(a.grad, c.x, ga); loadGrad(a.grad, c.y, gb); }
+        double2 ca = make_double2(0, 0), cb = make_double2(0, 0), mid = make_double2(0, 0);
         if (ORDER > 1) {
-            const double2 mid = m.fmid[f];
-            const double2 cL = m.cen[c.x], cR = m.cen[c.y];
-            double lx[4], ly[4], tL[4], tR[4];
-            loadLim<LP>(a.lim, c.x, lx, ly);
-            reconstruct<LP>(qL, gL, lx, ly, mid.x - cL.x, mid.y - cL.y, tL);
-            loadLim<LP>(a.lim, c.y, lx, ly);
-            reconstruct<LP>(qR, gR, lx, ly, mid.x - cR.x, mid.y - cR.y, tR);
-#pragma unroll
-            for (int k = 0; k < 4; k++) { qL[k] = tL[k]; qR[k] = tR[k]; }
+            loadLim<LP>(a.lim, c.x, lxa, lya);
+            loadLim<LP>(a.lim, c.y, lxb, lyb);
+            ca = m.cen[c.x]; cb = m.cen[c.y]; mid = m.fmid[f];
         }
-        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
-        if (VISC) {
-            double qa[4], dq[8], Fv[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-#pragma unroll
-            for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gL[k] + gR[k]);
-            viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
-#pragma unroll
-            for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
-        }
+        interiorFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, m.fn[f], mid, p, qL, qR, F, sMax);
         errKey = uint64_t(m.Fb) + f;   // boundary edges are checked first in the reference
     } else {
         const uint32_t b = (blockIdx.x - nbi) * BLK + threadIdx.x;
         if (b >= m.Fb) return;
         f = m.Fi + b;
-        const uint32_t o = m.fcell[f].x, gc = m.N + b;
-        const double2 n = m.fn[f];
+        const uint32_t o = m.fcell[f].x;
         len = m.flen[f];
-        const BFace& bf = m.bface[b];
-        double gO[8];
-        double qo[4];
+        double qo[4], qg[4], gO[8], lx[4], ly[4];
         ld4(a.qin, o, qo);
+        ld4(a.qin, m.N + b, qg);
         if (ORDER > 1 || VISC) loadGrad(a.grad, o, gO);
-        // solve.d:283-319 updateBoundaryEdge: the fallback uses lim[0] for both directions
+        double2 co = make_double2(0, 0), mid = make_double2(0, 0);
         if (ORDER > 1) {
-            const double2 mid = m.fmid[f], co = m.cen[o];
-            const double dx = mid.x - co.x, dy = mid.y - co.y;
-            double lx[4], ly[4];
             loadLim<LP>(a.lim, o, lx, ly);
-#pragma unroll
-            for (int j = 0; j < 4; j++) qL[j] = qo[j] + lx[j] * gO[2 * j] * dx + ly[j] * gO[2 * j + 1] * dy;
-            if (pressure(qL) < 0) {
-                double l0 = 1.0;
-#pragma unroll
-                for (int j = 0; j < 4; j++) l0 = fmin(l0, lx[j]);
-#pragma unroll
-                for (int j = 0; j < 4; j++) qL[j] = qo[j] + l0 * gO[2 * j] * dx + l0 * gO[2 * j + 1] * dy;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; j++) qL[j] = qo[j];
+            co = m.cen[o]; mid = m.fmid[f];
         }
-        const int type = bf.type;
-        if (type == BT_FULL || type == BT_DIRICHLET) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) qR[k] = bf.far[k];
-            numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
-            if (VISC) {
-                double qa[4], dq[8], Fv[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-                if (type == BT_DIRICHLET) {
-#pragma unroll
-                    for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + bf.gg[k]);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + 0.0);
-                }
-                viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
-#pragma unroll
-                for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
-            }
-        } else if (type == BT_INVWALL) {
-            const double inv = 1 / qL[0];
-            const double v0 = inv * qL[1], v1 = inv * qL[2];
-            const double vn = v0 * n.x + v1 * n.y;
-            const double t0 = v0 - vn * n.x, t1 = v1 - vn * n.y;
-            const double vel = sqrt(t0 * t0 + t1 * t1);
-            double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
-            const double aw = sqrt(G_ * (pw / qL[0]));
-            if (pw < 0) pw = 1.0e-12;
-            F[0] = 0; F[1] = pw * n.x; F[2] = pw * n.y; F[3] = 0;
-            sMax = fabs(aw);
-            ld4(a.qin, gc, qR);
-        } else if (type == BT_VISCWALL) {
-            const double vel = 0.0;
-            double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
-            const double aw = sqrt(G_ * (pw / qL[0]));
-            if (pw < 0) pw = 1.0e-12;
-            qR[0] = qL[0]; qR[1] = qL[1] * -1; qR[2] = qL[2] * -1; qR[3] = qL[3];
-            double qa[4], dq[8], Fv[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-            qa[1] = 0.0; qa[2] = 0.0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) dq[k] = gO[k];
-            dq[6] = (qa[3] / qa[0]) * dq[0];
-            dq[7] = (qa[3] / qa[0]) * dq[1];
-            viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
-            F[0] = 0 - Fv[0]; F[1] = pw * n.x - Fv[1]; F[2] = pw * n.y - Fv[2]; F[3] = 0 - Fv[3];
-            sMax = fabs(aw);
-        } else if (type == BT_CONSTP) {
-            ld4(a.qin, gc, qR);
-            const double pg = pressure(qR);
-            const double ug = qR[1] / qR[0], vg = qR[2] / qR[0];
-            const double ag = sqrt(G_ * (pg / qR[0]));
-            convective(pg, ug, vg, qR[0], qR[3], n.x, n.y, F);
-            sMax = fabs(ag);
-        } else if (type == BT_SYM) {
-            const double v0 = qL[1] / qL[0], v1 = qL[2] / qL[0];
-            double w0, w1;
-            if (!bf.vertical) {
-                w0 = 0.0 + bf.R[0] * v0 + bf.R[1] * v1;
-                w1 = 0.0 + bf.R[2] * v0 + bf.R[3] * v1;
-            } else {
-                w0 = -v0;
-                w1 = v1;
-            }
-            qR[0] = qL[0]; qR[1] = qL[0] * w0; qR[2] = qL[0] * w1; qR[3] = qL[3];
-            numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
-            if (VISC) {
-                double qa[4], bg[8], Fv[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-                const double o00 = n.x * n.x, o01 = n.x * n.y, o10 = n.y * n.x, o11 = n.y * n.y;
-                const double V00 = 1 - o00, V01 = 0 - o01, V10 = 0 - o10, V11 = 1 - o11;
-                const double T00 = 1 - 2 * o00, T01 = 0 - 2 * o01, T10 = 0 - 2 * o10, T11 = 1 - 2 * o11;
-                // tmp1 = A*gradp with A = identity except the momentum block V (4x4 * 4x2, all columns)
-                double t1[8];
-#pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    t1[0 + c] = 0.0 + 1.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 0.0 * gO[6 + c];
-                    t1[2 + c] = 0.0 + 0.0 * gO[0 + c] + V00 * gO[2 + c] + V01 * gO[4 + c] + 0.0 * gO[6 + c];
-                    t1[4 + c] = 0.0 + 0.0 * gO[0 + c] + V10 * gO[2 + c] + V11 * gO[4 + c] + 0.0 * gO[6 + c];
-                    t1[6 + c] = 0.0 + 0.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 1.0 * gO[6 + c];
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const double s30 = 0.0 + t1[2 * r] * T00 + t1[2 * r + 1] * T10;
-                    const double s31 = 0.0 + t1[2 * r] * T01 + t1[2 * r + 1] * T11;
-                    const double s10 = 0.0 + gO[2 * r] * o00 + gO[2 * r + 1] * o10;
-                    const double s11 = 0.0 + gO[2 * r] * o01 + gO[2 * r + 1] * o11;
-                    bg[2 * r] = s10 + s30;
-                    bg[2 * r + 1] = s11 + s31;
-                }
-                viscousFlux(p.Pr, p.mu, qa, bg, n.x, n.y, Fv);
-#pragma unroll
-                for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
-            }
-        } else {  // TempPresInflow
-            ld4(a.qin, gc, qR);
-            numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
-            if (VISC) {
-                double qa[4], Fv[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-                viscousFlux(p.Pr, p.mu, qa, gO, n.x, n.y, Fv);
-#pragma unroll
-                for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
-            }
-        }
+        boundaryFace<FLUX, VISC, ORDER, LP>(qo, gO, lx, ly, co, qg, m.bface[b], m.fn[f], mid, p, qL, qR, F, sMax);
         errKey = b;
     }
     if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
@@ -663,11 +703,120 @@ DEV double warpMax(double v)
     return v;
 }
 
+// solve.d:1092-1117 + the integrator's stage update for interior cell i, from
+// its summed face fluxes r and sum of len*sMax
+// met: (perimeter, d, -1/area) of the cell, or null to read them from the mesh
+template <int KIND>
+DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t i, const double r[4],
+                   double sAve, const double* qinI, const double* met, double& dtCand, double& rCand)
+{
+    const double cper = met ? met[0] : m.cper[i], cd = met ? met[1] : m.cd[i], nia = met ? met[2] : m.nia[i];
+    sAve /= cper;
+    const double CFL = a.sc->CFL;
+    const double dtc = (CFL * cd) / sAve;
+    if (!isnan(dtc)) dtCand = fmin(dtCand, dtc);
+    double hc = 0.0;
+    if (p.localTimestep) {
+        if (a.dtUpdate) {
+            hc = CFL * cd / sAve;
+            if (isnan(hc)) hc = 0;
+            a.celldt[i] = hc;
+        } else {
+            hc = a.celldt[i];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) if (fabs(r[k]) > rCand) rCand = fabs(r[k]);
+    const double s = nia;
+    double R[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) R[k] = r[k] * s;
+    if (m.src) {
+        const double4 sv = m.src[i];
+        R[0] = R[0] + sv.x; R[1] = R[1] + sv.y; R[2] = R[2] + sv.z; R[3] = R[3] + sv.w;
+    }
+    const double dt = a.sc->dt;
+    const double h = p.localTimestep ? hc : dt;
+    double qn[4], o[4];
+    if (KIND != SK_RESIDUAL) ld4(a.qn, i, qn);
+    if (KIND == SK_RESIDUAL) {
+        st4(a.R, i, R);
+    } else if (KIND == SK_EULER) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
+        st4(a.qout, i, o);
+    } else if (KIND == SK_RK2_1) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = qn[k] + (dt * R[k]);
+        st4(a.acc, i, R);
+        st4(a.qout, i, o);
+    } else if (KIND == SK_RK2_2) {
+        double k1[4];
+        ld4(a.acc, i, k1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 2.0) * (k1[k] + R[k]);
+        st4(a.qout, i, o);
+    } else if (KIND == SK_TVD_1) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
+        st4(a.qout, i, o);
+    } else if (KIND == SK_TVD_2) {
+        double fe[4];
+        if (qinI) { fe[0] = qinI[0]; fe[1] = qinI[1]; fe[2] = qinI[2]; fe[3] = qinI[3]; }
+        else ld4(a.qin, i, fe);
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = 0.5 * (qn[k] + fe[k] + h * R[k]);
+        st4(a.qout, i, o);
+    } else if (KIND == SK_RK4_1) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = qn[k] + ((dt / 2.0) * R[k]);
+        st4(a.acc, i, R);
+        st4(a.qout, i, o);
+    } else if (KIND == SK_RK4_2 || KIND == SK_RK4_3) {
+        double ac[4];
+        ld4(a.acc, i, ac);
+#pragma unroll
+        for (int k = 0; k < 4; k++) ac[k] = ac[k] + 2.0 * R[k];
+        const double c = (KIND == SK_RK4_2) ? (dt / 2.0) : dt;
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = qn[k] + (c * R[k]);
+        st4(a.acc, i, ac);
+        st4(a.qout, i, o);
+    } else if (KIND == SK_RK4_4) {
+        double ac[4];
+        ld4(a.acc, i, ac);
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 6.0) * (ac[k] + R[k]);
+        st4(a.qout, i, o);
+    }
+    if (KIND != SK_RESIDUAL && a.last && a.recordErrors &&
+        (isnan(o[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3])))
+        atomicMin(&a.sc->errCell, (uint64_t(a.stepId) << 32) | m.refId[i]);
+}
+
+// per-workgroup min dt / max |R| partial, combined later by k_reduce_stage (device-scope
+// atomics from every workgroup on one address serialise across the XCDs)
+DEV void blockPartial(double dtCand, double rCand, double* partial)
+{
+    __shared__ double sMin[BLK / 64], sMax_[BLK / 64];
+    dtCand = warpMin(dtCand);
+    rCand = warpMax(rCand);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) { sMin[w] = dtCand; sMax_[w] = rCand; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = sMin[0], b0 = sMax_[0];
+#pragma unroll
+        for (int k = 1; k < BLK / 64; k++) { a0 = fmin(a0, sMin[k]); b0 = fmax(b0, sMax_[k]); }
+        partial[2 * blockIdx.x] = a0;
+        partial[2 * blockIdx.x + 1] = b0;
+    }
+}
+
 template <int S, int KIND>
 __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageArgs a)
 {
     if (a.checkActive && !a.sc->active) return;
-    __shared__ double sMin[BLK / 64], sMax_[BLK / 64];
     const uint32_t i = xcdBlock(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     double dtCand = INFINITY, rCand = 0.0;
     if (i < m.N) {
@@ -682,101 +831,254 @@ __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageAr
             else { r[0] = r[0] + lf.x; r[1] = r[1] + lf.y; r[2] = r[2] + lf.z; r[3] = r[3] + lf.w; }
             sAve += a.lenS[f];
         }
-        sAve /= m.cper[i];
-        const double CFL = a.sc->CFL;
-        const double dtc = (CFL * m.cd[i]) / sAve;
-        if (!isnan(dtc)) dtCand = dtc;
-        double hc = 0.0;
-        if (p.localTimestep) {
-            if (a.dtUpdate) {
-                hc = CFL * m.cd[i] / sAve;
-                if (isnan(hc)) hc = 0;
-                a.celldt[i] = hc;
+        cellStage<KIND>(m, p, a, i, r, sAve, nullptr, nullptr, dtCand, rCand);
+    }
+    blockPartial(dtCand, rCand, a.partial);
+}
+
+// ---------------------------------------------------------------- fused stage
+// One workgroup per tile: load the states of own + 2 halo rings into LDS, fill
+// the ghosts, compute gradients/limiters for own + ring 1, every face flux of the
+// own cells, and the own cells' residual + stage update -- one pass over HBM.
+// The per-cell and per-face arithmetic is the same device code as the split
+// kernels above, so results are bit-identical.
+constexpr int TBLK = 256;
+__global__ void k_reduce_stage(const double* __restrict__ partial, uint32_t n, DevScalars* sc, int checkActive);
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for every
+// outstanding global load and store of the wave, which would stall on the next
+// tile's prefetch; the tile kernel shares data between threads through LDS only.
+DEV void ldsBarrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Persistent form: a workgroup walks a contiguous run of tiles; while it computes
+// one tile, the next tile's blob is already in flight into registers.
+constexpr int TILE_PREFETCH = TILE_BLOB_MAX / (16 * TBLK);   // 16-byte chunks per thread
+
+DEV void loadBlob(const TileArgs& t, const uint4& d1, uint4 (&pre)[TILE_PREFETCH])
+{
+    const uint4* src = t.blob + d1.y;
+    const uint32_t last = d1.z - 1;   // blobs are never empty
+    // unconditional loads (clamped index) so that all of them are in flight at once
+#pragma unroll
+    for (int u = 0; u < TILE_PREFETCH; u++) pre[u] = src[min(u * TBLK + threadIdx.x, last)];
+}
+
+template <int S, int FLUX, int VISC, int LP>
+__global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p, StageArgs a, TileArgs t)
+{
+    if (a.checkActive && !a.sc->active) return;
+    extern __shared__ uint4 ldsv[];
+    uint8_t* base = reinterpret_cast<uint8_t*>(ldsv);
+    const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
+    if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
+    const uint32_t per = t.nTiles / gridDim.x, rem = t.nTiles % gridDim.x;
+    const uint32_t tBeg = bid * per + (bid < rem ? bid : rem), tEnd = tBeg + per + (bid < rem ? 1 : 0);
+    constexpr int LW = LP ? 8 : 4;
+    const uint32_t tid = threadIdx.x;
+    double dtCand = INFINITY, rCand = 0.0;
+    uint4 pre[TILE_PREFETCH], d0 = make_uint4(0, 0, 0, 0), d1 = make_uint4(0, 0, 0, 0);
+    if (tBeg < tEnd) {
+        d0 = t.desc[2 * tBeg];
+        d1 = t.desc[2 * tBeg + 1];
+        loadBlob(t, d1, pre);
+    }
+    for (uint32_t tile = tBeg; tile < tEnd; tile++) {
+        const uint32_t nC = d0.x, nG = d0.y, nQ = d0.z, nL = d0.w, nF = d1.x;
+        const TileImg I = tileImage(nC, nG, nQ, nL, nF, S, LW);
+#pragma unroll
+        for (int u = 0; u < TILE_PREFETCH; u++) {
+            const uint32_t k = u * TBLK + tid;
+            if (k < d1.z) ldsv[k] = pre[u];
+        }
+        ldsBarrier();
+        const uint32_t* ids = reinterpret_cast<const uint32_t*>(base + I.ids);
+        double* qs = reinterpret_cast<double*>(base + I.qs);
+        for (uint32_t k = tid; k < nQ; k += TBLK) {
+            const double4 v = a.qin[ids[k]];
+            qs[4 * k] = v.x; qs[4 * k + 1] = v.y; qs[4 * k + 2] = v.z; qs[4 * k + 3] = v.w;
+        }
+        const uint4 n0 = ldsv[I.hdr / 16], n1 = ldsv[I.hdr / 16 + 1];
+        if (tile + 1 < tEnd) loadBlob(t, n1, pre);   // overlaps the compute below
+        ldsBarrier();
+        if (nL > nQ) {
+            const uint16_t* gown = reinterpret_cast<const uint16_t*>(base + I.gown);
+            for (uint32_t k = nQ + tid; k < nL; k += TBLK) {
+                const uint32_t g = ids[k], own = gown[k - nQ];
+                double qg[4];
+                ghostFromOwner(m.bface[g - m.N], qs + 4 * own, a.qin, g, qg);
+#pragma unroll
+                for (int c = 0; c < 4; c++) qs[4 * k + c] = qg[c];
+                if (own < nC) st4(a.qin, g, qg);   // one writer per ghost: the tile owning its cell
+            }
+            ldsBarrier();
+        }
+        const double2* cs = reinterpret_cast<const double2*>(base + I.cs);
+        double* gs = reinterpret_cast<double*>(base + I.gs);
+        double* ls = reinterpret_cast<double*>(base + I.ls);
+        if (p.order > 1) {
+            const bool lim = p.limited && a.limit;
+            const uint16_t* nbS = reinterpret_cast<const uint16_t*>(base + I.nb);
+            const double* gmS = reinterpret_cast<const double*>(base + I.gm);
+            for (uint32_t k = tid; k < nG; k += TBLK) {
+                double qn[S][4];
+                double2 cn[S];
+                bool real[S];
+#pragma unroll
+                for (int j = 0; j < S; j++) {
+                    const uint32_t nb = nbS[k * S + j];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) qn[j][c] = qs[4 * nb + c];
+                    cn[j] = cs[nb];
+                    real[j] = nb != k;
+                }
+                double g[4][2], lx[4], ly[4];
+                const double* gmi = gmS + k;
+                if (!lim) gradLimit<S, 1>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 0, p.lpThresh, g, lx, ly);
+                else if (LP) gradLimit<S, 3>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 1, p.lpThresh, g, lx, ly);
+                else gradLimit<S, 2>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 1, p.lpThresh, g, lx, ly);
+#pragma unroll
+                for (int c = 0; c < 4; c++) { gs[8 * k + 2 * c] = g[c][0]; gs[8 * k + 2 * c + 1] = g[c][1]; }
+                if (LP) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) { ls[8 * k + 2 * c] = lx[c]; ls[8 * k + 2 * c + 1] = ly[c]; }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) ls[4 * k + c] = lx[c];
+                }
+            }
+        } else if (VISC) {
+            for (uint32_t k = tid; k < 8 * nG; k += TBLK) gs[k] = 0.0;   // order 1: zero gradients
+        }
+        ldsBarrier();
+        double* fs = reinterpret_cast<double*>(base + I.fs);
+        const uint2* fcS = reinterpret_cast<const uint2*>(base + I.fc);
+        for (uint32_t k = tid; k < nF; k += TBLK) {
+            const uint2 fe = fcS[k];
+            const uint32_t f = fe.x, la = fe.y & 0xffffu, lb = fe.y >> 16;
+            double* slot = fs + 5 * k;
+            const double2 n = make_double2(slot[0], slot[1]);
+            const double2 mid = make_double2(slot[2], slot[3]);
+            const double len = slot[4];
+            double qL[4], qR[4], F[4], sMax;
+            double la_x[4], la_y[4], lb_x[4], lb_y[4];
+            if (p.order > 1) {
+                if (LP) {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) { la_x[c] = ls[8 * la + 2 * c]; la_y[c] = ls[8 * la + 2 * c + 1]; }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; c++) { la_x[c] = ls[4 * la + c]; la_y[c] = la_x[c]; }
+                }
+            }
+            uint64_t errKey;
+            if (f < m.Fi) {
+                if (p.order > 1) {
+                    if (LP) {
+#pragma unroll
+                        for (int c = 0; c < 4; c++) { lb_x[c] = ls[8 * lb + 2 * c]; lb_y[c] = ls[8 * lb + 2 * c + 1]; }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 4; c++) { lb_x[c] = ls[4 * lb + c]; lb_y[c] = lb_x[c]; }
+                    }
+                }
+                const double2 ca = cs[la], cb = cs[lb];
+                if (p.order > 1)
+                    interiorFace<FLUX, VISC, 2, LP>(qs + 4 * la, gs + 8 * la, la_x, la_y, ca, qs + 4 * lb, gs + 8 * lb,
+                                                    lb_x, lb_y, cb, n, mid, p, qL, qR, F, sMax);
+                else
+                    interiorFace<FLUX, VISC, 1, LP>(qs + 4 * la, gs + 8 * la, la_x, la_y, ca, qs + 4 * lb, gs + 8 * lb,
+                                                    lb_x, lb_y, cb, n, mid, p, qL, qR, F, sMax);
+                errKey = uint64_t(m.Fb) + f;
             } else {
-                hc = a.celldt[i];
+                const uint32_t b = f - m.Fi;
+                const double2 co = cs[la];
+                if (p.order > 1)
+                    boundaryFace<FLUX, VISC, 2, LP>(qs + 4 * la, gs + 8 * la, la_x, la_y, co, qs + 4 * lb, m.bface[b],
+                                                    n, mid, p, qL, qR, F, sMax);
+                else
+                    boundaryFace<FLUX, VISC, 1, LP>(qs + 4 * la, gs + 8 * la, la_x, la_y, co, qs + 4 * lb, m.bface[b],
+                                                    n, mid, p, qL, qR, F, sMax);
+                errKey = b;
+            }
+            if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
+                atomicMin(&a.sc->errEdge, (uint64_t(a.stageId) << 32) | errKey);
+            slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
+            slot[4] = len * sMax;
+            if (f >= m.Fi) {   // boundary faces (each in exactly one tile) feed the force integral
+                *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) =
+                    make_double4(len * F[0], len * F[1], len * F[2], len * F[3]);
+                a.lenS[f] = len * sMax;
             }
         }
+        ldsBarrier();
+        const uint16_t* cfS = reinterpret_cast<const uint16_t*>(base + I.cf);
+        const double* met = reinterpret_cast<const double*>(base + I.met);
+        for (uint32_t k = tid; k < nC; k += TBLK) {
+            double r[4] = {0.0, 0.0, 0.0, 0.0};
+            double sAve = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) if (fabs(r[k]) > rCand) rCand = fabs(r[k]);
-        const double s = m.nia[i];
-        double R[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) R[k] = r[k] * s;
-        if (m.src) {
-            const double4 sv = m.src[i];
-            R[0] = R[0] + sv.x; R[1] = R[1] + sv.y; R[2] = R[2] + sv.z; R[3] = R[3] + sv.w;
+            for (int j = 0; j < S; j++) {
+                const uint32_t fe = cfS[k * S + j];
+                const double* lf = fs + 5 * (fe & 0x7fffu);
+                if (fe >> 15) { r[0] = r[0] - lf[0]; r[1] = r[1] - lf[1]; r[2] = r[2] - lf[2]; r[3] = r[3] - lf[3]; }
+                else { r[0] = r[0] + lf[0]; r[1] = r[1] + lf[1]; r[2] = r[2] + lf[2]; r[3] = r[3] + lf[3]; }
+                sAve += lf[4];
+            }
+            const uint32_t gi = ids[k];
+            const double* qi = qs + 4 * k;
+            const double* mk = met + 3 * k;
+            switch (a.kind) {
+            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            default: cellStage<SK_RESIDUAL>(m, p, a, gi, r, sAve, qi, mk, dtCand, rCand); break;
+            }
         }
-        const double dt = a.sc->dt;
-        const double h = p.localTimestep ? hc : dt;
-        double qn[4], o[4];
-        if (KIND != SK_RESIDUAL) ld4(a.qn, i, qn);
-        if (KIND == SK_RESIDUAL) {
-            st4(a.R, i, R);
-        } else if (KIND == SK_EULER) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
-            st4(a.qout, i, o);
-        } else if (KIND == SK_RK2_1) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = qn[k] + (dt * R[k]);
-            st4(a.acc, i, R);
-            st4(a.qout, i, o);
-        } else if (KIND == SK_RK2_2) {
-            double k1[4];
-            ld4(a.acc, i, k1);
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 2.0) * (k1[k] + R[k]);
-            st4(a.qout, i, o);
-        } else if (KIND == SK_TVD_1) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
-            st4(a.qout, i, o);
-        } else if (KIND == SK_TVD_2) {
-            double fe[4];
-            ld4(a.qin, i, fe);
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = 0.5 * (qn[k] + fe[k] + h * R[k]);
-            st4(a.qout, i, o);
-        } else if (KIND == SK_RK4_1) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = qn[k] + ((dt / 2.0) * R[k]);
-            st4(a.acc, i, R);
-            st4(a.qout, i, o);
-        } else if (KIND == SK_RK4_2 || KIND == SK_RK4_3) {
-            double ac[4];
-            ld4(a.acc, i, ac);
-#pragma unroll
-            for (int k = 0; k < 4; k++) ac[k] = ac[k] + 2.0 * R[k];
-            const double c = (KIND == SK_RK4_2) ? (dt / 2.0) : dt;
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = qn[k] + (c * R[k]);
-            st4(a.acc, i, ac);
-            st4(a.qout, i, o);
-        } else if (KIND == SK_RK4_4) {
-            double ac[4];
-            ld4(a.acc, i, ac);
-#pragma unroll
-            for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 6.0) * (ac[k] + R[k]);
-            st4(a.qout, i, o);
-        }
-        if (KIND != SK_RESIDUAL && a.last && a.recordErrors &&
-            (isnan(o[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3])))
-            atomicMin(&a.sc->errCell, (uint64_t(a.stepId) << 32) | m.refId[i]);
+        ldsBarrier();   // the next tile overwrites the LDS image
+        d0 = n0;
+        d1 = n1;
     }
-    dtCand = warpMin(dtCand);
-    rCand = warpMax(rCand);
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    if (l == 0) { sMin[w] = dtCand; sMax_[w] = rCand; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a0 = sMin[0], b0 = sMax_[0];
-#pragma unroll
-        for (int k = 1; k < BLK / 64; k++) { a0 = fmin(a0, sMin[k]); b0 = fmax(b0, sMax_[k]); }
-        // one partial per workgroup; k_reduce_stage combines them (device-scope
-        // atomics from every workgroup on one address serialise across the XCDs)
-        a.partial[2 * blockIdx.x] = a0;
-        a.partial[2 * blockIdx.x + 1] = b0;
+    blockPartial(dtCand, rCand, a.partial);
+}
+
+template <int S, int FLUX, int VISC, int LP>
+static void tileLaunch(const DevMesh& m, const PhysParams& p, const StageArgs& a, const TileArgs& t, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_stage_tile<S, FLUX, VISC, LP>), dim3(t.nBlocks), dim3(TBLK), t.ldsBytes, st, m, p, a, t);
+}
+
+template <int S>
+static void tileS(const DevMesh& m, const PhysParams& p, const StageArgs& a, const TileArgs& t, hipStream_t st)
+{
+    const bool lp = p.order > 1 && p.limited && p.lpThresh > 0;
+    if (p.flux == 1) {
+        if (p.viscous) { if (lp) tileLaunch<S, 1, 1, 1>(m, p, a, t, st); else tileLaunch<S, 1, 1, 0>(m, p, a, t, st); }
+        else { if (lp) tileLaunch<S, 1, 0, 1>(m, p, a, t, st); else tileLaunch<S, 1, 0, 0>(m, p, a, t, st); }
+    } else {
+        if (p.viscous) { if (lp) tileLaunch<S, 0, 1, 1>(m, p, a, t, st); else tileLaunch<S, 0, 1, 0>(m, p, a, t, st); }
+        else { if (lp) tileLaunch<S, 0, 0, 1>(m, p, a, t, st); else tileLaunch<S, 0, 0, 0>(m, p, a, t, st); }
     }
+}
+
+void launchStageTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const TileArgs& t, int S,
+                      hipStream_t st)
+{
+    if (t.nTiles == 0) return;
+    if (S == 3) tileS<3>(m, p, a, t, st);
+    else if (S == 4) tileS<4>(m, p, a, t, st);
+    else tileS<6>(m, p, a, t, st);
+    hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nBlocks, a.sc, a.checkActive);
 }
 
 // newDt = min over the stage's workgroup partials; Rmax = max with the step's running value
@@ -832,9 +1134,9 @@ DEV void rmsElement(const DiagArgs& d, uint64_t e, double out[8])
     }
 }
 
-constexpr int PER_T = RMS_CHUNK_BLOCKS / BLK;   // 16-blocks per thread
+constexpr int RBLK = 1024;   // threads per chunk workgroup
 
-__global__ __launch_bounds__(BLK) void k_rms_chunks(DiagArgs d)
+__global__ __launch_bounds__(RBLK) void k_rms_chunks(DiagArgs d)
 {
     if (!d.sc->active) return;
     __shared__ double v[8][RMS_CHUNK_BLOCKS];
@@ -843,7 +1145,7 @@ __global__ __launch_bounds__(BLK) void k_rms_chunks(DiagArgs d)
     const uint32_t cnt = uint32_t((nb - c0) < RMS_CHUNK_BLOCKS ? (nb - c0) : RMS_CHUNK_BLOCKS);
     // 16 consecutive lanes own one 16-element block: the xor butterfly over
     // 16 lanes adds exactly like the fixed tree (((r0+r1)+(r2+r3))+...)
-    constexpr uint32_t BPI = BLK / 16;
+    constexpr uint32_t BPI = RBLK / 16;
     for (uint32_t it = 0; it < RMS_CHUNK_BLOCKS / BPI; it++) {
         const uint32_t lb = it * BPI + threadIdx.x / 16;
         double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -867,15 +1169,15 @@ __global__ __launch_bounds__(BLK) void k_rms_chunks(DiagArgs d)
             mask |= 1u << level;
         }
         const uint32_t half = c / 2;
-        double tmp[PER_T * 4];
+        double tmp[(8 * RMS_CHUNK_BLOCKS / 2 + RBLK - 1) / RBLK];
         int n = 0;
-        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += BLK) {
+        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += RBLK) {
             const uint32_t q = idx / half, j = idx % half;
             tmp[n++] = v[q][2 * j] + v[q][2 * j + 1];
         }
         __syncthreads();
         n = 0;
-        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += BLK) {
+        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += RBLK) {
             const uint32_t q = idx / half, j = idx % half;
             v[q][j] = tmp[n++];
         }
@@ -1141,7 +1443,7 @@ void launchRmsChunks(const DiagArgs& d, hipStream_t st)
 {
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint32_t nc = uint32_t((nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS);
-    if (nc) hipLaunchKernelGGL(k_rms_chunks, dim3(nc), dim3(BLK), 0, st, d);
+    if (nc) hipLaunchKernelGGL(k_rms_chunks, dim3(nc), dim3(RBLK), 0, st, d);
 }
 
 void launchFinalize(const DiagArgs& d, hipStream_t st)

@@ -20,6 +20,7 @@
 #include "../../include/ebbcfd.h"
 #include "dev.hpp"
 #include "host.hpp"
+#include "tiles.hpp"
 
 using namespace ebb;
 
@@ -138,6 +139,11 @@ struct ebb_solver {
     DBuf<BFace> d_bface;
     DBuf<double4> d_src, d_qA, d_qB, d_T, d_T2, d_acc, d_R, d_estate;
     DBuf<uint32_t> d_chunkMask;
+    // fused-stage tiles
+    DBuf<uint4> d_tDesc, d_tBlob;
+    TileArgs tiles{};
+    bool useTiles = false;
+    uint32_t tileCells = 0;
     DevScalars* d_sc = nullptr;
     DevScalars* h_sc = nullptr;  // pinned
     double* h_hist = nullptr;    // pinned
@@ -194,9 +200,13 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
     a.estate = estate; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
     a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
     a.checkActive = checkActive; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = stepId;
-    launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
-    launchFaces(s->dm, s->phys, a, s->stream);
-    launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
+    if (s->useTiles && !estate && kind != SK_RESIDUAL) {
+        launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
+    } else {
+        launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+        launchFaces(s->dm, s->phys, a, s->stream);
+        launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
+    }
     s->lastInput = qin;
     s->lastLimit = limit;
     s->haveStage = true;
@@ -295,55 +305,60 @@ int clearErrors(ebb_solver* s)
     return EBB_OK;
 }
 
+// Pick the tile size (tileCells, or automatically the largest multiple of 16 up
+// to 128 whose LDS lets two workgroups share a CU) and upload the tiles.
+int buildTiles(ebb_solver* s, const Layout& L)
+{
+    s->useTiles = false;
+    s->tiles = TileArgs{};
+    const long long want = s->cfg.tileCells;
+    if (want == 0 || L.N == 0) return EBB_OK;
+    if (want < -1 || want > 1024) return fail(s, EBB_ERR_ARG, "tileCells must be -1 (auto) or in 0..1024");
+    const int lp = (s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0) ? 1 : 0;
+    constexpr uint32_t LDS_MAX = 65536, LDS_TWO_PER_CU = 160 * 1024 / 2;
+    TileHost h;
+    uint32_t T = want > 0 ? uint32_t(want) : 128;
+    for (;;) {
+        std::string err;
+        if (!makeTiles(L, T, s->cfg.tileOrder, h, err)) return fail(s, EBB_ERR_MESH, err);
+        const uint32_t lds = h.maxEnd[lp];
+        const bool fits = lds <= LDS_MAX && h.maxBlob <= TILE_BLOB_MAX;
+        if (want > 0 || (fits && lds <= LDS_TWO_PER_CU) || T <= 64) {
+            if (!fits) return EBB_OK;   // does not fit: split kernels
+            break;
+        }
+        T -= 16;
+    }
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    HIPCHK(trackUpload(s, s->d_tDesc, h.desc));
+    HIPCHK(trackUpload(s, s->d_tBlob, h.blob));
+    TileArgs ta{};
+    ta.nTiles = h.nTiles;
+    ta.nBlocks = std::min<uint32_t>(h.nTiles, 2 * uint32_t(std::max(cus, 1)));   // two resident per CU
+    ta.ldsBytes = h.maxEnd[lp];
+    ta.desc = s->d_tDesc.p;
+    ta.blob = s->d_tBlob.p;
+    s->tiles = ta;
+    s->tileCells = T;
+    s->useTiles = true;
+    return EBB_OK;
+}
+
 int buildDevice(ebb_solver* s)
 {
     const BuiltMesh& b = s->b;
     const Config& c = s->cfg;
-    s->N = b.nElem;
-    s->G = b.nCells - b.nElem;
-    s->Fi = uint32_t(b.interiorEdges.size());
-    s->Fb = uint32_t(b.boundaryEdges.size());
-    uint32_t maxE = 3;
-    for (uint32_t i = 0; i < s->N; i++) maxE = std::max<uint32_t>(maxE, b.nEdgesCell[i]);
-    s->S = maxE <= 3 ? 3 : (maxE == 4 ? 4 : 6);
-    const uint32_t S = s->S, N = s->N, F = s->Fi + s->Fb;
-    s->faceOfEdge.assign(b.nEdges, 0);
-    s->edgeOfFace.assign(F, 0);
-    for (uint32_t k = 0; k < s->Fi; k++) { s->faceOfEdge[b.interiorEdges[k]] = k; s->edgeOfFace[k] = b.interiorEdges[k]; }
-    for (uint32_t k = 0; k < s->Fb; k++) { s->faceOfEdge[b.boundaryEdges[k]] = s->Fi + k; s->edgeOfFace[s->Fi + k] = b.boundaryEdges[k]; }
-
-    std::vector<uint32_t> nbr(size_t(N) * S), cface(size_t(N) * S), refId(N);
-    std::vector<double> gm(size_t(N) * 2 * S), nia(N), cd(N), cper(N);
-    for (uint32_t i = 0; i < N; i++) {
-        const uint32_t ne = b.nEdgesCell[i];
-        for (uint32_t j = 0; j < S; j++) {
-            if (j < ne) {
-                nbr[size_t(i) * S + j] = b.nbr[size_t(i) * MAXE + j];
-                const uint32_t f = s->faceOfEdge[b.cellEdges[size_t(i) * MAXE + j]];
-                cface[size_t(i) * S + j] = f | (b.fluxMult[size_t(i) * MAXE + j] < 0 ? 0x80000000u : 0u);
-            } else {
-                nbr[size_t(i) * S + j] = i;   // self: du = 0, dx = dy = 0
-                cface[size_t(i) * S + j] = F; // all-zero padding face
-            }
-            for (int r = 0; r < 2; r++)
-                gm[size_t(i) * 2 * S + r * S + j] = j < uint32_t(MAXE) ? b.gradMat[size_t(i) * 2 * MAXE + r * MAXE + j] : 0.0;
-        }
-        nia[i] = -(1.0 / b.area[i]);
-        cd[i] = b.d[i];
-        cper[i] = b.perim[i];
-        refId[i] = i;
-    }
-    std::vector<double2> cen(b.nCells), fn(F), fmid(F);
-    std::vector<uint2> fcell(F);
-    std::vector<double> flen(F);
-    for (uint32_t k = 0; k < b.nCells; k++) cen[k] = make_double2(b.cx[k], b.cy[k]);
-    for (uint32_t f = 0; f < F; f++) {
-        const uint32_t e = s->edgeOfFace[f];
-        fcell[f] = make_uint2(b.eCell0[e], b.eCell1[e]);
-        fn[f] = make_double2(b.nx[e], b.ny[e]);
-        fmid[f] = make_double2(b.mx[e], b.my[e]);
-        flen[f] = b.len[e];
-    }
+    Layout lay;
+    makeLayout(b, lay);
+    s->N = lay.N; s->G = lay.G; s->Fi = lay.Fi; s->Fb = lay.Fb; s->S = lay.S;
+    const uint32_t N = s->N, F = s->Fi + s->Fb;
+    s->faceOfEdge = lay.faceOfEdge;
+    s->edgeOfFace = lay.edgeOfFace;
+    const std::vector<uint32_t>&nbr = lay.nbr, &cface = lay.cface, &refId = lay.refId;
+    const std::vector<double>&gm = lay.gm, &nia = lay.nia, &cd = lay.cd, &cper = lay.cper, &flen = lay.flen;
+    const std::vector<double2>&cen = lay.cen, &fn = lay.fn, &fmid = lay.fmid;
+    const std::vector<uint2>& fcell = lay.fcell;
     // boundary binding (solve.d:222-278)
     s->bfaces.assign(s->Fb, BFace{});
     std::vector<int> bcOfTag(s->raw.btags.size(), -1);
@@ -460,7 +475,9 @@ int buildDevice(ebb_solver* s)
     HIPCHK((hipError_t)trackAlloc(s, s->d_lenF, size_t(F + 1) * 4));
     HIPCHK((hipError_t)trackAlloc(s, s->d_lenS, size_t(F + 1)));
     if (c.localTimestep) HIPCHK((hipError_t)trackAlloc(s, s->d_celldt, N));
-    HIPCHK((hipError_t)trackAlloc(s, s->d_partial, 2 * size_t((N + 255) / 256) + 2));
+    // physics parameters are set above; the tile builder needs them for the LDS size
+    { int rc_ = buildTiles(s, lay); if (rc_) return rc_; }
+    HIPCHK((hipError_t)trackAlloc(s, s->d_partial, 2 * std::max<size_t>((N + 255) / 256, s->tiles.nBlocks) + 2));
     HIPCHK((hipError_t)trackAlloc(s, s->d_hist, size_t(s->histCap) * HIST_FIELDS));
     const uint64_t nb = uint64_t(C) / 16;
     const uint64_t nchunks = (nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
@@ -606,6 +623,27 @@ int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, 
     return EBB_OK;
 }
 
+int ebb_mesh_tile_stats(const ebb_mesh* m, int tile_cells, int order, double out[8])
+{
+    if (!m || !out || tile_cells <= 0 || tile_cells > 1024 || order < 0 || order > 1) return EBB_ERR_ARG;
+    Layout L;
+    makeLayout(m->b, L);
+    TileHost h;
+    std::string err;
+    if (!makeTiles(L, uint32_t(tile_cells), order, h, err)) {
+        std::fprintf(stderr, "ebb_mesh_tile_stats: %s\n", err.c_str());
+        return EBB_ERR_MESH;
+    }
+    const double n = L.N ? double(L.N) : 1.0, nt = h.nTiles ? double(h.nTiles) : 1.0;
+    out[0] = double(h.nTiles);
+    out[1] = double(h.sumC) / nt;
+    out[2] = double(h.sumG) / n; out[3] = double(h.sumQ) / n; out[4] = double(h.sumL) / n;
+    out[5] = double(h.sumF) / n;
+    out[6] = h.maxL;
+    out[7] = h.maxEnd[0];
+    return EBB_OK;
+}
+
 void ebb_mesh_destroy(ebb_mesh* m) { delete m; }
 
 int ebb_config_parse(const char* json, ebb_config** out)
@@ -635,6 +673,7 @@ static double* cfgField(Config& c, const std::string& k, bool& isInt, long long*
     else if (k == "plotIter") ip = &c.plotIter;
     else if (k == "order") ip = &c.order;
     else if (k == "maxSteps") ip = &c.maxSteps;
+    else if (k == "tileCells") ip = &c.tileCells;
     else if (k == "limited") bp = &c.limited;
     else if (k == "dynamicDt") bp = &c.dynamicDt;
     else if (k == "localTimestep") bp = &c.localTimestep;
@@ -937,6 +976,8 @@ int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
 {
     if (!s) return EBB_ERR_ARG;
     HIPCHK(hipSetDevice(s->device));
+    // the fused stage kernel keeps gradients in LDS: recompute them for the last stage input
+    if (s->useTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
     if (grad) HIPCHK(hipMemcpy(grad, s->d_grad.p, size_t(s->N) * 8 * sizeof(double), hipMemcpyDeviceToHost));
     if (lim) {
         const bool lp = s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0;
@@ -1089,8 +1130,7 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
     const int nst = stagesPerStep(s->cfg);
     std::vector<hipEvent_t> ev(size_t(steps) * nst * 3 + 1);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    const double4* q0 = s->qcur();
-    (void)q0;
+    std::vector<int> part;   // which figure each event interval adds to
     // same launch sequence as stepStages, with an event after each kernel
     HIPCHK(hipEventRecord(ev[0], s->stream));
     size_t k = 1;
@@ -1112,12 +1152,19 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
             a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
             a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
             a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
-            launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
-            HIPCHK(hipEventRecord(ev[k++], s->stream));
-            launchFaces(s->dm, s->phys, a, s->stream);
-            HIPCHK(hipEventRecord(ev[k++], s->stream));
-            launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
-            HIPCHK(hipEventRecord(ev[k++], s->stream));
+            if (s->useTiles) {
+                launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
+                HIPCHK(hipEventRecord(ev[k++], s->stream));
+                part.push_back(3);
+            } else {
+                launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+                HIPCHK(hipEventRecord(ev[k++], s->stream));
+                launchFaces(s->dm, s->phys, a, s->stream);
+                HIPCHK(hipEventRecord(ev[k++], s->stream));
+                launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
+                HIPCHK(hipEventRecord(ev[k++], s->stream));
+                part.push_back(0); part.push_back(1); part.push_back(2);
+            }
             s->lastInput = x.in;
             s->lastLimit = x.limit;
             s->haveStage = true;
@@ -1126,18 +1173,16 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(ev[k - 1]));
-    double tg = 0, tf = 0, tu = 0;
-    for (size_t j = 1; j < k; j += 3) {
-        float a, b, cc;
+    double tp[4] = {0, 0, 0, 0};
+    for (size_t j = 1; j < k; j++) {
+        float a;
         HIPCHK(hipEventElapsedTime(&a, ev[j - 1], ev[j]));
-        HIPCHK(hipEventElapsedTime(&b, ev[j], ev[j + 1]));
-        HIPCHK(hipEventElapsedTime(&cc, ev[j + 1], ev[j + 2]));
-        tg += a; tf += b; tu += cc;
+        tp[part[j - 1]] += a;
     }
     float total;
     HIPCHK(hipEventElapsedTime(&total, ev[0], ev[k - 1]));
     const double n = double(steps) * nst;
-    out[0] = tg / n; out[1] = tf / n; out[2] = tu / n; out[3] = 0;
+    out[0] = tp[0] / n; out[1] = tp[1] / n; out[2] = tp[2] / n; out[3] = tp[3] / n;
     out[4] = double(total) / steps;
     out[5] = nst;
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -1162,6 +1207,7 @@ void ebb_solver_destroy(ebb_solver* s)
     s->d_cen.release(); s->d_fn.release(); s->d_fmid.release(); s->d_fcell.release(); s->d_bface.release();
     s->d_src.release(); s->d_qA.release(); s->d_qB.release(); s->d_T.release(); s->d_T2.release();
     s->d_acc.release(); s->d_R.release(); s->d_estate.release();
+    s->d_tDesc.release(); s->d_tBlob.release();
     if (s->d_sc) (void)hipFree(s->d_sc);
     if (s->h_sc) (void)hipHostFree(s->h_sc);
     if (s->h_hist) (void)hipHostFree(s->h_hist);

@@ -118,6 +118,15 @@ class UMesh2:
                                             _p(out["bGroup"], C.c_int32)), "edge geometry")
         return out
 
+    def tile_stats(self, tile_cells=128, order="grow"):
+        """Tiles of the fused stage kernel for this mesh (host only)."""
+        out = np.zeros(8)
+        _check(lib().ebb_mesh_tile_stats(self.h, int(tile_cells), {"grow": 0, "morton": 1}[order], _p(out)),
+               "tile stats")
+        keys = ("tiles", "cells_per_tile", "grad_ratio", "load_ratio", "local_ratio", "faces_per_cell",
+                "max_local", "lds_bytes")
+        return dict(zip(keys, out.tolist()))
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().ebb_mesh_destroy(self.h)
@@ -236,7 +245,8 @@ class Solver:
         """Device time per kernel launch (ms) over `steps` integrator steps, HIP events on the solver stream."""
         out = np.zeros(6)
         _check(lib().ebb_time_steps(self.h, int(steps), _p(out)), "time steps", self.h)
-        return dict(gradient_ms=out[0], faces_ms=out[1], update_ms=out[2], step_ms=out[4], stages=int(out[5]))
+        return dict(gradient_ms=out[0], faces_ms=out[1], update_ms=out[2], fused_ms=out[3], step_ms=out[4],
+                    stages=int(out[5]))
 
     @property
     def device_bytes(self):

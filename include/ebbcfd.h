@@ -71,6 +71,15 @@ int ebb_mesh_cell_geometry(const ebb_mesh* m, double* area, double* d, double* p
 int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, double* len, double* normal,
                            double* mid, int32_t* bgroup);
 
+/* Tiles the fused stage kernel would use for this mesh (no equivalent in the
+ * reference; host only, needs no GPU).  tile_cells > 0; order 0: greedily
+ * grown regions, 1: Morton order.  out[0] tiles, out[1] mean own cells per tile,
+ * out[2..5] cells with gradients (own + ring 1), cells loaded (own + 2 rings),
+ * local cells incl. ghosts and faces, each summed over tiles and divided by the
+ * interior cell count; out[6] largest local cell count; out[7] LDS bytes per
+ * workgroup (scalar limiter layout). */
+int ebb_mesh_tile_stats(const ebb_mesh* m, int tile_cells, int order, double out[8]);
+
 void ebb_mesh_destroy(ebb_mesh* m);
 
 /* ---------------------------------------------------------------- config */
@@ -78,13 +87,16 @@ void ebb_mesh_destroy(ebb_mesh* m);
 /* loadConfig (source/config.d:112-313): the reference JSON keys and defaults
  * ("boudaryConditions", "dt" as a JSON float, ...).  Extra options live in an
  * optional "extensions" object: {"mms": bool, "maxSteps": int,
- * "precision": "fp64"|"mixed"}. */
+ * "precision": "fp64"|"mixed", "tileCells": int (0, the default: the split
+ * gradient/face/update kernels; n: the fused kernel with tiles of up to n
+ * cells; -1: the fused kernel with an automatic tile size),
+ * "tileOrder": "grow"|"morton"}. */
 int ebb_config_parse(const char* json, ebb_config** out);
 
 /* Read / override a numeric field by its JSON key: "dt", "tEnd", "CFL",
  * "lpThresh", "saveIter", "plotIter", "order", "maxSteps", "limited",
  * "dynamicDt", "localTimestep", "multistageLimiting", "cflAdjust",
- * "viscosity", "mms", "Pr", "R", "Re", "L", "ic0".."ic3". */
+ * "viscosity", "mms", "tileCells", "Pr", "R", "Re", "L", "ic0".."ic3". */
 int ebb_config_get(const ebb_config* c, const char* key, double* value);
 int ebb_config_set(ebb_config* c, const char* key, double value);
 /* Mesh file named by the config ("mesh"), copied into buf. */
@@ -155,8 +167,10 @@ int ebb_write_forces(ebb_solver* s, const char* path, int64_t first, int64_t cou
 int ebb_last_error(ebb_solver* s, int* kind, int64_t* index, int64_t* step, char* msg, size_t len);
 
 /* Device timing of `steps` integrator steps with HIP events on the solver's
- * stream.  out[0..3]: average ms per launch of the gradient, face and update
- * kernels; out[4]: ms per step; out[5]: launches per step of each kernel kind. */
+ * stream.  out[0..2]: average ms per stage of the split gradient, face and
+ * update kernels; out[3]: average ms per stage of the fused tile kernel (with
+ * its stage reduction); the unused path reads 0.  out[4]: ms per step of
+ * the stage kernels; out[5]: stages per step. */
 int ebb_time_steps(ebb_solver* s, int steps, double out[6]);
 
 /* Device bytes held by the solver. */

@@ -33,8 +33,10 @@ def channel_case(nx=24, ny=8, height=0.06, seed=3, walls="inviscidWall", M=0.5, 
 
 
 def cylinder_case(n_theta=64, n_r=32, M=0.2, viscous=True, flux="rusanovFlux", integrator="RK2_TVD", r_far=30.0,
-                  **over):
-    raw = meshgen.cylinder_ogrid(n_theta, n_r, r_far=r_far, stretch=1.006 if n_r >= 512 else 1.08)
+                  stretch=None, **over):
+    if stretch is None:
+        stretch = 1.006 if n_r >= 512 else 1.08
+    raw = meshgen.cylinder_ogrid(n_theta, n_r, r_far=r_far, stretch=stretch)
     q = [M, 0.0, 1.0, 1.0]
     bcs = [{"tag": "Wall", "type": "viscidWall" if viscous else "inviscidWall", "q": q},
            {"tag": "Farfield", "type": "fullState", "q": q}]
