@@ -32,6 +32,9 @@ HBM_PEAK_GBS = 8000.0
 STAGE_BYTES_PER_CELL = 268.0
 FACE_BYTES_PER_FACE = 88.0      # owner/other ids 8 + normal 16 + midpoint 16 + length 8 + len*flux,len*sMax 40
 FACE_BYTES_PER_CELL = 144.0     # state 32 + gradient 64 + limiter 32 + centroid 16, each cell read once
+# face/update tile kernel: faces without the len*flux round trip, plus the cell update
+TILE_BYTES_PER_FACE = 48.0      # cell pair 8 + normal 16 + midpoint 16 + length 8
+TILE_BYTES_PER_CELL = 244.0     # cell data 144 + face refs 12 + perimeter/d/-1/area 24 + q^n 32 + new state 32
 
 
 def log(msg):
@@ -82,11 +85,13 @@ def main():
     ap.add_argument("--n-r", type=int, default=1024)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--time-steps", type=int, default=20, help="steps timed per kernel with HIP events")
-    ap.add_argument("--tile-cells", type=int, default=0,
-                    help="0: split gradient/face/update kernels; n: fused stage kernel with tiles of up to n cells; "
-                         "-1: fused kernel, automatic tile size")
-    ap.add_argument("--tile-order", default="grow", choices=("grow", "morton"),
-                    help="how the fused kernel's tiles are cut: greedily grown regions or Morton order")
+    ap.add_argument("--stage-path", default="faceTiles", choices=("split", "faceTiles", "fused"),
+                    help="kernels of one stage: gradient/face/update, gradient + face/update tiles, or one fused kernel")
+    ap.add_argument("--tile-cells", type=int, default=-1, help="cells per tile of the tiled paths (-1: automatic)")
+    ap.add_argument("--tile-order", default="blocks", choices=("blocks", "grow", "morton"),
+                    help="how tiles are cut: Morton order of 16-cell blocks, greedily grown regions, Morton order")
+    ap.add_argument("--cell-order", default="reference", choices=("reference", "tiles"),
+                    help="device numbering of the cells: the mesh's, or tile by tile")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -103,7 +108,8 @@ def main():
     log("generating O-grid %d x %d" % (args.n_theta, args.n_r))
     raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1)
     cfg = cylinder_config()
-    cfg["extensions"] = {"tileCells": args.tile_cells, "tileOrder": args.tile_order}
+    cfg["extensions"] = {"stagePath": args.stage_path, "tileCells": args.tile_cells, "tileOrder": args.tile_order,
+                         "cellOrder": args.cell_order}
     mesh = build_mesh(raw)
     N = mesh.n_interior
     log("mesh built: %d cells, %d edges" % (N, mesh.n_edges))
@@ -127,9 +133,12 @@ def main():
     F = mesh.n_edges
     stage_ms = tk["step_ms"] / stages
     stage_gbs = N * STAGE_BYTES_PER_CELL / (stage_ms * 1e-3) / 1e9
-    log("kernels: fused %.3f ms | gradient %.3f ms, faces %.3f ms, update %.3f ms | stages of a step %.3f ms"
+    log("kernels: tile %.3f ms | gradient %.3f ms, faces %.3f ms, update %.3f ms | stages of a step %.3f ms"
         % (tk["fused_ms"], tk["gradient_ms"], tk["faces_ms"], tk["update_ms"], tk["step_ms"]))
-    if tk["fused_ms"] > 0:
+    if tk["fused_ms"] > 0 and args.stage_path == "faceTiles":
+        kernel, launch_ms = "k_face_tile", tk["fused_ms"]
+        launch_bytes = F * TILE_BYTES_PER_FACE + N * TILE_BYTES_PER_CELL
+    elif tk["fused_ms"] > 0:
         kernel, launch_ms = "k_stage_tile", tk["fused_ms"]
         launch_bytes = N * STAGE_BYTES_PER_CELL
     else:
@@ -160,6 +169,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r), "cells": N,
+                   "stage_path": args.stage_path, "tile_cells": args.tile_cells, "tile_order": args.tile_order,
+                   "cell_order": args.cell_order,
                    "edges": F, "flux": "rusanovFlux", "limiter": "scalar (solve.d:360-403)",
                    "integrator": "RK2_TVD", "stages_per_step": stages, "M": 0.2, "Re": 100, "CFL": cfg["CFL"]},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

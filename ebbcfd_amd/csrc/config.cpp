@@ -216,13 +216,25 @@ bool parseConfig(const std::string& json, Config& c, std::string& err)
         const JVal* e;
         if ((e = v->get("mms"))) c.mms = isTrue(e);
         if ((e = v->get("maxSteps")) && e->kind == JVal::Int) c.maxSteps = e->i;
+        if ((e = v->get("cellOrder")) && e->kind == JVal::Str) {
+            if (e->s == "reference") c.renumber = false;
+            else if (e->s == "tiles") c.renumber = true;
+            else { err = "unknown cellOrder " + e->s; return false; }
+        }
+        if ((e = v->get("stagePath")) && e->kind == JVal::Str) {
+            if (e->s == "split") c.stagePath = 0;
+            else if (e->s == "faceTiles") c.stagePath = 1;
+            else if (e->s == "fused") c.stagePath = 2;
+            else { err = "unknown stagePath " + e->s; return false; }
+        }
         if ((e = v->get("tileCells")) && e->kind == JVal::Int) {
-            if (e->i < -1 || e->i > 1024) { err = "tileCells must be -1 (auto) or in 0..1024"; return false; }
+            if (e->i != -1 && (e->i < 1 || e->i > 1024)) { err = "tileCells must be -1 (automatic) or in 1..1024"; return false; }
             c.tileCells = e->i;
         }
         if ((e = v->get("tileOrder")) && e->kind == JVal::Str) {
             if (e->s == "grow") c.tileOrder = 0;
             else if (e->s == "morton") c.tileOrder = 1;
+            else if (e->s == "blocks") c.tileOrder = 2;
             else { err = "unknown tileOrder " + e->s; return false; }
         }
         if ((e = v->get("precision")) && e->kind == JVal::Str) {

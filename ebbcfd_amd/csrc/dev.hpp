@@ -99,6 +99,8 @@ struct StageArgs {
     double4* estate;       // optional (F)*2 edge states for .esln, null otherwise
     double* celldt;        // N local time step (localTimestep)
     double* partial;       // 2 per update workgroup: min dt candidate, max |R|
+    double* rmsBlocks;     // last stage of a time-loop step: per 16-cell reference block, the 8 leaf
+                           // sums of the RMS pairwise sum (null: not computed here)
     DevScalars* sc;
     int kind;              // StageKind
     int limit;             // limiter on for this stage
@@ -120,18 +122,18 @@ struct StageArgs {
 struct TileArgs {
     uint32_t nTiles;
     uint32_t nBlocks;              // persistent workgroups, each walking a contiguous run of tiles
-    uint32_t ldsBytes;             // LDS per workgroup (largest tile image)
+    uint32_t slotBytes;            // LDS per blob buffer (largest blob); two buffers alternate
+    uint32_t ldsBytes;             // LDS per workgroup: two blob buffers + the largest working set
     const uint4* desc;             // 2 per tile: (nC, nG, nQ, nL), (nF, blob offset, blob length, 0) in 16 B;
                                    // each blob starts with the descriptor of the tile after it
     const uint4* blob;
 };
 
-// Byte offsets of a tile's LDS image; [0, blobEnd) is the packed blob.
+// Byte offsets inside a tile's blob, which is copied into LDS as is.
 struct TileImg {
-    uint32_t hdr, cs, gm, fs, met, fc, ids, nb, cf, gown, blobEnd, qs, gs, ls, end;
+    uint32_t hdr, cs, gm, fs, met, fc, ids, nb, cf, gown, end;
 };
-__host__ __device__ inline TileImg tileImage(uint32_t nC, uint32_t nG, uint32_t nQ, uint32_t nL, uint32_t nF, int S,
-                                             int LW)
+__host__ __device__ inline TileImg tileImage(uint32_t nC, uint32_t nG, uint32_t nQ, uint32_t nL, uint32_t nF, int S)
 {
     auto al = [](uint32_t x) { return (x + 15u) & ~15u; };
     TileImg I;
@@ -146,15 +148,43 @@ __host__ __device__ inline TileImg tileImage(uint32_t nC, uint32_t nG, uint32_t 
     I.nb = o; o = al(o + 2 * S * nG);               // local neighbours of own + ring 1
     I.cf = o; o = al(o + 2 * S * nC);               // own cells' local faces, bit 15: not the owner
     I.gown = o; o = al(o + 2 * (nL - nQ));          // ghosts: local index of the owner cell
-    I.blobEnd = o;
-    I.qs = o; o = al(o + 32 * nL);                  // states
-    I.gs = o; o = al(o + 64 * nG);                  // gradients (var, dir)
-    I.ls = o; o = al(o + 8 * LW * nG);              // limiters
     I.end = o;
     return I;
 }
 
+// Byte offsets of a tile's working set (computed in the kernel, not stored).
+struct TileDyn {
+    uint32_t qs, gs, ls, end;
+};
+__host__ __device__ inline TileDyn tileDyn(uint32_t nG, uint32_t nL, int LW)
+{
+    TileDyn D;
+    D.qs = 0;                                       // states
+    D.gs = D.qs + 32 * nL;                          // gradients (var, dir)
+    D.ls = D.gs + 64 * nG;                          // limiters
+    D.end = D.ls + 8 * LW * nG;
+    return D;
+}
+
+// Face tiles for the face+update kernel: groups of interior cells; a workgroup
+// computes every face of its cells into LDS (faces between two tiles are
+// computed by both, identically) and then updates its cells.  Gradients,
+// limiters and ghost states come from the gradient kernel.  Per tile, the face
+// geometry and references are stored contiguously.
+struct FaceTileArgs {
+    uint32_t nTiles;
+    uint32_t ldsBytes;
+    const uint4* desc;             // per tile: nC, nF, first cell, first face
+    const uint32_t* ids;           // own cells' device ids
+    const uint16_t* cf;            // S per own cell: local face, bit 15 set where the cell is not the owner;
+                                   // nF (an all-zero slot) for padding
+    const uint4* fref;             // per face: global face id, cell a, cell b (ghost for boundary faces), 0
+    const double* fgeo;            // per face: normal x, y, midpoint x, y, length
+};
+
 // Launchers (kernels.hip).  All run on `stream`.
+void launchFaceTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, int S,
+                     hipStream_t stream);
 void launchStageTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const TileArgs& t, int S,
                       hipStream_t stream);
 void launchGradient(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t stream);
@@ -166,6 +196,7 @@ struct DiagArgs {
     const double4* qold;     // state before the step
     uint32_t N, L;           // interior cells, cells incl. ghosts (reference array length)
     const uint32_t* perm;    // reference index -> device index (null = identity)
+    const double* blockSums; // per 16-element block, its 8 leaf sums from the last stage (null: read the states)
     double* chunkTrees;      // nChunks * 8 * 16 partial pairwise trees
     uint32_t* chunkMask;     // nChunks * 8 bit masks of the levels present
     const uint32_t* forceFaces; uint32_t nForce;
@@ -180,7 +211,6 @@ struct DiagArgs {
     DevScalars* sc;
 };
 constexpr int HIST_FIELDS = 10;
-constexpr uint32_t TILE_BLOB_MAX = 8 * 256 * 16;   // largest tile blob the fused kernel prefetches
 constexpr uint32_t RMS_CHUNK_BLOCKS = 1024;     // 16-element blocks per workgroup
 void launchRmsChunks(const DiagArgs& d, hipStream_t stream);
 void launchFinalize(const DiagArgs& d, hipStream_t stream);

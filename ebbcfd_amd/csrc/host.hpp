@@ -63,8 +63,12 @@ struct Config {
     bool mms = false;                 // manufactured solution: dirichlet = exact, source per stage
     long long maxSteps = -1;          // stop after this many steps (-1: only tEnd)
     int precision = 0;                // 0 = fp64, 1 = mixed fp32 flux/reconstruction/limiter
-    long long tileCells = 0;          // cells per tile of the fused stage kernel (0: split kernels, -1: automatic)
-    int tileOrder = 0;                // 0: greedily grown regions, 1: Morton order of centroids
+    int stagePath = 1;                // 0: gradient, face, update kernels; 1: gradient + face/update tiles;
+                                      // 2: one fused kernel per stage
+    long long tileCells = -1;         // cells per tile of the tiled paths (-1: automatic)
+    int tileOrder = 2;                // 0: greedily grown regions, 1: Morton order of centroids,
+                                      // 2: Morton order of aligned 16-cell blocks
+    bool renumber = false;            // tiled paths: number device cells tile by tile
     std::vector<std::string> warnings;
     int fluxKind() const { return flux == "roeFlux" ? RoeFlux : RusanovFlux; }
     int integratorKind() const {

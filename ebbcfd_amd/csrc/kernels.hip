@@ -706,9 +706,11 @@ DEV double warpMax(double v)
 // solve.d:1092-1117 + the integrator's stage update for interior cell i, from
 // its summed face fluxes r and sum of len*sMax
 // met: (perimeter, d, -1/area) of the cell, or null to read them from the mesh
+// qNew/qOld (optional): the new state and q^n when this stage completes a step
 template <int KIND>
 DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t i, const double r[4],
-                   double sAve, const double* qinI, const double* met, double& dtCand, double& rCand)
+                   double sAve, const double* qinI, const double* met, double& dtCand, double& rCand,
+                   double* qNew = nullptr, double* qOld = nullptr)
 {
     const double cper = met ? met[0] : m.cper[i], cd = met ? met[1] : m.cd[i], nia = met ? met[2] : m.nia[i];
     sAve /= cper;
@@ -792,6 +794,36 @@ DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, ui
     if (KIND != SK_RESIDUAL && a.last && a.recordErrors &&
         (isnan(o[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3])))
         atomicMin(&a.sc->errCell, (uint64_t(a.stepId) << 32) | m.refId[i]);
+    if ((KIND == SK_EULER || KIND == SK_RK2_2 || KIND == SK_TVD_2 || KIND == SK_RK4_4) && qNew) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { qNew[k] = o[k]; qOld[k] = qn[k]; }
+    }
+}
+
+// Leaf sums of the RMS residual's pairwise sum (app/solver.d:153-168), fused into
+// the stage that completes a step.  Each group of 16 lanes holds one aligned block
+// of 16 consecutive reference cells, lane j its element j (absent cells add
+// zeros); the xor butterfly adds exactly like the fixed 16-element tree, and lane
+// 0 of the group stores the block's 8 sums: (new - old)^2 and new per variable.
+// Every lane of the wave must call this.
+DEV void rmsBlockLeaf(bool have, const double qNew[4], const double qOld[4], uint32_t refCell, double* rmsBlocks)
+{
+    double o[8];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const double x = have ? qNew[v] - qOld[v] : 0.0;
+        o[v] = have ? x * x : 0.0;
+        o[4 + v] = have ? qNew[v] : 0.0;
+    }
+#pragma unroll
+    for (int mm = 1; mm < 16; mm <<= 1)
+#pragma unroll
+        for (int q = 0; q < 8; q++) o[q] = o[q] + __shfl_xor(o[q], mm, 64);
+    if (have && (threadIdx.x & 15u) == 0) {
+        double* dst = rmsBlocks + size_t(refCell / 16) * 8;
+        *reinterpret_cast<double4*>(dst) = make_double4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<double4*>(dst + 4) = make_double4(o[4], o[5], o[6], o[7]);
+    }
 }
 
 // per-workgroup min dt / max |R| partial, combined later by k_reduce_stage (device-scope
@@ -819,6 +851,7 @@ __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageAr
     if (a.checkActive && !a.sc->active) return;
     const uint32_t i = xcdBlock(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
     double dtCand = INFINITY, rCand = 0.0;
+    double qNew[4] = {0, 0, 0, 0}, qOld[4] = {0, 0, 0, 0};
     if (i < m.N) {
         double r[4] = {0.0, 0.0, 0.0, 0.0};
         double sAve = 0;
@@ -831,8 +864,10 @@ __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageAr
             else { r[0] = r[0] + lf.x; r[1] = r[1] + lf.y; r[2] = r[2] + lf.z; r[3] = r[3] + lf.w; }
             sAve += a.lenS[f];
         }
-        cellStage<KIND>(m, p, a, i, r, sAve, nullptr, nullptr, dtCand, rCand);
+        cellStage<KIND>(m, p, a, i, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld);
     }
+    if ((KIND == SK_EULER || KIND == SK_RK2_2 || KIND == SK_TVD_2 || KIND == SK_RK4_4) && a.rmsBlocks)
+        rmsBlockLeaf(i < m.N, qNew, qOld, i < m.N ? m.refId[i] : 0u, a.rmsBlocks);
     blockPartial(dtCand, rCand, a.partial);
 }
 
@@ -845,27 +880,26 @@ __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageAr
 constexpr int TBLK = 256;
 __global__ void k_reduce_stage(const double* __restrict__ partial, uint32_t n, DevScalars* sc, int checkActive);
 
-// Workgroup barrier that orders LDS only.  __syncthreads() also waits for every
-// outstanding global load and store of the wave, which would stall on the next
-// tile's prefetch; the tile kernel shares data between threads through LDS only.
-DEV void ldsBarrier()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
+// Workgroup barrier for data shared through LDS: waits for this wave's LDS
+// operations only.  __syncthreads() (and an LDS-scope fence, which covers the
+// direct global-to-LDS copies) also waits for every outstanding global access,
+// which would stall on the next tile's prefetch.  The "memory" clobber keeps the
+// compiler from moving memory accesses across it.
+DEV void ldsBarrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// Persistent form: a workgroup walks a contiguous run of tiles; while it computes
-// one tile, the next tile's blob is already in flight into registers.
-constexpr int TILE_PREFETCH = TILE_BLOB_MAX / (16 * TBLK);   // 16-byte chunks per thread
+// Persistent form: a workgroup walks a contiguous run of tiles.  Two LDS buffers
+// alternate: while the workgroup computes one tile, the next tile's blob is
+// copied into the other buffer by direct global-to-LDS loads.
+typedef __attribute__((address_space(3))) void* LdsPtr;
 
-DEV void loadBlob(const TileArgs& t, const uint4& d1, uint4 (&pre)[TILE_PREFETCH])
+DEV void dmaBlob(const TileArgs& t, const uint4& d1, uint8_t* slot)
 {
+    const uint32_t n = d1.z, lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint4* src = t.blob + d1.y;
-    const uint32_t last = d1.z - 1;   // blobs are never empty
-    // unconditional loads (clamped index) so that all of them are in flight at once
-#pragma unroll
-    for (int u = 0; u < TILE_PREFETCH; u++) pre[u] = src[min(u * TBLK + threadIdx.x, last)];
+    for (uint32_t g = w; g * 64 < n; g += TBLK / 64) {
+        const uint32_t c = g * 64 + lane;
+        if (c < n) __builtin_amdgcn_global_load_lds((const void*)(src + c), (LdsPtr)(slot + g * 1024), 16, 0, 0);
+    }
 }
 
 template <int S, int FLUX, int VISC, int LP>
@@ -873,7 +907,8 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
 {
     if (a.checkActive && !a.sc->active) return;
     extern __shared__ uint4 ldsv[];
-    uint8_t* base = reinterpret_cast<uint8_t*>(ldsv);
+    uint8_t* lds = reinterpret_cast<uint8_t*>(ldsv);
+    uint8_t* dyn = lds + 2 * t.slotBytes;
     const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
     if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
     const uint32_t per = t.nTiles / gridDim.x, rem = t.nTiles % gridDim.x;
@@ -881,29 +916,29 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
     constexpr int LW = LP ? 8 : 4;
     const uint32_t tid = threadIdx.x;
     double dtCand = INFINITY, rCand = 0.0;
-    uint4 pre[TILE_PREFETCH], d0 = make_uint4(0, 0, 0, 0), d1 = make_uint4(0, 0, 0, 0);
+    uint4 d0 = make_uint4(0, 0, 0, 0), d1 = make_uint4(0, 0, 0, 0);
     if (tBeg < tEnd) {
         d0 = t.desc[2 * tBeg];
         d1 = t.desc[2 * tBeg + 1];
-        loadBlob(t, d1, pre);
+        dmaBlob(t, d1, lds);
     }
+    uint32_t par = 0;
     for (uint32_t tile = tBeg; tile < tEnd; tile++) {
         const uint32_t nC = d0.x, nG = d0.y, nQ = d0.z, nL = d0.w, nF = d1.x;
-        const TileImg I = tileImage(nC, nG, nQ, nL, nF, S, LW);
-#pragma unroll
-        for (int u = 0; u < TILE_PREFETCH; u++) {
-            const uint32_t k = u * TBLK + tid;
-            if (k < d1.z) ldsv[k] = pre[u];
-        }
+        const TileImg I = tileImage(nC, nG, nQ, nL, nF, S);
+        const TileDyn D = tileDyn(nG, nL, LW);
+        uint8_t* base = lds + par * t.slotBytes;
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's copies of the blob have landed
         ldsBarrier();
         const uint32_t* ids = reinterpret_cast<const uint32_t*>(base + I.ids);
-        double* qs = reinterpret_cast<double*>(base + I.qs);
+        double* qs = reinterpret_cast<double*>(dyn + D.qs);
         for (uint32_t k = tid; k < nQ; k += TBLK) {
             const double4 v = a.qin[ids[k]];
             qs[4 * k] = v.x; qs[4 * k + 1] = v.y; qs[4 * k + 2] = v.z; qs[4 * k + 3] = v.w;
         }
-        const uint4 n0 = ldsv[I.hdr / 16], n1 = ldsv[I.hdr / 16 + 1];
-        if (tile + 1 < tEnd) loadBlob(t, n1, pre);   // overlaps the compute below
+        const uint4 n0 = reinterpret_cast<const uint4*>(base + I.hdr)[0];
+        const uint4 n1 = reinterpret_cast<const uint4*>(base + I.hdr)[1];
+        if (tile + 1 < tEnd) dmaBlob(t, n1, lds + (par ^ 1u) * t.slotBytes);   // overlaps the compute below
         ldsBarrier();
         if (nL > nQ) {
             const uint16_t* gown = reinterpret_cast<const uint16_t*>(base + I.gown);
@@ -918,8 +953,8 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
             ldsBarrier();
         }
         const double2* cs = reinterpret_cast<const double2*>(base + I.cs);
-        double* gs = reinterpret_cast<double*>(base + I.gs);
-        double* ls = reinterpret_cast<double*>(base + I.ls);
+        double* gs = reinterpret_cast<double*>(dyn + D.gs);
+        double* ls = reinterpret_cast<double*>(dyn + D.ls);
         if (p.order > 1) {
             const bool lim = p.limited && a.limit;
             const uint16_t* nbS = reinterpret_cast<const uint16_t*>(base + I.nb);
@@ -1048,8 +1083,138 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
         ldsBarrier();   // the next tile overwrites the LDS image
         d0 = n0;
         d1 = n1;
+        par ^= 1u;
     }
     blockPartial(dtCand, rCand, a.partial);
+}
+
+// ---------------------------------------------------------------- faces + update
+// One workgroup per face tile: every face of the tile's cells (states,
+// gradients, limiters and ghosts as the gradient kernel left them) into LDS,
+// then the cells' residual and stage update from LDS.  Same device code as
+// k_faces and k_update, so bit-identical to them.
+template <int S, int FLUX, int VISC, int ORDER, int LP>
+__global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
+{
+    if (a.checkActive && !a.sc->active) return;
+    extern __shared__ double fsd[];
+    const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x);
+    const uint4 d = t.desc[tile];
+    const uint32_t nC = d.x, nF = d.y, tid = threadIdx.x;
+    for (uint32_t k = tid; k < nF; k += BLK) {
+        const uint4 r = t.fref[d.w + k];
+        const double* g5 = t.fgeo + 5 * size_t(d.w + k);
+        const uint32_t f = r.x;
+        const double2 n = make_double2(g5[0], g5[1]);
+        const double len = g5[4];
+        double2 mid = make_double2(0, 0);
+        if (ORDER > 1) mid = make_double2(g5[2], g5[3]);
+        double qL[4], qR[4], F[4], sMax;
+        uint64_t errKey;
+        if (f < m.Fi) {
+            double qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
+            ld4(a.qin, r.y, qa);
+            ld4(a.qin, r.z, qb);
+            if (ORDER > 1 || VISC) { loadGrad(a.grad, r.y, ga); loadGrad(a.grad, r.z, gb); }
+            double2 ca = make_double2(0, 0), cb = make_double2(0, 0);
+            if (ORDER > 1) {
+                loadLim<LP>(a.lim, r.y, lxa, lya);
+                loadLim<LP>(a.lim, r.z, lxb, lyb);
+                ca = m.cen[r.y]; cb = m.cen[r.z];
+            }
+            interiorFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, n, mid, p, qL, qR, F, sMax);
+            errKey = uint64_t(m.Fb) + f;
+        } else {
+            const uint32_t b = f - m.Fi;
+            double qo[4], qg[4], gO[8], lx[4], ly[4];
+            ld4(a.qin, r.y, qo);
+            ld4(a.qin, r.z, qg);
+            if (ORDER > 1 || VISC) loadGrad(a.grad, r.y, gO);
+            double2 co = make_double2(0, 0);
+            if (ORDER > 1) {
+                loadLim<LP>(a.lim, r.y, lx, ly);
+                co = m.cen[r.y];
+            }
+            boundaryFace<FLUX, VISC, ORDER, LP>(qo, gO, lx, ly, co, qg, m.bface[b], n, mid, p, qL, qR, F, sMax);
+            errKey = b;
+            // boundary faces (each in exactly one tile) feed the force integral
+            *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) =
+                make_double4(len * F[0], len * F[1], len * F[2], len * F[3]);
+            a.lenS[f] = len * sMax;
+        }
+        if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
+            atomicMin(&a.sc->errEdge, (uint64_t(a.stageId) << 32) | errKey);
+        double* slot = fsd + 5 * k;
+        slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
+        slot[4] = len * sMax;
+    }
+    if (tid < 5) fsd[5 * nF + tid] = 0.0;   // padding face
+    ldsBarrier();
+    double dtCand = INFINITY, rCand = 0.0;
+    // uniform trip count: the RMS leaf sums need every lane of a 16-lane group
+    for (uint32_t k0 = 0; k0 < nC; k0 += BLK) {
+        const uint32_t k = k0 + tid;
+        const bool have = k < nC;
+        double qNew[4] = {0, 0, 0, 0}, qOld[4] = {0, 0, 0, 0};
+        uint32_t gi = 0;
+        if (have) {
+            double r[4] = {0.0, 0.0, 0.0, 0.0};
+            double sAve = 0;
+#pragma unroll
+            for (int j = 0; j < S; j++) {
+                const uint32_t fe = t.cf[size_t(d.z + k) * S + j];
+                const double* lf = fsd + 5 * (fe & 0x7fffu);
+                if (fe >> 15) { r[0] = r[0] - lf[0]; r[1] = r[1] - lf[1]; r[2] = r[2] - lf[2]; r[3] = r[3] - lf[3]; }
+                else { r[0] = r[0] + lf[0]; r[1] = r[1] + lf[1]; r[2] = r[2] + lf[2]; r[3] = r[3] + lf[3]; }
+                sAve += lf[4];
+            }
+            gi = t.ids[d.z + k];
+            switch (a.kind) {
+            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
+            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
+            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
+            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
+            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
+            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
+            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
+            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
+            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
+            default: cellStage<SK_RESIDUAL>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
+            }
+        }
+        if (a.rmsBlocks) rmsBlockLeaf(have, qNew, qOld, have ? m.refId[gi] : 0u, a.rmsBlocks);
+    }
+    blockPartial(dtCand, rCand, a.partial);
+}
+
+template <int S, int FLUX, int VISC>
+static void faceTileO(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, hipStream_t st)
+{
+    const bool lp = p.order > 1 && p.limited && p.lpThresh > 0;
+    const dim3 g(t.nTiles), b(BLK);
+    if (p.order == 1) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 1, 0>), g, b, t.ldsBytes, st, m, p, a, t);
+    else if (lp) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 1>), g, b, t.ldsBytes, st, m, p, a, t);
+    else hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 0>), g, b, t.ldsBytes, st, m, p, a, t);
+}
+
+template <int S>
+static void faceTileS(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, hipStream_t st)
+{
+    if (p.flux == 1) {
+        if (p.viscous) faceTileO<S, 1, 1>(m, p, a, t, st); else faceTileO<S, 1, 0>(m, p, a, t, st);
+    } else {
+        if (p.viscous) faceTileO<S, 0, 1>(m, p, a, t, st); else faceTileO<S, 0, 0>(m, p, a, t, st);
+    }
+}
+
+void launchFaceTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, int S,
+                     hipStream_t st)
+{
+    if (t.nTiles == 0) return;
+    if (S == 3) faceTileS<3>(m, p, a, t, st);
+    else if (S == 4) faceTileS<4>(m, p, a, t, st);
+    else faceTileS<6>(m, p, a, t, st);
+    hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nTiles, a.sc, a.checkActive);
 }
 
 template <int S, int FLUX, int VISC, int LP>
@@ -1149,6 +1314,18 @@ __global__ __launch_bounds__(RBLK) void k_rms_chunks(DiagArgs d)
     for (uint32_t it = 0; it < RMS_CHUNK_BLOCKS / BPI; it++) {
         const uint32_t lb = it * BPI + threadIdx.x / 16;
         double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (d.blockSums) {
+            // leaf sums from the stage update; blocks wholly past the interior cells are zero
+            const uint64_t b = c0 + lb;
+            if ((threadIdx.x % 16) == 0 && lb < cnt) {
+                if (b * 16 < d.N)
+#pragma unroll
+                    for (int q = 0; q < 8; q++) o[q] = d.blockSums[b * 8 + q];
+#pragma unroll
+                for (int q = 0; q < 8; q++) v[q][lb] = o[q];
+            }
+            continue;
+        }
         if (lb < cnt) rmsElement(d, (c0 + lb) * 16 + (threadIdx.x % 16), o);
 #pragma unroll
         for (int m = 1; m < 16; m <<= 1)

@@ -144,6 +144,20 @@ struct ebb_solver {
     TileArgs tiles{};
     bool useTiles = false;
     uint32_t tileCells = 0;
+    DBuf<uint4> d_ftDesc, d_ftRef;
+    DBuf<uint32_t> d_ftIds;
+    DBuf<uint16_t> d_ftCf;
+    DBuf<double> d_ftGeo;
+    FaceTileArgs ftiles{};
+    bool useFaceTiles = false;
+    // device cell numbering of the tiled paths: device cell d is reference cell
+    // refOf[d] (empty: identical numbering); d_devOf maps back for the RMS sums
+    std::vector<uint32_t> refOf, devOf;
+    DBuf<uint32_t> d_devOf;
+    // RMS leaf sums computed by the stage that completes a step (split path, and
+    // face tiles cut along aligned 16-cell blocks)
+    bool rmsFused = false;
+    DBuf<double> d_rmsBlocks;
     DevScalars* d_sc = nullptr;
     DevScalars* h_sc = nullptr;  // pinned
     double* h_hist = nullptr;    // pinned
@@ -198,10 +212,14 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
     a.qin = qin; a.qn = qn; a.qout = qout; a.acc = s->d_acc.p; a.R = R;
     a.grad = s->d_grad.p; a.lim = s->d_lim.p; a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p;
     a.estate = estate; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
+    a.rmsBlocks = (last && checkActive && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
     a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
     a.checkActive = checkActive; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = stepId;
     if (s->useTiles && !estate && kind != SK_RESIDUAL) {
         launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
+    } else if (s->useFaceTiles && !estate && kind != SK_RESIDUAL) {
+        launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+        launchFaceTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream);
     } else {
         launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
         launchFaces(s->dm, s->phys, a, s->stream);
@@ -251,7 +269,8 @@ int stagesPerStep(const Config& c)
 DiagArgs diagArgs(ebb_solver* s, const double4* qnew, const double4* qold, int64_t histBase, int64_t maxSteps)
 {
     DiagArgs d{};
-    d.qnew = qnew; d.qold = qold; d.N = s->N; d.L = s->N + s->G; d.perm = nullptr;
+    d.qnew = qnew; d.qold = qold; d.N = s->N; d.L = s->N + s->G; d.perm = s->refOf.empty() ? nullptr : s->d_devOf.p;
+    d.blockSums = s->rmsFused ? s->d_rmsBlocks.p : nullptr;
     d.chunkTrees = s->d_chunkTrees.p; d.chunkMask = s->d_chunkMask.p;
     d.forceFaces = s->d_forceFaces.p; d.nForce = uint32_t(s->d_forceFaces.n);
     d.lenF = s->d_lenF.p;
@@ -306,29 +325,88 @@ int clearErrors(ebb_solver* s)
 }
 
 // Pick the tile size (tileCells, or automatically the largest multiple of 16 up
-// to 128 whose LDS lets two workgroups share a CU) and upload the tiles.
-int buildTiles(ebb_solver* s, const Layout& L)
+// to 128 whose two blob buffers and working set fit 64 KB of LDS) and upload
+// the tiles.
+// Tiled paths: choose the tile size (tileCells, or automatically: 512 cells per
+// face tile; for the fused kernel the largest multiple of 16 up to 128 whose two
+// blob buffers and working set fit 64 KB of LDS) and cut the tiles on the
+// reference numbering; with renumbering the interior cells are then numbered tile
+// by tile.  `lay` receives the device-order layout, tile t is the device cells
+// tileCellsOrder[start[t] .. start[t+1]).
+int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& start,
+              std::vector<uint32_t>& tileCellsOrder)
+{
+    const Config& c = s->cfg;
+    start.clear();
+    s->refOf.clear();
+    s->devOf.clear();
+    if (c.stagePath == 0 || ref.N == 0) { lay = std::move(ref); return EBB_OK; }
+    const long long want = c.tileCells;
+    if (want != -1 && (want < 1 || want > 1024)) return fail(s, EBB_ERR_ARG, "tileCells must be -1 or in 1..1024");
+    std::vector<uint32_t> order;
+    uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 ? 512 : 128);
+    if (c.stagePath == 2 && want <= 0) {
+        const int lp = (c.order > 1 && c.limited && c.lpThresh > 0) ? 1 : 0;
+        for (;; T -= 16) {
+            tileOrder(ref, T, c.tileOrder, order, start);
+            TileHost h;
+            std::string err;
+            if (!makeTiles(ref, order, start, h, err)) return fail(s, EBB_ERR_MESH, err);
+            if (2 * h.maxBlob + h.maxDyn[lp] <= 65536 || T <= 32) break;
+        }
+    } else {
+        tileOrder(ref, T, c.tileOrder, order, start);
+    }
+    s->tileCells = T;
+    if (!c.renumber) {   // tiles list their cells by reference number
+        lay = std::move(ref);
+        tileCellsOrder = std::move(order);
+        return EBB_OK;
+    }
+    permuteLayout(ref, order, lay);
+    s->refOf = order;
+    s->devOf.assign(ref.N, 0);
+    for (uint32_t d = 0; d < ref.N; d++) s->devOf[order[d]] = d;
+    tileCellsOrder.resize(ref.N);
+    for (uint32_t d = 0; d < ref.N; d++) tileCellsOrder[d] = d;
+    return EBB_OK;
+}
+
+int buildFaceTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start)
+{
+    FaceTileHost h;
+    std::string err;
+    if (!makeFaceTiles(L, order, start, h, err)) return fail(s, EBB_ERR_MESH, err);
+    const uint32_t lds = 40 * (h.maxF + 1);
+    if (lds > 65536) return EBB_OK;   // does not fit: split kernels
+    HIPCHK(trackUpload(s, s->d_ftDesc, h.desc));
+    HIPCHK(trackUpload(s, s->d_ftRef, h.fref));
+    HIPCHK(trackUpload(s, s->d_ftIds, h.ids));
+    HIPCHK(trackUpload(s, s->d_ftCf, h.cf));
+    HIPCHK(trackUpload(s, s->d_ftGeo, h.fgeo));
+    FaceTileArgs ft{};
+    ft.nTiles = h.nTiles;
+    ft.ldsBytes = lds;
+    ft.desc = s->d_ftDesc.p; ft.ids = s->d_ftIds.p; ft.cf = s->d_ftCf.p; ft.fref = s->d_ftRef.p; ft.fgeo = s->d_ftGeo.p;
+    s->ftiles = ft;
+    s->useFaceTiles = true;
+    return EBB_OK;
+}
+
+// Build and upload the tiles of the chosen path on the device-order layout.
+int buildTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start)
 {
     s->useTiles = false;
     s->tiles = TileArgs{};
-    const long long want = s->cfg.tileCells;
-    if (want == 0 || L.N == 0) return EBB_OK;
-    if (want < -1 || want > 1024) return fail(s, EBB_ERR_ARG, "tileCells must be -1 (auto) or in 0..1024");
+    s->useFaceTiles = false;
+    s->ftiles = FaceTileArgs{};
+    if (L.N == 0 || s->cfg.stagePath == 0) return EBB_OK;
+    if (s->cfg.stagePath == 1) return buildFaceTiles(s, L, order, start);
     const int lp = (s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0) ? 1 : 0;
-    constexpr uint32_t LDS_MAX = 65536, LDS_TWO_PER_CU = 160 * 1024 / 2;
     TileHost h;
-    uint32_t T = want > 0 ? uint32_t(want) : 128;
-    for (;;) {
-        std::string err;
-        if (!makeTiles(L, T, s->cfg.tileOrder, h, err)) return fail(s, EBB_ERR_MESH, err);
-        const uint32_t lds = h.maxEnd[lp];
-        const bool fits = lds <= LDS_MAX && h.maxBlob <= TILE_BLOB_MAX;
-        if (want > 0 || (fits && lds <= LDS_TWO_PER_CU) || T <= 64) {
-            if (!fits) return EBB_OK;   // does not fit: split kernels
-            break;
-        }
-        T -= 16;
-    }
+    std::string err;
+    if (!makeTiles(L, order, start, h, err)) return fail(s, EBB_ERR_MESH, err);
+    if (2 * h.maxBlob + h.maxDyn[lp] > 65536) return EBB_OK;   // does not fit: split kernels
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
     HIPCHK(trackUpload(s, s->d_tDesc, h.desc));
@@ -336,13 +414,27 @@ int buildTiles(ebb_solver* s, const Layout& L)
     TileArgs ta{};
     ta.nTiles = h.nTiles;
     ta.nBlocks = std::min<uint32_t>(h.nTiles, 2 * uint32_t(std::max(cus, 1)));   // two resident per CU
-    ta.ldsBytes = h.maxEnd[lp];
+    ta.slotBytes = h.maxBlob;
+    ta.ldsBytes = 2 * h.maxBlob + h.maxDyn[lp];
     ta.desc = s->d_tDesc.p;
     ta.blob = s->d_tBlob.p;
     s->tiles = ta;
-    s->tileCells = T;
     s->useTiles = true;
     return EBB_OK;
+}
+
+// Interior-cell arrays between the reference numbering and the device numbering.
+template <typename T>
+void toDevice(const ebb_solver* s, const T* ref, T* dev, size_t per = 1)
+{
+    if (s->refOf.empty()) { std::copy(ref, ref + size_t(s->N) * per, dev); return; }
+    for (uint32_t d = 0; d < s->N; d++) std::copy(ref + size_t(s->refOf[d]) * per, ref + size_t(s->refOf[d] + 1) * per, dev + size_t(d) * per);
+}
+template <typename T>
+void toReference(const ebb_solver* s, const T* dev, T* ref, size_t per = 1)
+{
+    if (s->refOf.empty()) { std::copy(dev, dev + size_t(s->N) * per, ref); return; }
+    for (uint32_t d = 0; d < s->N; d++) std::copy(dev + size_t(d) * per, dev + size_t(d + 1) * per, ref + size_t(s->refOf[d]) * per);
 }
 
 int buildDevice(ebb_solver* s)
@@ -350,7 +442,13 @@ int buildDevice(ebb_solver* s)
     const BuiltMesh& b = s->b;
     const Config& c = s->cfg;
     Layout lay;
-    makeLayout(b, lay);
+    std::vector<uint32_t> tileStart, tileOrd;
+    {
+        Layout ref;
+        makeLayout(b, ref);
+        int rc_ = planTiles(s, ref, lay, tileStart, tileOrd);
+        if (rc_) return rc_;
+    }
     s->N = lay.N; s->G = lay.G; s->Fi = lay.Fi; s->Fb = lay.Fb; s->S = lay.S;
     const uint32_t N = s->N, F = s->Fi + s->Fb;
     s->faceOfEdge = lay.faceOfEdge;
@@ -433,7 +531,8 @@ int buildDevice(ebb_solver* s)
         src.resize(N);
         for (uint32_t i = 0; i < N; i++) {
             double S4[4];
-            mmsSourceTerm(b.cx[i], b.cy[i], c.viscosity, c.R, p.mu, c.Pr, S4);
+            const uint32_t r = refId[i];
+            mmsSourceTerm(b.cx[r], b.cy[r], c.viscosity, c.R, p.mu, c.Pr, S4);
             src[i] = make_double4(S4[0], S4[1], S4[2], S4[3]);
         }
     }
@@ -476,8 +575,13 @@ int buildDevice(ebb_solver* s)
     HIPCHK((hipError_t)trackAlloc(s, s->d_lenS, size_t(F + 1)));
     if (c.localTimestep) HIPCHK((hipError_t)trackAlloc(s, s->d_celldt, N));
     // physics parameters are set above; the tile builder needs them for the LDS size
-    { int rc_ = buildTiles(s, lay); if (rc_) return rc_; }
-    HIPCHK((hipError_t)trackAlloc(s, s->d_partial, 2 * std::max<size_t>((N + 255) / 256, s->tiles.nBlocks) + 2));
+    { int rc_ = buildTiles(s, lay, tileOrd, tileStart); if (rc_) return rc_; }
+    if (!s->refOf.empty()) HIPCHK(trackUpload(s, s->d_devOf, s->devOf));
+    // leaf blocks need 16 aligned consecutive reference cells per 16 lanes of the last stage
+    s->rmsFused = !s->useTiles && ((s->useFaceTiles || !s->refOf.empty()) ? c.tileOrder == TILE_BLOCKS : true);
+    if (s->rmsFused) HIPCHK((hipError_t)trackAlloc(s, s->d_rmsBlocks, size_t((N + 15) / 16) * 8));
+    HIPCHK((hipError_t)trackAlloc(s, s->d_partial,
+                                  2 * std::max<size_t>({(N + 255) / 256, s->tiles.nBlocks, s->ftiles.nTiles}) + 2));
     HIPCHK((hipError_t)trackAlloc(s, s->d_hist, size_t(s->histCap) * HIST_FIELDS));
     const uint64_t nb = uint64_t(C) / 16;
     const uint64_t nchunks = (nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
@@ -501,7 +605,7 @@ int buildDevice(ebb_solver* s)
     for (uint32_t i = 0; i < N; i++) {
         if (c.mms) {
             double qq[4];
-            mmsSolution(b.cx[i], b.cy[i], qq);
+            mmsSolution(b.cx[refId[i]], b.cy[refId[i]], qq);
             q0[i] = make_double4(qq[0], qq[1], qq[2], qq[3]);
         } else {
             q0[i] = make_double4(fs[0], fs[1], fs[2], fs[3]);
@@ -625,12 +729,14 @@ int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, 
 
 int ebb_mesh_tile_stats(const ebb_mesh* m, int tile_cells, int order, double out[8])
 {
-    if (!m || !out || tile_cells <= 0 || tile_cells > 1024 || order < 0 || order > 1) return EBB_ERR_ARG;
+    if (!m || !out || tile_cells <= 0 || tile_cells > 1024 || order < 0 || order > 2) return EBB_ERR_ARG;
     Layout L;
     makeLayout(m->b, L);
     TileHost h;
     std::string err;
-    if (!makeTiles(L, uint32_t(tile_cells), order, h, err)) {
+    std::vector<uint32_t> ord, start;
+    tileOrder(L, uint32_t(tile_cells), order, ord, start);
+    if (!makeTiles(L, ord, start, h, err)) {
         std::fprintf(stderr, "ebb_mesh_tile_stats: %s\n", err.c_str());
         return EBB_ERR_MESH;
     }
@@ -640,7 +746,7 @@ int ebb_mesh_tile_stats(const ebb_mesh* m, int tile_cells, int order, double out
     out[2] = double(h.sumG) / n; out[3] = double(h.sumQ) / n; out[4] = double(h.sumL) / n;
     out[5] = double(h.sumF) / n;
     out[6] = h.maxL;
-    out[7] = h.maxEnd[0];
+    out[7] = 2.0 * h.maxBlob + h.maxDyn[0];
     return EBB_OK;
 }
 
@@ -751,29 +857,60 @@ int ebb_solver_create(const ebb_mesh* m, const ebb_config* c, int device, ebb_so
     return EBB_OK;
 }
 
+// Copy an N-row array (rows of `per` doubles) between host (reference numbering)
+// and device (device numbering).
+static int uploadCells(ebb_solver* s, const double* host, void* devPtr, size_t per)
+{
+    const size_t n = size_t(s->N) * per;
+    std::vector<double> tmp;
+    const double* src = host;
+    if (!s->refOf.empty()) {
+        tmp.resize(n);
+        toDevice(s, host, tmp.data(), per);
+        src = tmp.data();
+    }
+    HIPCHK(hipMemcpyAsync(devPtr, src, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EBB_OK;
+}
+
+static int downloadCells(ebb_solver* s, const void* devPtr, double* host, size_t per)
+{
+    const size_t n = size_t(s->N) * per;
+    if (s->refOf.empty()) {
+        HIPCHK(hipMemcpyAsync(host, devPtr, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return EBB_OK;
+    }
+    std::vector<double> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), devPtr, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    toReference(s, tmp.data(), host, per);
+    return EBB_OK;
+}
+
 int ebb_state_upload(ebb_solver* s, const double* q)
 {
     if (!s || !q) return EBB_ERR_ARG;
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpyAsync(s->qcur(), q, size_t(s->N) * sizeof(double4), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return EBB_OK;
+    return uploadCells(s, q, s->qcur(), 4);
 }
 
 int ebb_state_download(ebb_solver* s, double* q)
 {
     if (!s || !q) return EBB_ERR_ARG;
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpyAsync(q, s->qcur(), size_t(s->N) * sizeof(double4), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return EBB_OK;
+    return downloadCells(s, s->qcur(), q, 4);
 }
 
 int ebb_state_download_all(ebb_solver* s, double* q)
 {
     if (!s || !q) return EBB_ERR_ARG;
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpyAsync(q, s->qcur(), size_t(s->N + s->G) * sizeof(double4), hipMemcpyDeviceToHost, s->stream));
+    int rc = downloadCells(s, s->qcur(), q, 4);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(q + size_t(s->N) * 4, s->qcur() + s->N, size_t(s->G) * sizeof(double4), hipMemcpyDeviceToHost,
+                          s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return EBB_OK;
 }
@@ -793,7 +930,7 @@ int ebb_residual(ebb_solver* s, int limit, int dt_update, double* R, double* new
     stage(s, SK_RESIDUAL, s->qcur(), s->qcur(), nullptr, limit, dt_update, 0, 0, 0, s->d_R.p, s->d_estate.p, 0);
     HIPCHK(hipGetLastError());
     { int rc_ = readScalars(s); if (rc_) return rc_; }
-    if (R) HIPCHK(hipMemcpy(R, s->d_R.p, size_t(s->N) * sizeof(double4), hipMemcpyDeviceToHost));
+    if (R) { int rc_ = downloadCells(s, s->d_R.p, R, 4); if (rc_) return rc_; }
     double nd, rm;
     std::memcpy(&nd, &s->h_sc->newDtBits, 8);
     std::memcpy(&rm, &s->h_sc->rmaxBits, 8);
@@ -978,11 +1115,11 @@ int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
     HIPCHK(hipSetDevice(s->device));
     // the fused stage kernel keeps gradients in LDS: recompute them for the last stage input
     if (s->useTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
-    if (grad) HIPCHK(hipMemcpy(grad, s->d_grad.p, size_t(s->N) * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    if (grad) { int rc_ = downloadCells(s, s->d_grad.p, grad, 8); if (rc_) return rc_; }
     if (lim) {
         const bool lp = s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0;
         std::vector<double> l(size_t(s->N) * (lp ? 8 : 4));
-        HIPCHK(hipMemcpy(l.data(), s->d_lim.p, l.size() * sizeof(double), hipMemcpyDeviceToHost));
+        { int rc_ = downloadCells(s, s->d_lim.p, l.data(), lp ? 8 : 4); if (rc_) return rc_; }
         for (uint32_t i = 0; i < s->N; i++)
             for (int k = 0; k < 4; k++) {
                 lim[size_t(i) * 8 + 2 * k] = lp ? l[size_t(i) * 8 + 2 * k] : l[size_t(i) * 4 + k];
@@ -1012,7 +1149,7 @@ static int saveSolutionImpl(ebb_solver* s, const char* path, const double* tdt)
     const uint32_t F = s->Fi + s->Fb, N = s->N;
     std::vector<double4> es(size_t(F) * 2), q(N);
     HIPCHK(hipMemcpy(es.data(), s->d_estate.p, es.size() * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(q.data(), s->qcur(), size_t(N) * sizeof(double4), hipMemcpyDeviceToHost));
+    { int rc_ = downloadCells(s, s->qcur(), reinterpret_cast<double*>(q.data()), 4); if (rc_) return rc_; }
     { int rc_ = readScalars(s); if (rc_) return rc_; }
     if (!s->haveStage) {
         // no residual evaluated yet: edge states are the set-up values (far states)
@@ -1088,7 +1225,7 @@ int ebb_load_solution(ebb_solver* s, const char* path)
         return EBB_ERR_PARSE;
     }
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpy(s->qcur(), q.data(), size_t(s->N) * sizeof(double4), hipMemcpyHostToDevice));
+    { int rc_ = uploadCells(s, reinterpret_cast<const double*>(q.data()), s->qcur(), 4); if (rc_) return rc_; }
     return ebb_run_reset(s, t, dt);
 }
 
@@ -1151,11 +1288,18 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
             a.qin = x.in; a.qn = q; a.qout = x.out; a.acc = s->d_acc.p; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
             a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
             a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
+            a.rmsBlocks = (x.last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
             a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
             if (s->useTiles) {
                 launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
                 HIPCHK(hipEventRecord(ev[k++], s->stream));
                 part.push_back(3);
+            } else if (s->useFaceTiles) {
+                launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+                HIPCHK(hipEventRecord(ev[k++], s->stream));
+                launchFaceTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream);
+                HIPCHK(hipEventRecord(ev[k++], s->stream));
+                part.push_back(0); part.push_back(3);
             } else {
                 launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
                 HIPCHK(hipEventRecord(ev[k++], s->stream));
@@ -1208,6 +1352,8 @@ void ebb_solver_destroy(ebb_solver* s)
     s->d_src.release(); s->d_qA.release(); s->d_qB.release(); s->d_T.release(); s->d_T2.release();
     s->d_acc.release(); s->d_R.release(); s->d_estate.release();
     s->d_tDesc.release(); s->d_tBlob.release();
+    s->d_ftDesc.release(); s->d_ftRef.release(); s->d_ftIds.release(); s->d_ftCf.release(); s->d_ftGeo.release();
+    s->d_devOf.release(); s->d_rmsBlocks.release();
     if (s->d_sc) (void)hipFree(s->d_sc);
     if (s->h_sc) (void)hipHostFree(s->h_sc);
     if (s->h_hist) (void)hipHostFree(s->h_hist);

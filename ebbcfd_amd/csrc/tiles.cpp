@@ -63,6 +63,35 @@ void makeLayout(const BuiltMesh& b, Layout& L)
     }
 }
 
+void permuteLayout(const Layout& ref, const std::vector<uint32_t>& order, Layout& L)
+{
+    const uint32_t N = ref.N, S = ref.S, F = ref.Fi + ref.Fb;
+    std::vector<uint32_t> dev(N);
+    for (uint32_t d = 0; d < N; d++) dev[order[d]] = d;
+    auto map = [&](uint32_t c) { return c < N ? dev[c] : c; };
+    L = Layout{};
+    L.N = ref.N; L.G = ref.G; L.Fi = ref.Fi; L.Fb = ref.Fb; L.S = ref.S;
+    L.faceOfEdge = ref.faceOfEdge;
+    L.edgeOfFace = ref.edgeOfFace;
+    L.nbr.resize(size_t(N) * S); L.cface.resize(size_t(N) * S); L.refId.resize(N);
+    L.gm.resize(size_t(N) * 2 * S); L.nia.resize(N); L.cd.resize(N); L.cper.resize(N);
+    L.cen = ref.cen;
+    for (uint32_t d = 0; d < N; d++) {
+        const uint32_t r = order[d];
+        for (uint32_t j = 0; j < S; j++) {
+            L.nbr[size_t(d) * S + j] = map(ref.nbr[size_t(r) * S + j]);
+            L.cface[size_t(d) * S + j] = ref.cface[size_t(r) * S + j];
+        }
+        for (uint32_t k = 0; k < 2 * S; k++) L.gm[size_t(d) * 2 * S + k] = ref.gm[size_t(r) * 2 * S + k];
+        L.nia[d] = ref.nia[r]; L.cd[d] = ref.cd[r]; L.cper[d] = ref.cper[r];
+        L.refId[d] = ref.refId[r];
+        L.cen[d] = ref.cen[r];
+    }
+    L.fn = ref.fn; L.fmid = ref.fmid; L.flen = ref.flen;
+    L.fcell.resize(F);
+    for (uint32_t f = 0; f < F; f++) L.fcell[f] = make_uint2(map(ref.fcell[f].x), map(ref.fcell[f].y));
+}
+
 namespace {
 
 uint64_t spread21(uint64_t v)
@@ -97,6 +126,46 @@ void mortonTiles(const Layout& L, uint32_t T, std::vector<uint32_t>& order, std:
     std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
     start.clear();
     for (uint32_t k = 0; k < N; k += T) start.push_back(k);
+    start.push_back(N);
+}
+
+// aligned blocks of 16 consecutive cells in Morton order of their centroids, T/16 per tile
+void blockTiles(const Layout& L, uint32_t T, std::vector<uint32_t>& order, std::vector<uint32_t>& start)
+{
+    const uint32_t N = L.N, nb = (N + 15) / 16, per = std::max<uint32_t>(1, T / 16);
+    std::vector<double2> bc(nb);
+    for (uint32_t b = 0; b < nb; b++) {
+        double x = 0, y = 0;
+        const uint32_t e = std::min(N, 16 * b + 16);
+        for (uint32_t i = 16 * b; i < e; i++) { x += L.cen[i].x; y += L.cen[i].y; }
+        bc[b] = make_double2(x / (e - 16 * b), y / (e - 16 * b));
+    }
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (uint32_t b = 0; b < nb; b++) {
+        x0 = std::min(x0, bc[b].x); x1 = std::max(x1, bc[b].x);
+        y0 = std::min(y0, bc[b].y); y1 = std::max(y1, bc[b].y);
+    }
+    const double sx = x1 > x0 ? 2097151.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 2097151.0 / (y1 - y0) : 0.0;
+    std::vector<uint64_t> key(nb);
+    for (uint32_t b = 0; b < nb; b++) {
+        const uint64_t ux = uint64_t(std::min(2097151.0, std::max(0.0, (bc[b].x - x0) * sx)));
+        const uint64_t uy = uint64_t(std::min(2097151.0, std::max(0.0, (bc[b].y - y0) * sy)));
+        key[b] = spread21(ux) | (spread21(uy) << 1);
+    }
+    const bool shortLast = (N % 16) != 0;
+    std::vector<uint32_t> blocks;
+    for (uint32_t b = 0; b < nb; b++)
+        if (!(shortLast && b == nb - 1)) blocks.push_back(b);
+    std::sort(blocks.begin(), blocks.end(), [&](uint32_t a, uint32_t b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
+    if (shortLast) blocks.push_back(nb - 1);
+    order.clear();
+    order.reserve(N);
+    start.clear();
+    for (size_t k = 0; k < blocks.size(); k++) {
+        if (k % per == 0) start.push_back(uint32_t(order.size()));
+        const uint32_t b = blocks[k], e = std::min(N, 16 * b + 16);
+        for (uint32_t i = 16 * b; i < e; i++) order.push_back(i);
+    }
     start.push_back(N);
 }
 
@@ -172,14 +241,65 @@ void growTiles(const Layout& L, uint32_t T, std::vector<uint32_t>& order, std::v
 
 }  // namespace
 
-bool makeTiles(const Layout& L, uint32_t T, int orderKind, TileHost& h, std::string& err)
+void tileOrder(const Layout& L, uint32_t T, int kind, std::vector<uint32_t>& order, std::vector<uint32_t>& start)
+{
+    if (L.N == 0 || T == 0) { order.clear(); start.assign(1, 0); return; }
+    if (kind == TILE_MORTON) mortonTiles(L, T, order, start);
+    else if (kind == TILE_BLOCKS) blockTiles(L, T, order, start);
+    else growTiles(L, T, order, start);
+}
+
+bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
+                   FaceTileHost& h, std::string& err)
+{
+    const uint32_t N = L.N, S = L.S, F = L.Fi + L.Fb;
+    h = FaceTileHost{};
+    if (N == 0 || start.size() < 2) return true;
+    const uint32_t nT = uint32_t(start.size() - 1);
+    h.nTiles = nT;
+    h.desc.resize(nT);
+    std::vector<uint32_t> fstamp(F, ~0u), flidx(F, 0), fl;
+    for (uint32_t t = 0; t < nT; t++) {
+        const uint32_t c0 = start[t], nC = start[t + 1] - c0;
+        fl.clear();
+        for (uint32_t k = 0; k < nC; k++)
+            for (uint32_t j = 0; j < S; j++) {
+                const uint32_t f = L.cface[size_t(order[c0 + k]) * S + j] & 0x7fffffffu;
+                if (f == F || fstamp[f] == t) continue;
+                fstamp[f] = t;
+                flidx[f] = uint32_t(fl.size());
+                fl.push_back(f);
+            }
+        const uint32_t nF = uint32_t(fl.size());
+        if (nF >= 32767) { err = "tile has too many faces; lower tileCells"; return false; }
+        h.desc[t] = make_uint4(nC, nF, uint32_t(h.ids.size()), uint32_t(h.fref.size()));
+        for (uint32_t k = 0; k < nC; k++) {
+            const uint32_t c = order[c0 + k];
+            h.ids.push_back(c);
+            for (uint32_t j = 0; j < S; j++) {
+                const uint32_t fe = L.cface[size_t(c) * S + j], f = fe & 0x7fffffffu;
+                h.cf.push_back(f == F ? uint16_t(nF) : uint16_t(flidx[f] | ((fe >> 31) ? 0x8000u : 0u)));
+            }
+        }
+        for (uint32_t k = 0; k < nF; k++) {
+            const uint32_t f = fl[k];
+            const uint32_t ca = L.fcell[f].x, cb = f < L.Fi ? L.fcell[f].y : N + (f - L.Fi);
+            h.fref.push_back(make_uint4(f, ca, cb, 0));
+            const double g5[5] = {L.fn[f].x, L.fn[f].y, L.fmid[f].x, L.fmid[f].y, L.flen[f]};
+            h.fgeo.insert(h.fgeo.end(), g5, g5 + 5);
+        }
+        h.maxF = std::max(h.maxF, nF);
+        h.sumF += nF;
+    }
+    return true;
+}
+
+bool makeTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start, TileHost& h,
+               std::string& err)
 {
     const uint32_t N = L.N, S = L.S, F = L.Fi + L.Fb, Ctot = L.N + L.G;
     h = TileHost{};
-    if (N == 0 || T == 0) return true;
-    std::vector<uint32_t> order, start;
-    if (orderKind == TILE_MORTON) mortonTiles(L, T, order, start);
-    else growTiles(L, T, order, start);
+    if (N == 0 || start.size() < 2) return true;
     const uint32_t nT = uint32_t(start.size() - 1);
     h.nTiles = nT;
     h.desc.resize(2 * size_t(nT));
@@ -223,8 +343,8 @@ bool makeTiles(const Layout& L, uint32_t T, int orderKind, TileHost& h, std::str
             }
         const uint32_t nF = uint32_t(fl.size());
         if (nF >= 32767) { err = "tile has too many faces; lower tileCells"; return false; }
-        const TileImg I = tileImage(nC, nG, nQ, nL, nF, int(S), 4);
-        img.assign(I.blobEnd, 0);
+        const TileImg I = tileImage(nC, nG, nQ, nL, nF, int(S));
+        img.assign(I.end, 0);
         auto put = [&](uint32_t off, const void* p, size_t n) { std::memcpy(img.data() + off, p, n); };
         for (uint32_t k = 0; k < nL; k++) put(I.cs + 16 * k, &L.cen[C[k]], 16);
         for (uint32_t r = 0; r < 2; r++)
@@ -260,16 +380,16 @@ bool makeTiles(const Layout& L, uint32_t T, int orderKind, TileHost& h, std::str
                 put(I.cf + 2 * (k * S + j), &v, 2);
             }
         if (!gown.empty()) put(I.gown, gown.data(), 2 * gown.size());
-        const uint32_t boff = uint32_t(h.blob.size()), blen = I.blobEnd / 16;
+        const uint32_t boff = uint32_t(h.blob.size()), blen = I.end / 16;
         h.blob.resize(h.blob.size() + blen);
-        std::memcpy(h.blob.data() + boff, img.data(), I.blobEnd);
+        std::memcpy(h.blob.data() + boff, img.data(), I.end);
         h.desc[2 * size_t(t)] = make_uint4(nC, nG, nQ, nL);
         h.desc[2 * size_t(t) + 1] = make_uint4(nF, boff, blen, 0);
-        h.maxEnd[0] = std::max(h.maxEnd[0], I.end);
-        h.maxEnd[1] = std::max(h.maxEnd[1], tileImage(nC, nG, nQ, nL, nF, int(S), 8).end);
+        h.maxDyn[0] = std::max(h.maxDyn[0], tileDyn(nG, nL, 4).end);
+        h.maxDyn[1] = std::max(h.maxDyn[1], tileDyn(nG, nL, 8).end);
         h.sumC += nC; h.sumG += nG; h.sumQ += nQ; h.sumL += nL; h.sumF += nF;
         h.maxL = std::max(h.maxL, nL);
-        h.maxBlob = std::max(h.maxBlob, I.blobEnd);
+        h.maxBlob = std::max(h.maxBlob, I.end);
     }
     if (h.blob.size() >= (uint64_t(1) << 32)) { err = "tile data too large"; return false; }
     for (uint32_t t = 0; t + 1 < nT; t++) {   // blob header: the next tile's descriptor

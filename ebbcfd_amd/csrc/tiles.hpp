@@ -22,22 +22,44 @@ struct Layout {
     std::vector<double> flen;                        // F
 };
 void makeLayout(const BuiltMesh& b, Layout& L);
+// The same layout with interior cells renumbered: device cell d is cell order[d]
+// of `ref` (ghosts and faces keep their numbers); refId holds the reference ids.
+void permuteLayout(const Layout& ref, const std::vector<uint32_t>& order, Layout& dev);
 
-enum TileOrder { TILE_GROW = 0, TILE_MORTON = 1 };
+enum TileOrder { TILE_GROW = 0, TILE_MORTON = 1, TILE_BLOCKS = 2 };
 
 struct TileHost {
     std::vector<uint4> desc, blob;
     uint32_t nTiles = 0;
-    uint32_t maxEnd[2] = {0, 0};   // largest LDS image: scalar limiter layout, LP layout
+    uint32_t maxDyn[2] = {0, 0};   // largest working set: scalar limiter layout, LP layout
     // totals for statistics
     uint64_t sumC = 0, sumG = 0, sumQ = 0, sumL = 0, sumF = 0;
     uint32_t maxL = 0;
     uint32_t maxBlob = 0;          // bytes
 };
 
+struct FaceTileHost {
+    std::vector<uint4> desc, fref;
+    std::vector<uint32_t> ids;
+    std::vector<uint16_t> cf;
+    std::vector<double> fgeo;
+    uint32_t nTiles = 0, maxF = 0;
+    uint64_t sumF = 0;
+};
+
 // Cut the interior cells into tiles of at most T cells (TILE_GROW: compact regions
-// grown greedily over the cell graph; TILE_MORTON: runs of the Morton order of
-// the centroids) and pack each tile's halo, tables and static data.
-bool makeTiles(const Layout& L, uint32_t T, int order, TileHost& h, std::string& err);
+// grown greedily over the cell graph; TILE_MORTON: runs of the Morton order of the
+// centroids; TILE_BLOCKS: runs of T/16 aligned blocks of 16 consecutive cells in
+// Morton order of the block centroids, each block in its own order, the short last
+// block last): tile t is cells order[start[t] .. start[t+1]).
+void tileOrder(const Layout& L, uint32_t T, int kind, std::vector<uint32_t>& order, std::vector<uint32_t>& start);
+
+// Face tiles: each cell group with the faces of its cells.
+bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
+                   FaceTileHost& h, std::string& err);
+
+// Fused-kernel tiles: each cell group with its halo, tables and static data packed.
+bool makeTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start, TileHost& h,
+               std::string& err);
 
 }  // namespace ebb

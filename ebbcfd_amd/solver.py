@@ -121,7 +121,7 @@ class UMesh2:
     def tile_stats(self, tile_cells=128, order="grow"):
         """Tiles of the fused stage kernel for this mesh (host only)."""
         out = np.zeros(8)
-        _check(lib().ebb_mesh_tile_stats(self.h, int(tile_cells), {"grow": 0, "morton": 1}[order], _p(out)),
+        _check(lib().ebb_mesh_tile_stats(self.h, int(tile_cells), {"grow": 0, "morton": 1, "blocks": 2}[order], _p(out)),
                "tile stats")
         keys = ("tiles", "cells_per_tile", "grad_ratio", "load_ratio", "local_ratio", "faces_per_cell",
                 "max_local", "lds_bytes")

@@ -73,7 +73,7 @@ int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, 
 
 /* Tiles the fused stage kernel would use for this mesh (no equivalent in the
  * reference; host only, needs no GPU).  tile_cells > 0; order 0: greedily
- * grown regions, 1: Morton order.  out[0] tiles, out[1] mean own cells per tile,
+ * grown regions, 1: Morton order of cells, 2: Morton order of 16-cell blocks.  out[0] tiles, out[1] mean own cells per tile,
  * out[2..5] cells with gradients (own + ring 1), cells loaded (own + 2 rings),
  * local cells incl. ghosts and faces, each summed over tiles and divided by the
  * interior cell count; out[6] largest local cell count; out[7] LDS bytes per
@@ -87,10 +87,13 @@ void ebb_mesh_destroy(ebb_mesh* m);
 /* loadConfig (source/config.d:112-313): the reference JSON keys and defaults
  * ("boudaryConditions", "dt" as a JSON float, ...).  Extra options live in an
  * optional "extensions" object: {"mms": bool, "maxSteps": int,
- * "precision": "fp64"|"mixed", "tileCells": int (0, the default: the split
- * gradient/face/update kernels; n: the fused kernel with tiles of up to n
- * cells; -1: the fused kernel with an automatic tile size),
- * "tileOrder": "grow"|"morton"}. */
+ * "precision": "fp64"|"mixed", "stagePath": "faceTiles" (default: gradient
+ * kernel + one face/update kernel over tiles of cells) | "split" (gradient,
+ * face and update kernels) | "fused" (one kernel per stage over tiles with
+ * halos), "tileCells": int (cells per tile, -1 automatic), "tileOrder":
+ * "blocks" (default: Morton order of aligned 16-cell blocks) | "morton" |
+ * "grow", "cellOrder": "reference" (default) | "tiles" (renumber the device
+ * cells tile by tile; every interface still uses the reference numbering)}. */
 int ebb_config_parse(const char* json, ebb_config** out);
 
 /* Read / override a numeric field by its JSON key: "dt", "tEnd", "CFL",
@@ -167,10 +170,10 @@ int ebb_write_forces(ebb_solver* s, const char* path, int64_t first, int64_t cou
 int ebb_last_error(ebb_solver* s, int* kind, int64_t* index, int64_t* step, char* msg, size_t len);
 
 /* Device timing of `steps` integrator steps with HIP events on the solver's
- * stream.  out[0..2]: average ms per stage of the split gradient, face and
- * update kernels; out[3]: average ms per stage of the fused tile kernel (with
- * its stage reduction); the unused path reads 0.  out[4]: ms per step of
- * the stage kernels; out[5]: stages per step. */
+ * stream.  out[0..2]: average ms per stage of the gradient, face and update
+ * kernels; out[3]: average ms per stage of the tile kernel (face/update or
+ * fused, with its stage reduction); kernels the path does not use read 0.
+ * out[4]: ms per step of the stage kernels; out[5]: stages per step. */
 int ebb_time_steps(ebb_solver* s, int steps, double out[6]);
 
 /* Device bytes held by the solver. */

@@ -1,9 +1,11 @@
-"""The fused tile stage kernel against the split kernels and the oracle.
+"""The tiled stage paths against the split kernels and the oracle.
 
-The fused kernel evaluates ghosts, gradients/limiters, face fluxes and the stage
-update of a tile of cells in one workgroup; faces on tile borders are computed by
-both tiles with the same device code.  Every result must be bit-identical to the
-three-kernel path (tileCells = 0), for tile sizes down to one cell.
+"faceTiles": the gradient kernel, then one kernel that computes every face of a
+tile of cells into LDS and updates those cells.  "fused": one kernel per stage
+that also computes ghosts and gradients/limiters for the tile and its halo.
+Faces on tile borders are computed by both tiles with the same device code, so
+every result must be bit-identical to the three-kernel path, for tile sizes
+down to one cell.
 """
 import copy
 
@@ -19,6 +21,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = {
     "euler_roe": lambda: channel_case(integrator="Euler"),
+    "rk2tvd_odd_count": lambda: channel_case(nx=25, ny=7, integrator="RK2_TVD"),   # 350 cells: short last block
     "rk2_roe_order1": lambda: channel_case(integrator="RK2", order=1),
     "rk2tvd_rusanov_sym_visc": lambda: channel_case(integrator="RK2_TVD", flux="rusanovFlux", walls="symmetry",
                                                     viscosity=True),
@@ -32,20 +35,25 @@ CASES = {
 }
 
 
-def with_tiles(cfg, n):
+def with_path(cfg, path, n=-1, order="blocks", cells="reference"):
     c = copy.deepcopy(cfg)
     ext = dict(c.get("extensions", {}))
-    ext["tileCells"] = n
+    ext.update(stagePath=path, tileCells=n, tileOrder=order, cellOrder=cells)
     c["extensions"] = ext
     return c
 
 
-@pytest.mark.parametrize("tile", [1, 13, 64, 128])
+@pytest.mark.parametrize("path,tile,order,cells", [
+    ("split", -1, "blocks", "reference"),
+    ("faceTiles", -1, "blocks", "reference"), ("faceTiles", 32, "blocks", "tiles"),
+    ("faceTiles", 29, "morton", "reference"), ("faceTiles", 1, "grow", "tiles"), ("faceTiles", 256, "grow", "reference"),
+    ("fused", 1, "grow", "tiles"), ("fused", 13, "grow", "reference"), ("fused", 64, "blocks", "tiles"),
+    ("fused", 128, "morton", "reference")])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_fused_equals_split(name, tile):
+def test_tiled_equals_split(name, path, tile, order, cells):
     raw, cfg = CASES[name]()
-    split = Solver(build_mesh(raw), with_tiles(cfg, 0))
-    fused = Solver(build_mesh(raw), with_tiles(cfg, tile))
+    split = Solver(build_mesh(raw), with_path(cfg, "split", order="grow"))
+    fused = Solver(build_mesh(raw), with_path(cfg, path, tile, order, cells))
     h_s, t_s, dt_s = split.run(15)
     h_f, t_f, dt_f = fused.run(15)
     np.testing.assert_array_equal(h_f, h_s)
@@ -61,9 +69,10 @@ def test_fused_equals_split(name, tile):
     np.testing.assert_array_equal(fl_f, fl_s)
 
 
-def test_fused_matches_oracle_many_tiles():
+@pytest.mark.parametrize("path", ["fused", "faceTiles"])
+def test_tiled_matches_oracle_many_tiles(path):
     raw, cfg = cylinder_case(n_theta=96, n_r=40, integrator="RK4", flux="roeFlux")
-    cfg = with_tiles(cfg, 32)
+    cfg = with_path(cfg, path, 32)
     om = oracle.Mesh(raw).setup(oracle_cfg(cfg))
     s = Solver(build_mesh(raw), cfg)
     h_o, t_o, _ = om.run(25)
@@ -72,25 +81,26 @@ def test_fused_matches_oracle_many_tiles():
     np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
 
 
-def test_fused_large_mesh_equals_split():
+@pytest.mark.parametrize("path", ["fused", "faceTiles"])
+def test_tiled_large_mesh_equals_split(path):
     raw, cfg = cylinder_case(n_theta=512, n_r=256, stretch=1.015, CFL=0.05)
-    split = Solver(build_mesh(raw), with_tiles(cfg, 0))
-    fused = Solver(build_mesh(raw), with_tiles(cfg, -1))   # automatic tile size
+    split = Solver(build_mesh(raw), with_path(cfg, "split"))
+    tiled = Solver(build_mesh(raw), with_path(cfg, path))   # automatic tile size
     h_s, _, _ = split.run(12)
-    h_f, _, _ = fused.run(12)
+    h_f, _, _ = tiled.run(12)
     np.testing.assert_array_equal(h_f, h_s)
-    np.testing.assert_array_equal(fused.q, split.q)
-    tf = fused.time_steps(2)
+    np.testing.assert_array_equal(tiled.q, split.q)
+    tf = tiled.time_steps(2)
     ts = split.time_steps(2)
     assert tf["fused_ms"] > 0 and tf["faces_ms"] == 0
     assert ts["fused_ms"] == 0 and ts["faces_ms"] > 0
 
 
-@pytest.mark.parametrize("tile", [0, 5, 128])
-def test_fused_nan_cell_and_edge_reported_like_split(tile):
+@pytest.mark.parametrize("path,tile", [("split", -1), ("fused", 5), ("fused", 128), ("faceTiles", 7)])
+def test_tiled_nan_cell_and_edge_reported_like_split(path, tile):
     raw, cfg = channel_case(integrator="RK2")
-    ref = Solver(build_mesh(raw), with_tiles(cfg, 0))
-    s = Solver(build_mesh(raw), with_tiles(cfg, tile))
+    ref = Solver(build_mesh(raw), with_path(cfg, "split"))
+    s = Solver(build_mesh(raw), with_path(cfg, path, tile))
     for x in (ref, s):
         q = x.q
         q[17, 0] = -1.0
@@ -101,3 +111,33 @@ def test_fused_nan_cell_and_edge_reported_like_split(tile):
         s.run(3)
     assert eg.value.code == er.value.code
     assert eg.value.index == er.value.index and eg.value.step == er.value.step
+
+
+@pytest.mark.parametrize("path", ["fused", "faceTiles"])
+def test_renumbered_interfaces_match_split(path, tmp_path):
+    """Tiled paths renumber the device cells; every interface still speaks the reference numbering."""
+    raw, cfg = channel_case(integrator="RK2_TVD", viscosity=True)
+    split = Solver(build_mesh(raw), with_path(cfg, "split"))
+    tiled = Solver(build_mesh(raw), with_path(cfg, path, 17, "morton", "tiles"))
+    q = split.q
+    q[:, 0] *= 1.0 + 0.01 * np.sin(np.arange(len(q)))
+    split.q = q
+    tiled.q = q
+    np.testing.assert_array_equal(tiled.q, q)
+    for x in (split, tiled):
+        x.run(4)
+    np.testing.assert_array_equal(tiled.q_all(), split.q_all())
+    R_s, nd_s, rm_s = split.ufvm_solver()
+    R_t, nd_t, rm_t = tiled.ufvm_solver()
+    np.testing.assert_array_equal(R_t, R_s)
+    assert nd_t == nd_s and rm_t == rm_s
+    for a, b in zip(tiled.cell_gradients(), split.cell_gradients()):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(tiled.edge_states(), split.edge_states()):
+        np.testing.assert_array_equal(a, b)
+    split.save_solution(str(tmp_path / "s.esln"))
+    tiled.save_solution(str(tmp_path / "t.esln"))
+    assert (tmp_path / "s.esln").read_bytes() == (tmp_path / "t.esln").read_bytes()
+    tiled2 = Solver(build_mesh(raw), with_path(cfg, path, 17, "morton", "tiles"))
+    tiled2.load_solution(str(tmp_path / "t.esln"))
+    np.testing.assert_array_equal(tiled2.q, tiled.q)
