@@ -48,7 +48,6 @@ struct DevMesh {
     uint32_t N, G, Fi, Fb;
     const uint32_t* __restrict__ nbr;    // N*S neighbour cells (ghosts >= N)
     const uint32_t* __restrict__ cface;  // N*S face ids, bit 31 set where the cell is not the owner
-    const double* __restrict__ gm;       // N*2*S least-squares weights (dir, nbr)
     const double2* __restrict__ cen;     // N+G centroids
     const double* __restrict__ nia;      // N: -(1.0/area)
     const double* __restrict__ cd;       // N: d = 2A/perim
@@ -60,6 +59,7 @@ struct DevMesh {
     const BFace* __restrict__ bface;     // Fb
     const double4* __restrict__ src;     // N MMS source (null if none)
     const uint32_t* __restrict__ refId;  // N: reference index of each device cell
+    const double2* __restrict__ node;    // mesh node coordinates
 };
 
 struct PhysParams {
@@ -170,7 +170,7 @@ __host__ __device__ inline TileDyn tileDyn(uint32_t nG, uint32_t nL, int LW)
 // computes every face of its cells into LDS (faces between two tiles are
 // computed by both, identically) and then updates its cells.  Gradients,
 // limiters and ghost states come from the gradient kernel.  Per tile, the face
-// geometry and references are stored contiguously.
+// references are stored contiguously.
 struct FaceTileArgs {
     uint32_t nTiles;
     uint32_t ldsBytes;
@@ -178,8 +178,8 @@ struct FaceTileArgs {
     const uint32_t* ids;           // own cells' device ids
     const uint16_t* cf;            // S per own cell: local face, bit 15 set where the cell is not the owner;
                                    // nF (an all-zero slot) for padding
-    const uint4* fref;             // per face: global face id, cell a, cell b (ghost for boundary faces), 0
-    const double* fgeo;            // per face: normal x, y, midpoint x, y, length
+    const uint4* fref;             // per face: global face id, cell a, cell b (ghost for boundary faces), node 0
+    const uint32_t* fn1;           // per face: node 1 (normal, midpoint and length are recomputed from the nodes)
 };
 
 // Launchers (kernels.hip).  All run on `stream`.

@@ -311,11 +311,50 @@ DEV void lpSolve(int rows, const double* A, const double* b, const double c[2], 
 // states (ghosts already filled), LSQ weights, centroids and which slots are real
 // neighbours (padding slots repeat the cell).  MODE 1: unlimited; 2: scalar
 // limiter; 3: scalar limiter then the LP vector limiter.
+// Least-squares weights (A^T A)^-1 A^T with A_j = centroid_j - centroid_i, in the
+// operation order of the mesh set-up (source/mesh.d; mesh.cpp), so they equal the
+// stored ones bit for bit; the padding slots (A_j = 0) add exact zeros, as the
+// set-up's zero rows do.
+template <int S>
+DEV void lsqWeights(double2 ci, const double2 cn[S], double w[2][S])
+{
+    double ax[S], ay[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) { ax[j] = cn[j].x - ci.x; ay[j] = cn[j].y - ci.y; }
+    double b00 = 0.0, b01 = 0.0, b10 = 0.0, b11 = 0.0;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        b00 += ax[j] * ax[j];
+        b01 += ax[j] * ay[j];
+        b10 += ay[j] * ax[j];
+        b11 += ay[j] * ay[j];
+    }
+    const double det = b00 * b11 - b01 * b10;
+    const double id = 1.0 / det;
+    const double i00 = b11 * id, i01 = -b01 * id, i10 = -b10 * id, i11 = b00 * id;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        w[0][j] = i00 * ax[j] + i01 * ay[j];
+        w[1][j] = i10 * ax[j] + i11 * ay[j];
+    }
+}
+
+// gm: stored weights ((dir, nbr)-major, stride gstride between entries), or null
+// to compute them from the centroids
 template <int S, int MODE>
 DEV void gradLimit(const double qi[4], const double qn[S][4], const double* gm, uint32_t gstride, double2 ci,
                    const double2 cn[S], const bool real[S], int limit, double lpThresh, double g[4][2], double lx[4],
                    double ly[4])
 {
+    double w[2][S];
+    if (gm) {
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int j = 0; j < S; j++) w[r][j] = gm[(r * S + j) * gstride];
+    } else {
+        lsqWeights<S>(ci, cn, w);
+    }
     double mn[4], mx[4], du[4][S];
 #pragma unroll
     for (int k = 0; k < 4; k++) { mn[k] = qi[k]; mx[k] = qi[k]; }
@@ -333,7 +372,7 @@ DEV void gradLimit(const double qi[4], const double qn[S][4], const double* gm, 
         for (int r = 0; r < 2; r++) {
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < S; j++) s += gm[(r * S + j) * gstride] * du[k][j];
+            for (int j = 0; j < S; j++) s += w[r][j] * du[k][j];
             g[k][r] = s;
         }
 #pragma unroll
@@ -396,8 +435,9 @@ DEV void ghostFromOwner(const BFace& bf, const double qo[4], const double4* qbuf
     ghostState(bf, qo, qg, old);
 }
 
-// MODE 0: ghost states only (order 1); 1: unlimited gradient; 2: scalar limiter;
-// 3: scalar limiter followed by the LP vector limiter (lim stored as x,y pairs).
+// MODE 0: ghost states only (order 1); 1: unlimited gradient (no limiter stored);
+// 2: scalar limiter; 3: scalar limiter followed by the LP vector limiter (lim
+// stored as x,y pairs).
 template <int S, int MODE>
 __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, StageArgs a)
 {
@@ -426,7 +466,7 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
 #pragma unroll
     for (int j = 0; j < S; j++) { cn[j] = m.cen[nb[j]]; real[j] = nb[j] != i; }
     double g[4][2], lx[4], ly[4];
-    gradLimit<S, MODE>(qi, qn, m.gm + size_t(i) * 2 * S, 1, m.cen[i], cn, real, a.limit, p.lpThresh, g, lx, ly);
+    gradLimit<S, MODE>(qi, qn, nullptr, 0, m.cen[i], cn, real, a.limit, p.lpThresh, g, lx, ly);
     double* gout = a.grad + size_t(i) * 8;
     *reinterpret_cast<double4*>(gout) = make_double4(g[0][0], g[0][1], g[1][0], g[1][1]);
     *reinterpret_cast<double4*>(gout + 4) = make_double4(g[2][0], g[2][1], g[3][0], g[3][1]);
@@ -434,16 +474,23 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
         double* lo = a.lim + size_t(i) * 8;
         *reinterpret_cast<double4*>(lo) = make_double4(lx[0], ly[0], lx[1], ly[1]);
         *reinterpret_cast<double4*>(lo + 4) = make_double4(lx[2], ly[2], lx[3], ly[3]);
-    } else {
+    } else if (MODE == 2) {
         *reinterpret_cast<double4*>(a.lim + size_t(i) * 4) = make_double4(lx[0], lx[1], lx[2], lx[3]);
     }
+    // MODE 1 (unlimited stage): every limiter is 1; the face kernels take LP = 2 and read none
 }
 
 // ---------------------------------------------------------------- faces
+// LP 0: scalar limiter (4 per cell); 1: LP vector limiter (x,y pairs); 2: an
+// unlimited stage, every limiter 1 (1*g is exactly g, so the products round as with
+// stored ones)
 template <int LP>
 DEV void loadLim(const double* lim, uint32_t c, double lx[4], double ly[4])
 {
-    if (LP) {
+    if (LP == 2) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { lx[k] = 1.0; ly[k] = 1.0; }
+    } else if (LP) {
         const double4 a = *reinterpret_cast<const double4*>(lim + size_t(c) * 8);
         const double4 b = *reinterpret_cast<const double4*>(lim + size_t(c) * 8 + 4);
         lx[0] = a.x; ly[0] = a.y; lx[1] = a.z; ly[1] = a.w; lx[2] = b.x; ly[2] = b.y; lx[3] = b.z; ly[3] = b.w;
@@ -1093,6 +1140,24 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
 // gradients, limiters and ghosts as the gradient kernel left them) into LDS,
 // then the cells' residual and stage update from LDS.  Same device code as
 // k_faces and k_update, so bit-identical to them.
+// Edge length, unit normal and midpoint from the two nodes, in the operation order
+// of the mesh set-up (source/mesh.d; mesh.cpp), so they equal the stored ones bit
+// for bit.  Interior edges: normal = (y1-y0, x0-x1)/|.|, whose norm is the length
+// (the same squares summed in the other order); boundary edges: (1/len) * (...).
+DEV void edgeGeometry(bool interior, double2 p0, double2 p1, double2& n, double2& mid, double& len)
+{
+    const double ddx = p0.x - p1.x, ddy = p0.y - p1.y;
+    len = sqrt(ddx * ddx + ddy * ddy);
+    const double vx = p1.y - p0.y, vy = p0.x - p1.x;
+    if (interior) {
+        n = make_double2(vx / len, vy / len);
+    } else {
+        const double inv = 1.0 / len;
+        n = make_double2(inv * vx, inv * vy);
+    }
+    mid = make_double2(0.5 * (p0.x + p1.x), 0.5 * (p0.y + p1.y));
+}
+
 template <int S, int FLUX, int VISC, int ORDER, int LP>
 __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
 {
@@ -1103,12 +1168,10 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
     const uint32_t nC = d.x, nF = d.y, tid = threadIdx.x;
     for (uint32_t k = tid; k < nF; k += BLK) {
         const uint4 r = t.fref[d.w + k];
-        const double* g5 = t.fgeo + 5 * size_t(d.w + k);
         const uint32_t f = r.x;
-        const double2 n = make_double2(g5[0], g5[1]);
-        const double len = g5[4];
-        double2 mid = make_double2(0, 0);
-        if (ORDER > 1) mid = make_double2(g5[2], g5[3]);
+        double2 n, mid;
+        double len;
+        edgeGeometry(f < m.Fi, m.node[r.w], m.node[t.fn1[d.w + k]], n, mid, len);
         double qL[4], qR[4], F[4], sMax;
         uint64_t errKey;
         if (f < m.Fi) {
@@ -1193,6 +1256,8 @@ static void faceTileO(const DevMesh& m, const PhysParams& p, const StageArgs& a,
     const bool lp = p.order > 1 && p.limited && p.lpThresh > 0;
     const dim3 g(t.nTiles), b(BLK);
     if (p.order == 1) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 1, 0>), g, b, t.ldsBytes, st, m, p, a, t);
+    else if (!(p.limited && a.limit))
+        hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 2>), g, b, t.ldsBytes, st, m, p, a, t);
     else if (lp) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 1>), g, b, t.ldsBytes, st, m, p, a, t);
     else hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 0>), g, b, t.ldsBytes, st, m, p, a, t);
 }
@@ -1541,20 +1606,12 @@ static void gradS(const DevMesh& m, const PhysParams& p, const StageArgs& a, hip
 {
     const uint32_t g = nblk(m.N);
     if (g == 0) return;
-    int mode = p.order == 1 ? 0 : (!(p.limited && a.limit) ? 1 : (p.lpThresh > 0 ? 3 : 2));
-    // lim keeps the LP layout whenever the configuration uses LP
-    if (mode == 1 && p.lpThresh > 0 && p.limited) mode = 13;
+    const int mode = p.order == 1 ? 0 : (!(p.limited && a.limit) ? 1 : (p.lpThresh > 0 ? 3 : 2));
     switch (mode) {
     case 0: hipLaunchKernelGGL((k_gradient<S, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     case 1: hipLaunchKernelGGL((k_gradient<S, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     case 2: hipLaunchKernelGGL((k_gradient<S, 2>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
-    case 3: hipLaunchKernelGGL((k_gradient<S, 3>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
-    default: {
-        // unlimited stage of an LP configuration: write unit x,y limiter pairs
-        StageArgs b = a;
-        b.limit = 0;
-        hipLaunchKernelGGL((k_gradient<S, 3>), dim3(g), dim3(BLK), 0, st, m, p, b);
-    }
+    default: hipLaunchKernelGGL((k_gradient<S, 3>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     }
 }
 
@@ -1581,6 +1638,7 @@ static void facesO(const DevMesh& m, const PhysParams& p, const StageArgs& a, hi
 {
     const bool lp = p.order > 1 && p.limited && p.lpThresh > 0;
     if (p.order == 1) facesT<FLUX, VISC, 1, 0>(m, p, a, st);
+    else if (!(p.limited && a.limit)) facesT<FLUX, VISC, 2, 2>(m, p, a, st);
     else if (lp) facesT<FLUX, VISC, 2, 1>(m, p, a, st);
     else facesT<FLUX, VISC, 2, 0>(m, p, a, st);
 }

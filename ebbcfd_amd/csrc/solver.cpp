@@ -132,7 +132,7 @@ struct ebb_solver {
     DevMesh dm{};
     // device buffers
     DBuf<uint32_t> d_nbr, d_cface, d_refId, d_forceFaces;
-    DBuf<double> d_gm, d_nia, d_cd, d_cper, d_flen, d_grad, d_lim, d_lenF, d_lenS, d_celldt, d_hist, d_chunkTrees;
+    DBuf<double> d_nia, d_cd, d_cper, d_flen, d_grad, d_lim, d_lenF, d_lenS, d_celldt, d_hist, d_chunkTrees;
     DBuf<double> d_partial;
     DBuf<double2> d_cen, d_fn, d_fmid;
     DBuf<uint2> d_fcell;
@@ -145,9 +145,9 @@ struct ebb_solver {
     bool useTiles = false;
     uint32_t tileCells = 0;
     DBuf<uint4> d_ftDesc, d_ftRef;
-    DBuf<uint32_t> d_ftIds;
+    DBuf<uint32_t> d_ftIds, d_ftN1;
     DBuf<uint16_t> d_ftCf;
-    DBuf<double> d_ftGeo;
+    DBuf<double2> d_node;
     FaceTileArgs ftiles{};
     bool useFaceTiles = false;
     // device cell numbering of the tiled paths: device cell d is reference cell
@@ -383,11 +383,11 @@ int buildFaceTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& 
     HIPCHK(trackUpload(s, s->d_ftRef, h.fref));
     HIPCHK(trackUpload(s, s->d_ftIds, h.ids));
     HIPCHK(trackUpload(s, s->d_ftCf, h.cf));
-    HIPCHK(trackUpload(s, s->d_ftGeo, h.fgeo));
+    HIPCHK(trackUpload(s, s->d_ftN1, h.fn1));
     FaceTileArgs ft{};
     ft.nTiles = h.nTiles;
     ft.ldsBytes = lds;
-    ft.desc = s->d_ftDesc.p; ft.ids = s->d_ftIds.p; ft.cf = s->d_ftCf.p; ft.fref = s->d_ftRef.p; ft.fgeo = s->d_ftGeo.p;
+    ft.desc = s->d_ftDesc.p; ft.ids = s->d_ftIds.p; ft.cf = s->d_ftCf.p; ft.fref = s->d_ftRef.p; ft.fn1 = s->d_ftN1.p;
     s->ftiles = ft;
     s->useFaceTiles = true;
     return EBB_OK;
@@ -454,7 +454,7 @@ int buildDevice(ebb_solver* s)
     s->faceOfEdge = lay.faceOfEdge;
     s->edgeOfFace = lay.edgeOfFace;
     const std::vector<uint32_t>&nbr = lay.nbr, &cface = lay.cface, &refId = lay.refId;
-    const std::vector<double>&gm = lay.gm, &nia = lay.nia, &cd = lay.cd, &cper = lay.cper, &flen = lay.flen;
+    const std::vector<double>&nia = lay.nia, &cd = lay.cd, &cper = lay.cper, &flen = lay.flen;
     const std::vector<double2>&cen = lay.cen, &fn = lay.fn, &fmid = lay.fmid;
     const std::vector<uint2>& fcell = lay.fcell;
     // boundary binding (solve.d:222-278)
@@ -548,7 +548,11 @@ int buildDevice(ebb_solver* s)
     HIPCHK(trackUpload(s, s->d_nbr, nbr));
     HIPCHK(trackUpload(s, s->d_cface, cface));
     HIPCHK(trackUpload(s, s->d_refId, refId));
-    HIPCHK(trackUpload(s, s->d_gm, gm));
+    {
+        std::vector<double2> nodes(s->raw.nodes.size() / 2);
+        for (size_t k = 0; k < nodes.size(); k++) nodes[k] = make_double2(s->raw.nodes[2 * k], s->raw.nodes[2 * k + 1]);
+        HIPCHK(trackUpload(s, s->d_node, nodes));
+    }
     HIPCHK(trackUpload(s, s->d_nia, nia));
     HIPCHK(trackUpload(s, s->d_cd, cd));
     HIPCHK(trackUpload(s, s->d_cper, cper));
@@ -593,10 +597,11 @@ int buildDevice(ebb_solver* s)
 
     DevMesh& dm = s->dm;
     dm.N = N; dm.G = s->G; dm.Fi = s->Fi; dm.Fb = s->Fb;
-    dm.nbr = s->d_nbr.p; dm.cface = s->d_cface.p; dm.gm = s->d_gm.p; dm.cen = s->d_cen.p;
+    dm.nbr = s->d_nbr.p; dm.cface = s->d_cface.p; dm.cen = s->d_cen.p;
     dm.nia = s->d_nia.p; dm.cd = s->d_cd.p; dm.cper = s->d_cper.p;
     dm.fcell = s->d_fcell.p; dm.fn = s->d_fn.p; dm.fmid = s->d_fmid.p; dm.flen = s->d_flen.p;
     dm.bface = s->d_bface.p; dm.src = c.mms ? s->d_src.p : nullptr; dm.refId = s->d_refId.p;
+    dm.node = s->d_node.p;
 
     // initial state (solve.d:194-207; app/ms.d:295-298)
     std::vector<double4> q0(C, make_double4(0, 0, 0, 0));
@@ -1116,7 +1121,10 @@ int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
     // the fused stage kernel keeps gradients in LDS: recompute them for the last stage input
     if (s->useTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
     if (grad) { int rc_ = downloadCells(s, s->d_grad.p, grad, 8); if (rc_) return rc_; }
-    if (lim) {
+    if (lim && !(s->phys.limited && s->lastLimit)) {
+        // an unlimited evaluation resets every limiter to 1 (solve.d:324-340) and stores none
+        std::fill(lim, lim + size_t(s->N) * 8, 1.0);
+    } else if (lim) {
         const bool lp = s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0;
         std::vector<double> l(size_t(s->N) * (lp ? 8 : 4));
         { int rc_ = downloadCells(s, s->d_lim.p, l.data(), lp ? 8 : 4); if (rc_) return rc_; }
@@ -1345,14 +1353,14 @@ void ebb_solver_destroy(ebb_solver* s)
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     s->d_nbr.release(); s->d_cface.release(); s->d_refId.release(); s->d_forceFaces.release();
-    s->d_gm.release(); s->d_nia.release(); s->d_cd.release(); s->d_cper.release(); s->d_flen.release();
+    s->d_nia.release(); s->d_cd.release(); s->d_cper.release(); s->d_flen.release();
     s->d_grad.release(); s->d_lim.release(); s->d_lenF.release(); s->d_lenS.release(); s->d_celldt.release();
     s->d_hist.release(); s->d_chunkTrees.release(); s->d_chunkMask.release(); s->d_partial.release();
     s->d_cen.release(); s->d_fn.release(); s->d_fmid.release(); s->d_fcell.release(); s->d_bface.release();
     s->d_src.release(); s->d_qA.release(); s->d_qB.release(); s->d_T.release(); s->d_T2.release();
     s->d_acc.release(); s->d_R.release(); s->d_estate.release();
     s->d_tDesc.release(); s->d_tBlob.release();
-    s->d_ftDesc.release(); s->d_ftRef.release(); s->d_ftIds.release(); s->d_ftCf.release(); s->d_ftGeo.release();
+    s->d_ftDesc.release(); s->d_ftRef.release(); s->d_ftIds.release(); s->d_ftCf.release(); s->d_ftN1.release(); s->d_node.release();
     s->d_devOf.release(); s->d_rmsBlocks.release();
     if (s->d_sc) (void)hipFree(s->d_sc);
     if (s->h_sc) (void)hipHostFree(s->h_sc);

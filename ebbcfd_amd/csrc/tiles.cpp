@@ -52,7 +52,7 @@ void makeLayout(const BuiltMesh& b, Layout& L)
         L.refId[i] = i;
     }
     L.cen.resize(b.nCells);
-    L.fn.resize(F); L.fmid.resize(F); L.fcell.resize(F); L.flen.resize(F);
+    L.fn.resize(F); L.fmid.resize(F); L.fcell.resize(F); L.flen.resize(F); L.fnode.resize(F);
     for (uint32_t k = 0; k < b.nCells; k++) L.cen[k] = make_double2(b.cx[k], b.cy[k]);
     for (uint32_t f = 0; f < F; f++) {
         const uint32_t e = L.edgeOfFace[f];
@@ -60,6 +60,7 @@ void makeLayout(const BuiltMesh& b, Layout& L)
         L.fn[f] = make_double2(b.nx[e], b.ny[e]);
         L.fmid[f] = make_double2(b.mx[e], b.my[e]);
         L.flen[f] = b.len[e];
+        L.fnode[f] = make_uint2(b.eNode0[e], b.eNode1[e]);
     }
 }
 
@@ -87,7 +88,7 @@ void permuteLayout(const Layout& ref, const std::vector<uint32_t>& order, Layout
         L.refId[d] = ref.refId[r];
         L.cen[d] = ref.cen[r];
     }
-    L.fn = ref.fn; L.fmid = ref.fmid; L.flen = ref.flen;
+    L.fn = ref.fn; L.fmid = ref.fmid; L.flen = ref.flen; L.fnode = ref.fnode;
     L.fcell.resize(F);
     for (uint32_t f = 0; f < F; f++) L.fcell[f] = make_uint2(map(ref.fcell[f].x), map(ref.fcell[f].y));
 }
@@ -284,9 +285,8 @@ bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const st
         for (uint32_t k = 0; k < nF; k++) {
             const uint32_t f = fl[k];
             const uint32_t ca = L.fcell[f].x, cb = f < L.Fi ? L.fcell[f].y : N + (f - L.Fi);
-            h.fref.push_back(make_uint4(f, ca, cb, 0));
-            const double g5[5] = {L.fn[f].x, L.fn[f].y, L.fmid[f].x, L.fmid[f].y, L.flen[f]};
-            h.fgeo.insert(h.fgeo.end(), g5, g5 + 5);
+            h.fref.push_back(make_uint4(f, ca, cb, L.fnode[f].x));
+            h.fn1.push_back(L.fnode[f].y);
         }
         h.maxF = std::max(h.maxF, nF);
         h.sumF += nF;

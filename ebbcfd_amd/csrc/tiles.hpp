@@ -20,6 +20,7 @@ struct Layout {
     std::vector<double2> cen, fn, fmid;              // N+G, F, F
     std::vector<uint2> fcell;                        // F
     std::vector<double> flen;                        // F
+    std::vector<uint2> fnode;                        // F: the edge's two mesh nodes, in mesh order
 };
 void makeLayout(const BuiltMesh& b, Layout& L);
 // The same layout with interior cells renumbered: device cell d is cell order[d]
@@ -40,9 +41,8 @@ struct TileHost {
 
 struct FaceTileHost {
     std::vector<uint4> desc, fref;
-    std::vector<uint32_t> ids;
+    std::vector<uint32_t> ids, fn1;
     std::vector<uint16_t> cf;
-    std::vector<double> fgeo;
     uint32_t nTiles = 0, maxF = 0;
     uint64_t sumF = 0;
 };
