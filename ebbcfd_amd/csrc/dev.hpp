@@ -42,6 +42,7 @@ struct DevScalars {
     int32_t pad;
     double rms[4];
     double force[2];
+    double fsum[2];                 // boundary force sums of the current step, before the AoA rotation
 };
 
 struct DevMesh {

@@ -1364,33 +1364,85 @@ DEV void rmsElement(const DiagArgs& d, uint64_t e, double out[8])
     }
 }
 
-constexpr int RBLK = 1024;   // threads per chunk workgroup
+constexpr int RBLK = 1024;   // threads per chunk workgroup (element path)
+constexpr int FTILE = 2048;   // force faces staged per pass
 
+// Pairwise levels of one chunk: v[q][0..cnt) holds its 16-element block sums.
+// Level l's odd leftover is stored as tree l of the chunk (the binary counter's
+// stack), mask marks the levels present.  NT threads.
+template <int NT>
+DEV void chunkLevels(double (*v)[RMS_CHUNK_BLOCKS], uint32_t cnt, double* out, uint32_t* maskOut)
+{
+    uint32_t c = cnt;
+    int level = 0;
+    uint32_t mask = 0;
+    while (c > 0) {
+        if (c & 1) {
+            if (threadIdx.x < 8) out[threadIdx.x * 16 + level] = v[threadIdx.x][c - 1];
+            mask |= 1u << level;
+        }
+        const uint32_t half = c / 2;
+        double tmp[(8 * RMS_CHUNK_BLOCKS / 2 + NT - 1) / NT];
+        int n = 0;
+        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += NT) {
+            const uint32_t q = idx / half, j = idx % half;
+            tmp[n++] = v[q][2 * j] + v[q][2 * j + 1];
+        }
+        __syncthreads();
+        n = 0;
+        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += NT) {
+            const uint32_t q = idx / half, j = idx % half;
+            v[q][j] = tmp[n++];
+        }
+        __syncthreads();
+        c = half;
+        level++;
+    }
+    if (threadIdx.x < 8) maskOut[threadIdx.x] = mask;
+}
+
+// The boundary force (mesh.d:645-671): len*flux of the force faces summed in the
+// reference's edge order by one lane, from LDS-staged values; NT threads stage.
+template <int NT>
+DEV void forceSum(const DiagArgs& d, double* fb)
+{
+    double f0 = 0, f1 = 0;
+    for (uint32_t k0 = 0; k0 < d.nForce; k0 += FTILE) {
+        const uint32_t cnt = (d.nForce - k0) < uint32_t(FTILE) ? (d.nForce - k0) : uint32_t(FTILE);
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < cnt; x += NT) {
+            const uint32_t f = d.forceFaces[k0 + x];
+            fb[x] = d.lenF[size_t(f) * 4 + 1];
+            fb[FTILE + x] = d.lenF[size_t(f) * 4 + 2];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll 8
+            for (uint32_t x = 0; x < cnt; x++) {
+                f0 += fb[x];
+                f1 += fb[FTILE + x];
+            }
+        }
+    }
+    if (threadIdx.x == 0) { d.sc->fsum[0] = f0; d.sc->fsum[1] = f1; }
+}
+
+// RMS sums from the states (element path): one workgroup per chunk of
+// RMS_CHUNK_BLOCKS 16-element blocks; 16 consecutive lanes own one block, and
+// the xor butterfly over them adds exactly like the fixed tree
+// (((r0+r1)+(r2+r3))+...).  The last workgroup sums the boundary force.
 __global__ __launch_bounds__(RBLK) void k_rms_chunks(DiagArgs d)
 {
     if (!d.sc->active) return;
     __shared__ double v[8][RMS_CHUNK_BLOCKS];
+    if (blockIdx.x == gridDim.x - 1) { forceSum<RBLK>(d, &v[0][0]); return; }
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint64_t c0 = uint64_t(blockIdx.x) * RMS_CHUNK_BLOCKS;
     const uint32_t cnt = uint32_t((nb - c0) < RMS_CHUNK_BLOCKS ? (nb - c0) : RMS_CHUNK_BLOCKS);
-    // 16 consecutive lanes own one 16-element block: the xor butterfly over
-    // 16 lanes adds exactly like the fixed tree (((r0+r1)+(r2+r3))+...)
     constexpr uint32_t BPI = RBLK / 16;
     for (uint32_t it = 0; it < RMS_CHUNK_BLOCKS / BPI; it++) {
         const uint32_t lb = it * BPI + threadIdx.x / 16;
         double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (d.blockSums) {
-            // leaf sums from the stage update; blocks wholly past the interior cells are zero
-            const uint64_t b = c0 + lb;
-            if ((threadIdx.x % 16) == 0 && lb < cnt) {
-                if (b * 16 < d.N)
-#pragma unroll
-                    for (int q = 0; q < 8; q++) o[q] = d.blockSums[b * 8 + q];
-#pragma unroll
-                for (int q = 0; q < 8; q++) v[q][lb] = o[q];
-            }
-            continue;
-        }
         if (lb < cnt) rmsElement(d, (c0 + lb) * 16 + (threadIdx.x % 16), o);
 #pragma unroll
         for (int m = 1; m < 16; m <<= 1)
@@ -1401,42 +1453,71 @@ __global__ __launch_bounds__(RBLK) void k_rms_chunks(DiagArgs d)
             for (int q = 0; q < 8; q++) v[q][lb] = o[q];
     }
     __syncthreads();
-    double* out = d.chunkTrees + size_t(blockIdx.x) * 8 * 16;
-    uint32_t c = cnt;
-    int level = 0;
-    uint32_t mask = 0;
-    while (c > 0) {
-        if (c & 1) {
-            if (threadIdx.x < 8) out[threadIdx.x * 16 + level] = v[threadIdx.x][c - 1];
-            mask |= 1u << level;
+    chunkLevels<RBLK>(v, cnt, d.chunkTrees + size_t(blockIdx.x) * 8 * 16, d.chunkMask + blockIdx.x * 8);
+}
+
+// RMS sums from the leaf sums the last stage stored (blockSums, 8 per block;
+// blocks wholly past the interior cells are zero).  Workgroups [0, nfull): one
+// full chunk each, only its top tree is needed: each lane adds 4 consecutive
+// blocks pairwise, the 64-lane xor butterfly adds levels 3..8 exactly as the
+// pairwise tree does (the two orders of each sum are equal), the 4 waves levels
+// 9 and 10.  Workgroup nfull: the partial last chunk; nfull+1: the boundary force.
+constexpr int RFB = 256;
+__global__ __launch_bounds__(RFB) void k_rms_blocks(DiagArgs d, uint32_t nfull)
+{
+    if (!d.sc->active) return;
+    __shared__ double v[8][RMS_CHUNK_BLOCKS];
+    const uint64_t nb = uint64_t(d.L) / 16;
+    if (blockIdx.x == nfull + 1) { forceSum<RFB>(d, &v[0][0]); return; }
+    const uint64_t c0 = uint64_t(blockIdx.x) * RMS_CHUNK_BLOCKS;
+    auto leaf = [&](uint64_t b, double o[8]) {
+        if (b * 16 < d.N) {
+            const double4 x = *reinterpret_cast<const double4*>(d.blockSums + b * 8);
+            const double4 y = *reinterpret_cast<const double4*>(d.blockSums + b * 8 + 4);
+            o[0] = x.x; o[1] = x.y; o[2] = x.z; o[3] = x.w; o[4] = y.x; o[5] = y.y; o[6] = y.z; o[7] = y.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) o[q] = 0.0;
         }
-        const uint32_t half = c / 2;
-        double tmp[(8 * RMS_CHUNK_BLOCKS / 2 + RBLK - 1) / RBLK];
-        int n = 0;
-        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += RBLK) {
-            const uint32_t q = idx / half, j = idx % half;
-            tmp[n++] = v[q][2 * j] + v[q][2 * j + 1];
+    };
+    if (blockIdx.x == nfull) {   // partial chunk
+        const uint32_t cnt = uint32_t(nb - c0);
+        for (uint32_t lb = threadIdx.x; lb < cnt; lb += RFB) {
+            double o[8];
+            leaf(c0 + lb, o);
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q][lb] = o[q];
         }
         __syncthreads();
-        n = 0;
-        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += RBLK) {
-            const uint32_t q = idx / half, j = idx % half;
-            v[q][j] = tmp[n++];
-        }
-        __syncthreads();
-        c = half;
-        level++;
+        chunkLevels<RFB>(v, cnt, d.chunkTrees + size_t(nfull) * 8 * 16, d.chunkMask + nfull * 8);
+        return;
     }
-    if (threadIdx.x < 8) d.chunkMask[blockIdx.x * 8 + threadIdx.x] = mask;
+    double b[4][8], t[8];
+#pragma unroll
+    for (int u = 0; u < 4; u++) leaf(c0 + 4 * threadIdx.x + u, b[u]);
+#pragma unroll
+    for (int q = 0; q < 8; q++) t[q] = (b[0][q] + b[1][q]) + (b[2][q] + b[3][q]);
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1)
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = t[q] + __shfl_xor(t[q], m, 64);
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q][w] = t[q];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int q = threadIdx.x;
+        d.chunkTrees[size_t(blockIdx.x) * 128 + q * 16 + 10] = (v[q][0] + v[q][1]) + (v[q][2] + v[q][3]);
+    }
 }
 
 // Step epilogue (app/solver.d:122-247), one workgroup.  The chunk trees are
 // merged level by level in LDS (a perfect tree over each aligned power-of-two
-// run of chunks is exactly the binary counter's merge), the boundary force is
-// summed in the reference's edge order by one lane from LDS-staged values.
+// run of chunks is exactly the binary counter's merge); the boundary force comes
+// summed from the RMS kernel.
 constexpr int FIN_T = 256;
 constexpr int FIN_CHUNKS = 1024;     // chunk trees merged in LDS per quantity
-constexpr int FIN_FTILE = 2048;      // force faces staged per tile
 
 __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
 {
@@ -1529,28 +1610,9 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
         }
         tot[q] = 0.0 + s;
     }
-    // boundary force, in bGroups order (mesh.d:645-671)
-    double f0 = 0, f1 = 0;
-    double* fb = &v[0][0];
-    for (uint32_t k0 = 0; k0 < d.nForce; k0 += FIN_FTILE) {
-        const uint32_t cnt = (d.nForce - k0) < uint32_t(FIN_FTILE) ? (d.nForce - k0) : uint32_t(FIN_FTILE);
-        __syncthreads();
-        for (uint32_t x = tid; x < cnt; x += FIN_T) {
-            const uint32_t f = d.forceFaces[k0 + x];
-            fb[x] = d.lenF[size_t(f) * 4 + 1];
-            fb[FIN_FTILE + x] = d.lenF[size_t(f) * 4 + 2];
-        }
-        __syncthreads();
-        if (tid == 0) {
-#pragma unroll 8
-            for (uint32_t x = 0; x < cnt; x++) {
-                f0 += fb[x];
-                f1 += fb[FIN_FTILE + x];
-            }
-        }
-    }
     __syncthreads();
     if (tid != 0) return;
+    const double f0 = sc->fsum[0], f1 = sc->fsum[1];   // summed by the RMS kernel's force workgroup
     double rms[4];
     for (int q = 0; q < 4; q++) rms[q] = sqrt(double(d.N) * tot[q]) / tot[4 + q];
     const double newDt = __longlong_as_double((long long)sc->newDtBits);
@@ -1678,7 +1740,13 @@ void launchRmsChunks(const DiagArgs& d, hipStream_t st)
 {
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint32_t nc = uint32_t((nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS);
-    if (nc) hipLaunchKernelGGL(k_rms_chunks, dim3(nc), dim3(RBLK), 0, st, d);
+    if (d.blockSums) {
+        // full chunks, the partial chunk (its workgroup returns at once when there is none), the force
+        const uint32_t nfull = uint32_t(nb / RMS_CHUNK_BLOCKS);
+        hipLaunchKernelGGL(k_rms_blocks, dim3(nfull + 2), dim3(RFB), 0, st, d, nfull);
+    } else {
+        hipLaunchKernelGGL(k_rms_chunks, dim3(nc + 1), dim3(RBLK), 0, st, d);
+    }
 }
 
 void launchFinalize(const DiagArgs& d, hipStream_t st)
