@@ -445,37 +445,57 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
     const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
     if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
     const uint32_t i = bid * BLK + threadIdx.x;
-    if (i >= m.N) return;
-    double qi[4];
-    ld4(a.qin, i, qi);
-    double qn[S][4];
-    uint32_t nb[S];
+    if (MODE == 0 && i >= m.N) return;
+    const bool have = i < m.N;   // no early return below: the staged stores need every thread
+    double g[4][2], lx[4], ly[4];
+    if (have) {
+        double qi[4];
+        ld4(a.qin, i, qi);
+        double qn[S][4];
+        uint32_t nb[S];
 #pragma unroll
-    for (int j = 0; j < S; j++) {
-        nb[j] = m.nbr[i * S + j];
-        if (nb[j] >= m.N) {
-            ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qn[j]);
-            st4(a.qin, nb[j], qn[j]);
-        } else {
-            ld4(a.qin, nb[j], qn[j]);
+        for (int j = 0; j < S; j++) {
+            nb[j] = m.nbr[i * S + j];
+            if (nb[j] >= m.N) {
+                ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qn[j]);
+                st4(a.qin, nb[j], qn[j]);
+            } else {
+                ld4(a.qin, nb[j], qn[j]);
+            }
         }
+        if (MODE == 0) return;
+        double2 cn[S];
+        bool real[S];
+#pragma unroll
+        for (int j = 0; j < S; j++) { cn[j] = m.cen[nb[j]]; real[j] = nb[j] != i; }
+        gradLimit<S, MODE>(qi, qn, nullptr, 0, m.cen[i], cn, real, a.limit, p.lpThresh, g, lx, ly);
     }
     if (MODE == 0) return;
-    double2 cn[S];
-    bool real[S];
+    // stage the rows in LDS and store the workgroup's contiguous run of cells with
+    // lane-consecutive 16-byte stores
+    constexpr int LWS = MODE == 3 ? 8 : (MODE == 2 ? 4 : 0);
+    __shared__ double sg[BLK][9];
+    __shared__ double sl[BLK][LWS + 1];
+    const uint32_t tid = threadIdx.x;
+    if (have) {
 #pragma unroll
-    for (int j = 0; j < S; j++) { cn[j] = m.cen[nb[j]]; real[j] = nb[j] != i; }
-    double g[4][2], lx[4], ly[4];
-    gradLimit<S, MODE>(qi, qn, nullptr, 0, m.cen[i], cn, real, a.limit, p.lpThresh, g, lx, ly);
-    double* gout = a.grad + size_t(i) * 8;
-    *reinterpret_cast<double4*>(gout) = make_double4(g[0][0], g[0][1], g[1][0], g[1][1]);
-    *reinterpret_cast<double4*>(gout + 4) = make_double4(g[2][0], g[2][1], g[3][0], g[3][1]);
-    if (MODE == 3) {
-        double* lo = a.lim + size_t(i) * 8;
-        *reinterpret_cast<double4*>(lo) = make_double4(lx[0], ly[0], lx[1], ly[1]);
-        *reinterpret_cast<double4*>(lo + 4) = make_double4(lx[2], ly[2], lx[3], ly[3]);
-    } else if (MODE == 2) {
-        *reinterpret_cast<double4*>(a.lim + size_t(i) * 4) = make_double4(lx[0], lx[1], lx[2], lx[3]);
+        for (int c = 0; c < 4; c++) { sg[tid][2 * c] = g[c][0]; sg[tid][2 * c + 1] = g[c][1]; }
+        if (MODE == 3) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) { sl[tid][2 * c] = lx[c]; sl[tid][2 * c + 1] = ly[c]; }
+        } else if (MODE == 2) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) sl[tid][c] = lx[c];
+        }
+    }
+    __syncthreads();
+    const uint32_t base = bid * BLK, n = (m.N - base) < BLK ? (m.N - base) : BLK;
+    double2* gd = reinterpret_cast<double2*>(a.grad + size_t(base) * 8);
+    for (uint32_t c = tid; c < 4 * n; c += BLK) gd[c] = make_double2(sg[c / 4][2 * (c % 4)], sg[c / 4][2 * (c % 4) + 1]);
+    if (LWS) {
+        double2* ld = reinterpret_cast<double2*>(a.lim + size_t(base) * LWS);
+        constexpr uint32_t PR = LWS / 2;
+        for (uint32_t c = tid; c < PR * n; c += BLK) ld[c] = make_double2(sl[c / PR][2 * (c % PR)], sl[c / PR][2 * (c % PR) + 1]);
     }
     // MODE 1 (unlimited stage): every limiter is 1; the face kernels take LP = 2 and read none
 }

@@ -327,7 +327,7 @@ int clearErrors(ebb_solver* s)
 // Pick the tile size (tileCells, or automatically the largest multiple of 16 up
 // to 128 whose two blob buffers and working set fit 64 KB of LDS) and upload
 // the tiles.
-// Tiled paths: choose the tile size (tileCells, or automatically: 512 cells per
+// Tiled paths: choose the tile size (tileCells, or automatically: 768 cells per
 // face tile; for the fused kernel the largest multiple of 16 up to 128 whose two
 // blob buffers and working set fit 64 KB of LDS) and cut the tiles on the
 // reference numbering; with renumbering the interior cells are then numbered tile
@@ -344,7 +344,7 @@ int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& st
     const long long want = c.tileCells;
     if (want != -1 && (want < 1 || want > 1024)) return fail(s, EBB_ERR_ARG, "tileCells must be -1 or in 1..1024");
     std::vector<uint32_t> order;
-    uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 ? 512 : 128);
+    uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 ? 768 : 128);
     if (c.stagePath == 2 && want <= 0) {
         const int lp = (c.order > 1 && c.limited && c.lpThresh > 0) ? 1 : 0;
         for (;; T -= 16) {
