@@ -26,15 +26,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes (FP64, triangles): per interior cell-stage for the whole
-# residual+update (the fused tile kernel's unit), and per launch unit for the
-# split path's face kernel
+# Algorithmic bytes per interior cell-stage (FP64, triangles): the compulsory
+# traffic of one residual + stage update, each datum once, intermediates such as
+# gradients not counted -- state in 32 + q^n 32 + new state 32, neighbour and face
+# ids 24, least-squares weights 48, centroid/area/d/perimeter 40, 1.5 faces of
+# geometry 60.  The tiled kernels finish a whole stage for their cells, so one
+# launch processes N cell-stages.
 STAGE_BYTES_PER_CELL = 268.0
+# split path: its face kernel's own unit
 FACE_BYTES_PER_FACE = 88.0      # owner/other ids 8 + normal 16 + midpoint 16 + length 8 + len*flux,len*sMax 40
 FACE_BYTES_PER_CELL = 144.0     # state 32 + gradient 64 + limiter 32 + centroid 16, each cell read once
-# face/update tile kernel: faces without the len*flux round trip, plus the cell update
-TILE_BYTES_PER_FACE = 48.0      # cell pair 8 + normal 16 + midpoint 16 + length 8
-TILE_BYTES_PER_CELL = 244.0     # cell data 144 + face refs 12 + perimeter/d/-1/area 24 + q^n 32 + new state 32
+# HBM bytes per launch of the face/update tile kernel on this workload, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain 1 GiB copy (x2.0 and
+# x1.0), averaged over the limited and unlimited stage; the summary and the
+# commands are in profiles/hbm_traffic_cylinder_ns_4M.md.  Re-measured whenever
+# the kernel changes; reported only for the default configuration it was
+# measured on.
+MEASURED_TRAFFIC = {("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference"): 1.407e9}
 
 
 def log(msg):
@@ -137,7 +145,7 @@ def main():
         % (tk["fused_ms"], tk["gradient_ms"], tk["faces_ms"], tk["update_ms"], tk["step_ms"]))
     if tk["fused_ms"] > 0 and args.stage_path == "faceTiles":
         kernel, launch_ms = "k_face_tile", tk["fused_ms"]
-        launch_bytes = F * TILE_BYTES_PER_FACE + N * TILE_BYTES_PER_CELL
+        launch_bytes = N * STAGE_BYTES_PER_CELL
     elif tk["fused_ms"] > 0:
         kernel, launch_ms = "k_stage_tile", tk["fused_ms"]
         launch_bytes = N * STAGE_BYTES_PER_CELL
@@ -145,6 +153,9 @@ def main():
         kernel, launch_ms = "k_faces", tk["faces_ms"]
         launch_bytes = F * FACE_BYTES_PER_FACE + N * FACE_BYTES_PER_CELL
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    workload = "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r)
+    traffic = MEASURED_TRAFFIC.get((kernel, workload, args.stage_path, args.tile_cells, args.tile_order,
+                                    args.cell_order))
 
     cpu = None
     if args.cpu_steps > 0:
@@ -168,13 +179,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r), "cells": N,
+        "config": {"workload": workload, "cells": N,
                    "stage_path": args.stage_path, "tile_cells": args.tile_cells, "tile_order": args.tile_order,
                    "cell_order": args.cell_order,
                    "edges": F, "flux": "rusanovFlux", "limiter": "scalar (solve.d:360-403)",
                    "integrator": "RK2_TVD", "stages_per_step": stages, "M": 0.2, "Re": 100, "CFL": cfg["CFL"]},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else int(traffic),
                      "avg_launch_ms": round(launch_ms, 4), "algorithmic_bytes_per_launch": int(launch_bytes)},
         "stage_roofline": {"bytes_per_cell_stage": STAGE_BYTES_PER_CELL, "stage_ms": round(stage_ms, 4),
                            "achieved": round(stage_gbs, 1), "frac": round(stage_gbs / HBM_PEAK_GBS, 4),
