@@ -84,6 +84,63 @@ def copy_calibration():
     log("copied %d bytes 5 times" % (x.numel() * 8))
 
 
+def host_threads():
+    """CPUs the mesh build may use: affinity set, capped by OMP_NUM_THREADS and 32."""
+    t = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and 0 < int(omp) < t:
+        t = int(omp)
+    return min(t, 32)
+
+
+def ingest_bench(args):
+    """Mesh ingest: gmsh 2.2 read + geometry precompute (importMesh + buildMesh,
+    source/io.d:689-990, source/mesh.d:217-643) of the benchmark's O-grid through
+    ebb_mesh_load, timed from the file on local disk (page cache warm after the write)."""
+    import tempfile
+    from ebbcfd_amd import import_mesh, meshgen
+    log("generating O-grid %d x %d" % (args.n_theta, args.n_r))
+    raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cylinder.msh")
+        meshgen.write_gmsh(raw, path)
+        size = os.path.getsize(path)
+        log("wrote %s (%.1f MB)" % (path, size / 1e6))
+        del raw
+        runs = []
+        for k in range(args.ingest_repeats):
+            t0 = time.perf_counter()
+            m = import_mesh(path)
+            runs.append(time.perf_counter() - t0)
+            N, F = m.n_interior, m.n_edges
+            del m
+            log("ingest %d: %.3f s" % (k, runs[-1]))
+    el = float(np.median(runs))
+    cpu = None
+    if args.cpu_steps > 0:
+        import oracle
+        from oracle.gmsh import parse_gmsh
+        nt, nr = args.n_theta // 2, args.n_r // 2
+        sample = meshgen.cylinder_ogrid(nt, nr, seed=1)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "sample.msh")
+            meshgen.write_gmsh(sample, path)
+            t0 = time.perf_counter()
+            om = oracle.Mesh(parse_gmsh(path))
+            el_cpu = time.perf_counter() - t0
+        cpu = {"value": round(om.n_interior / el_cpu / 1e6, 4), "unit": "Mcell/s", "cores": 1, "kind": "port",
+               "sample": "oracle gmsh reader + geometry build (numpy, 1 thread) on the %dx%dx2 O-grid "
+                         "(%d cells), %.1f s" % (nt, nr, om.n_interior, el_cpu)}
+        log("CPU baseline %.3f Mcell/s (%.1f s)" % (cpu["value"], el_cpu))
+    out = {"metric": "Mcells ingested/s (gmsh read + geometry precompute)", "value": round(N / el / 1e6, 3),
+           "unit": "Mcell/s", "higher_is_better": True, "ms_per_ingest": round(el * 1e3, 1),
+           "runs_s": [round(r, 3) for r in runs], "data": "synthetic",
+           "config": {"workload": "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r), "cells": N, "edges": F,
+                      "file_bytes": size, "host_threads": host_threads()},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +159,13 @@ def main():
                     help="device numbering of the cells: the mesh's, or tile by tile")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters) and exit")
+    ap.add_argument("--ingest", action="store_true",
+                    help="time the mesh ingest (gmsh read + geometry precompute) of the workload's mesh instead")
+    ap.add_argument("--ingest-repeats", type=int, default=3)
     args = ap.parse_args()
+    if args.ingest:
+        ingest_bench(args)
+        return
     if args.copy_calibration:
         copy_calibration()
         return

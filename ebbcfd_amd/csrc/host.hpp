@@ -1,7 +1,9 @@
 // Host-side data model: raw mesh, built geometry (reference numbering), config.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace ebb {
@@ -23,21 +25,34 @@ struct RawMesh {
     std::vector<uint64_t> bgroupStart;  // first bnode of each tag
 };
 
+// Allocator whose resize() leaves new elements uninitialised, so that large arrays
+// every element of which is written anyway are first touched by the threads that
+// fill them instead of being zero-filled by one.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <typename U> NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <typename U> void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+    template <typename U, typename... A> void construct(U* p, A&&... a) { ::new (static_cast<void*>(p)) U(std::forward<A>(a)...); }
+};
+template <typename T> using FillVec = std::vector<T, NoInitAlloc<T>>;
+
 // buildMesh output (source/mesh.d:217-643), reference cell/edge numbering.
 struct BuiltMesh {
     uint32_t nElem = 0, nCells = 0, nEdges = 0;
     // cells (ghosts have index >= nElem, one per boundary edge in boundaryEdges order)
-    std::vector<uint8_t> nEdgesCell;     // nElem
-    std::vector<uint32_t> cellEdges;     // MAXE per interior cell
-    std::vector<int8_t> fluxMult;        // MAXE per interior cell (+1 owner, -1 otherwise)
-    std::vector<uint32_t> nbr;           // MAXE per interior cell
-    std::vector<double> gradMat;         // 2*MAXE per interior cell, row-major (dir, nbr)
-    std::vector<double> area, d, perim;  // nCells (ghosts zero)
-    std::vector<double> cx, cy;          // nCells (ghost centroids reflected)
+    FillVec<uint8_t> nEdgesCell;         // nElem
+    FillVec<uint32_t> cellEdges;         // MAXE per interior cell (0 past the cell's edges)
+    FillVec<int8_t> fluxMult;            // MAXE per interior cell (+1 owner, -1 otherwise)
+    FillVec<uint32_t> nbr;               // MAXE per interior cell (0 past the cell's edges)
+    FillVec<double> gradMat;             // 2*MAXE per interior cell, row-major (dir, nbr)
+    FillVec<double> area, d, perim;      // nCells (ghosts zero)
+    FillVec<double> cx, cy;              // nCells (ghost centroids reflected)
     // edges
-    std::vector<uint32_t> eNode0, eNode1, eCell0, eCell1;
-    std::vector<double> len, nx, ny, mx, my;
-    std::vector<int32_t> eBGroup;        // -1 interior
+    FillVec<uint32_t> eNode0, eNode1, eCell0, eCell1;
+    FillVec<double> len, nx, ny, mx, my;
+    FillVec<int32_t> eBGroup;            // -1 interior
     std::vector<uint32_t> interiorEdges, boundaryEdges;
     std::vector<std::vector<uint32_t>> bGroups;
 };

@@ -1427,21 +1427,26 @@ template <int NT>
 DEV void forceSum(const DiagArgs& d, double* fb)
 {
     double f0 = 0, f1 = 0;
+    double2* fp = reinterpret_cast<double2*>(fb);   // (x, y) pairs: one LDS read per face
     for (uint32_t k0 = 0; k0 < d.nForce; k0 += FTILE) {
         const uint32_t cnt = (d.nForce - k0) < uint32_t(FTILE) ? (d.nForce - k0) : uint32_t(FTILE);
         __syncthreads();
         for (uint32_t x = threadIdx.x; x < cnt; x += NT) {
             const uint32_t f = d.forceFaces[k0 + x];
-            fb[x] = d.lenF[size_t(f) * 4 + 1];
-            fb[FTILE + x] = d.lenF[size_t(f) * 4 + 2];
+            fp[x] = make_double2(d.lenF[size_t(f) * 4 + 1], d.lenF[size_t(f) * 4 + 2]);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-#pragma unroll 8
-            for (uint32_t x = 0; x < cnt; x++) {
-                f0 += fb[x];
-                f1 += fb[FTILE + x];
+            // the two ordered chains interleave; 32 LDS reads are in flight ahead of the adds
+            uint32_t x = 0;
+            for (; x + 32 <= cnt; x += 32) {
+                double2 v[32];
+#pragma unroll
+                for (int u = 0; u < 32; u++) v[u] = fp[x + u];
+#pragma unroll
+                for (int u = 0; u < 32; u++) { f0 += v[u].x; f1 += v[u].y; }
             }
+            for (; x < cnt; x++) { const double2 v = fp[x]; f0 += v.x; f1 += v.y; }
         }
     }
     if (threadIdx.x == 0) { d.sc->fsum[0] = f0; d.sc->fsum[1] = f1; }

@@ -9,12 +9,18 @@
 // edge line, neighbour list in edge order and gradMat = (A^T A)^-1 A^T.
 // Every floating-point expression keeps the reference's operation order so the
 // geometry is bit-identical to the CPU restatement.
+#include <sched.h>
+
+#include <charconv>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <sstream>
 #include <algorithm>
+#include <atomic>
+#include <thread>
 
 #include "host.hpp"
 
@@ -54,9 +60,14 @@ struct EdgeMap {
         uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
         return (uint64_t(lo) << 32) | hi;
     }
+    // full-avalanche 64-bit mixer (splitmix64 finaliser): the table index uses the low
+    // bits, which after a single multiply would depend on the low key bits only and
+    // cluster structured node numberings into long probe runs
     static uint64_t mix(uint64_t k)
     {
-        k ^= k >> 31; k *= 0x9e3779b97f4a7c15ull; k ^= k >> 29;
+        k ^= k >> 30; k *= 0xbf58476d1ce4e5b9ull;
+        k ^= k >> 27; k *= 0x94d049bb133111ebull;
+        k ^= k >> 31;
         return k;
     }
     uint32_t get(uint32_t a, uint32_t b) const
@@ -79,86 +90,145 @@ struct EdgeMap {
 
 }  // namespace
 
+// Line reader over a file held in memory: tokens split on ' ', '\t', '\r' (as
+// cleanLine does) without allocating per line.
+struct LineTokens {
+    const char* p = nullptr;
+    const char* end = nullptr;
+    const char* lineBeg = nullptr;
+    const char* lineEnd = nullptr;
+    static constexpr int MAXT = 64;
+    const char* tb[MAXT];
+    const char* te[MAXT];
+    int n = 0;
+    bool next()
+    {
+        if (p >= end) return false;
+        lineBeg = p;
+        const char* e = static_cast<const char*>(std::memchr(p, '\n', size_t(end - p)));
+        if (!e) e = end;
+        lineEnd = e;
+        p = e < end ? e + 1 : end;
+        n = 0;
+        const char* q = lineBeg;
+        while (q < lineEnd) {
+            while (q < lineEnd && (*q == ' ' || *q == '\t' || *q == '\r')) q++;
+            if (q >= lineEnd) break;
+            const char* t0 = q;
+            while (q < lineEnd && !(*q == ' ' || *q == '\t' || *q == '\r')) q++;
+            if (n < MAXT) { tb[n] = t0; te[n] = q; }
+            n++;
+        }
+        return true;
+    }
+    std::string str(int k) const { return std::string(tb[k], te[k]); }
+    std::string line() const { return std::string(lineBeg, lineEnd); }
+    bool startsEnd() const { return n > 0 && str(0).find("$End") != std::string::npos; }
+    // integer prefix of token k, like std::stol (optional sign, digits)
+    bool num(int k, long& v) const
+    {
+        const char* a = tb[k];
+        if (a < te[k] && *a == '+') a++;
+        const auto r = std::from_chars(a, te[k], v);
+        return r.ec == std::errc();
+    }
+};
+
+bool readFile(const std::string& path, std::string& buf)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.seekg(0, std::ios::end);
+    const std::streamoff n = f.tellg();
+    if (n < 0) return false;
+    f.seekg(0, std::ios::beg);
+    buf.resize(size_t(n));
+    f.read(&buf[0], n);
+    return bool(f) || f.eof();
+}
+
 bool parseGmsh(const std::string& path, RawMesh& out, std::string& err)
 {
-    std::ifstream f(path);
-    if (!f) { err = "cannot open mesh file " + path; return false; }
-    std::string line;
-    auto next = [&](std::vector<std::string>& tok) -> bool {
-        if (!std::getline(f, line)) return false;
-        tok = cleanLine(line);
-        return true;
-    };
-    std::vector<std::string> tok;
+    std::string buf;
+    if (!readFile(path, buf)) { err = "cannot open mesh file " + path; return false; }
+    LineTokens L;
+    L.p = buf.data();
+    L.end = buf.data() + buf.size();
     std::vector<std::vector<uint32_t>> bnodes;         // per line element
     std::vector<std::vector<int>> bgroups;             // physical tag per line element, grouped by runs
-    std::vector<std::vector<uint32_t>> elements;
+    std::vector<uint32_t> elemNodes, elemOff(1, 0);    // area elements, 1-based nodes
     out = RawMesh();
+    long v[LineTokens::MAXT];
     while (true) {
-        bool ok = next(tok);
-        while (ok && tok.empty()) ok = next(tok);
+        bool ok = L.next();
+        while (ok && L.n == 0) ok = L.next();
         if (!ok) break;
-        if (tok.size() != 1 || tok[0][0] != '$') { err = "Incorrectly formatted section header: " + line; return false; }
-        std::string sec = tok[0].substr(1);
+        if (L.n != 1 || *L.tb[0] != '$') { err = "Incorrectly formatted section header: " + L.line(); return false; }
+        const std::string sec = L.str(0).substr(1);
         if (sec == "MeshFormat") {
-            if (!next(tok) || tok.size() < 2) { err = "bad $MeshFormat"; return false; }
-            if (std::stoi(tok[1]) != 0) { err = "Unsuported mesh format. Expected 0, got " + tok[1]; return false; }
-            if (!next(tok) || tok.empty() || tok[0].find("$End") == std::string::npos) { err = "Expected section end"; return false; }
+            long fmt;
+            if (!L.next() || L.n < 2 || !L.num(1, fmt)) { err = "bad $MeshFormat"; return false; }
+            if (fmt != 0) { err = "Unsuported mesh format. Expected 0, got " + L.str(1); return false; }
+            if (!L.next() || !L.startsEnd()) { err = "Expected section end"; return false; }
         } else if (sec == "PhysicalNames") {
-            if (!next(tok) || tok.size() != 1) { err = "bad $PhysicalNames count"; return false; }
-            int n = std::stoi(tok[0]);
-            for (int i = 0; i < n; i++) {
-                if (!next(tok) || tok.size() != 3) { err = "Expected 3 entries in $PhysicalNames"; return false; }
-                if (std::stoi(tok[0]) == 1) {
-                    std::string t = tok[2];
+            long n;
+            if (!L.next() || L.n != 1 || !L.num(0, n)) { err = "bad $PhysicalNames count"; return false; }
+            for (long i = 0; i < n; i++) {
+                long dim;
+                if (!L.next() || L.n != 3 || !L.num(0, dim)) { err = "Expected 3 entries in $PhysicalNames"; return false; }
+                if (dim == 1) {
+                    std::string t = L.str(2);
                     t.erase(std::remove(t.begin(), t.end(), '"'), t.end());
                     out.btags.push_back(t);
                 }
             }
-            if (!next(tok) || tok.empty() || tok[0].find("$End") == std::string::npos) { err = "Expected section end"; return false; }
+            if (!L.next() || !L.startsEnd()) { err = "Expected section end"; return false; }
         } else if (sec == "Nodes") {
-            if (!next(tok) || tok.size() != 1) { err = "bad $Nodes count"; return false; }
-            long n = std::stol(tok[0]);
-            out.nodes.reserve(2 * n);
+            long n;
+            if (!L.next() || L.n != 1 || !L.num(0, n) || n < 0) { err = "bad $Nodes count"; return false; }
+            out.nodes.reserve(2 * size_t(n));
             for (long i = 0; i < n; i++) {
-                if (!next(tok) || tok.size() < 3) { err = "Unexpected end of file"; return false; }
-                if (std::stol(tok[0]) != i + 1) { err = "Unexpected node number"; return false; }
-                out.nodes.push_back(std::strtod(tok[1].c_str(), nullptr));
-                out.nodes.push_back(std::strtod(tok[2].c_str(), nullptr));
+                long id;
+                if (!L.next() || L.n < 3) { err = "Unexpected end of file"; return false; }
+                if (!L.num(0, id) || id != i + 1) { err = "Unexpected node number"; return false; }
+                // strtod on the buffer: it stops at the whitespace after the number
+                out.nodes.push_back(std::strtod(L.tb[1], nullptr));
+                out.nodes.push_back(std::strtod(L.tb[2], nullptr));
             }
-            if (!next(tok) || tok.empty() || tok[0].find("$End") == std::string::npos) { err = "Expected section end"; return false; }
+            if (!L.next() || !L.startsEnd()) { err = "Expected section end"; return false; }
         } else if (sec == "Elements") {
-            if (!next(tok) || tok.size() != 1) { err = "bad $Elements count"; return false; }
-            long n = std::stol(tok[0]);
+            long n;
+            if (!L.next() || L.n != 1 || !L.num(0, n) || n < 0) { err = "bad $Elements count"; return false; }
             int current = -1;
             for (long i = 0; i < n; i++) {
-                if (!next(tok) || tok.size() <= 4) { err = "Element line must have at least 4 entries"; return false; }
-                std::vector<long> v(tok.size());
-                for (size_t k = 0; k < tok.size(); k++) v[k] = std::stol(tok[k]);
+                if (!L.next() || L.n <= 4) { err = "Element line must have at least 4 entries"; return false; }
+                if (L.n > LineTokens::MAXT) { err = "element line has too many entries"; return false; }
+                for (int k = 0; k < L.n; k++)
+                    if (!L.num(k, v[k])) { err = "bad number in element line"; return false; }
                 if (v[0] != i + 1) { err = "Unexpected element number"; return false; }
-                long type = v[1], ntags = v[2];
+                const long type = v[1], ntags = v[2];
                 if (ntags < 1) { err = "Expected at least one element tag"; return false; }
+                if (3 + ntags > L.n) { err = "Element line must have at least 4 entries"; return false; }
                 if (type == 1) {
-                    int g = int(v[3]);
+                    const int g = int(v[3]);
                     std::vector<uint32_t> bn;
-                    for (size_t k = 3 + ntags; k < v.size(); k++) bn.push_back(uint32_t(v[k] - 1));
+                    for (int k = int(3 + ntags); k < L.n; k++) bn.push_back(uint32_t(v[k] - 1));
                     if (g != current) { current = g; bgroups.emplace_back(); }
                     bgroups.back().push_back(g);
                     bnodes.push_back(bn);
                 } else if (type == 2 || type == 3) {
-                    std::vector<uint32_t> el;
-                    for (size_t k = 3 + ntags; k < v.size(); k++) el.push_back(uint32_t(v[k]));
-                    elements.push_back(el);
+                    for (int k = int(3 + ntags); k < L.n; k++) elemNodes.push_back(uint32_t(v[k]));
+                    elemOff.push_back(uint32_t(elemNodes.size()));
                 } else {
                     err = "Unsuported element type: " + std::to_string(type);
                     return false;
                 }
             }
-            if (!next(tok) || tok.empty() || tok[0].find("$End") == std::string::npos) { err = "Expected section end"; return false; }
+            if (!L.next() || !L.startsEnd()) { err = "Expected section end"; return false; }
         } else {
             while (true) {
-                if (!next(tok)) { err = "Unexpected end of file"; return false; }
-                if (!tok.empty() && tok[0].find("$End") != std::string::npos) break;
+                if (!L.next()) { err = "Unexpected end of file"; return false; }
+                if (L.startsEnd()) break;
             }
         }
     }
@@ -203,16 +273,22 @@ bool parseGmsh(const std::string& path, RawMesh& out, std::string& err)
         }
     }
     // io.d:969-987 clockwise -> counter-clockwise
+    const size_t nNodes = out.nodes.size() / 2;
+    for (uint32_t x : elemNodes)
+        if (x < 1 || x > nNodes) { err = "element references a node out of range"; return false; }
     out.eoff.assign(1, 0);
-    for (auto& el : elements) {
+    out.enodes.reserve(elemNodes.size());
+    out.eoff.reserve(elemOff.size());
+    for (size_t e = 0; e + 1 < elemOff.size(); e++) {
+        uint32_t* el = elemNodes.data() + elemOff[e];
+        const size_t m = elemOff[e + 1] - elemOff[e];
         double area = 0;
-        size_t m = el.size();
         for (size_t j = 0; j < m; j++) {
             uint32_t n1 = el[j] - 1, n2 = el[(j + 1) % m] - 1;
             area += out.nodes[2 * n1] * out.nodes[2 * n2 + 1] - out.nodes[2 * n2] * out.nodes[2 * n1 + 1];
         }
-        if (area < 0) std::reverse(el.begin(), el.end());
-        out.enodes.insert(out.enodes.end(), el.begin(), el.end());
+        if (area < 0) std::reverse(el, el + m);
+        out.enodes.insert(out.enodes.end(), el, el + m);
         out.eoff.push_back(uint32_t(out.enodes.size()));
     }
     return true;
@@ -290,6 +366,36 @@ bool parseGri(const std::string& path, RawMesh& out, std::string& err)
     return true;
 }
 
+// Threads for the geometry precompute: the CPUs this process may run on, at most
+// OMP_NUM_THREADS when that is set, at most 32.
+unsigned hostThreads()
+{
+    unsigned T = std::thread::hardware_concurrency();
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) T = unsigned(CPU_COUNT(&set));
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v > 0 && unsigned(v) < T) T = unsigned(v);
+    }
+    return T == 0 ? 1 : (T > 32 ? 32 : T);
+}
+
+// Run f(begin, end) over [0, n) on the host's cores (serially for small n).
+template <typename F>
+void parallelFor(uint32_t n, F&& f)
+{
+    static const unsigned hw = hostThreads();
+    unsigned T = n < 65536 ? 1 : hw;
+    if (T == 1) { f(0u, n); return; }
+    std::vector<std::thread> th;
+    const uint32_t chunk = (n + T - 1) / T;
+    for (unsigned t = 0; t < T; t++) {
+        const uint32_t b = t * chunk, e = std::min<uint32_t>(n, b + chunk);
+        if (b < e) th.emplace_back([&f, b, e] { f(b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
 bool buildMesh(const RawMesh& raw, BuiltMesh& m, std::string& err)
 {
     m = BuiltMesh();
@@ -304,99 +410,174 @@ bool buildMesh(const RawMesh& raw, BuiltMesh& m, std::string& err)
         if (v < 1 || v > nNodes) { err = "element references a node out of range"; return false; }
     for (uint32_t v : raw.bnodes)
         if (v >= nNodes) { err = "boundary line references a node out of range"; return false; }
-
-    EdgeMap edgeHash(maxEdges), boundaryHash(nB);
-    for (uint32_t i = 0; i < nB; i++) boundaryHash.put(raw.bnodes[2 * i], raw.bnodes[2 * i + 1], i + 1);
-
-    m.nEdgesCell.assign(nElem, 0);
-    m.cellEdges.assign(size_t(nElem) * MAXE, 0);
-    m.fluxMult.assign(size_t(nElem) * MAXE, 1);
-    m.nbr.assign(size_t(nElem) * MAXE, 0);
-    m.gradMat.assign(size_t(nElem) * 2 * MAXE, 0.0);
-    m.area.assign(nElem, 0.0); m.d.assign(nElem, 0.0); m.perim.assign(nElem, 0.0);
-    m.cx.assign(nElem, 0.0); m.cy.assign(nElem, 0.0);
-    m.bGroups.assign(nTags, {});
-    auto reserveE = [&](auto& v) { v.reserve(maxEdges / 2 + nB + 16); };
-    reserveE(m.eNode0); reserveE(m.eNode1); reserveE(m.eCell0); reserveE(m.eCell1);
-    reserveE(m.len); reserveE(m.nx); reserveE(m.ny); reserveE(m.mx); reserveE(m.my); reserveE(m.eBGroup);
-
     for (uint32_t i = 0; i < nElem; i++) {
         const uint32_t ne = raw.eoff[i + 1] - raw.eoff[i];
         if (ne < 3 || ne > uint32_t(MAXE)) { err = "element " + std::to_string(i) + " has an unsupported number of edges"; return false; }
-        m.nEdgesCell[i] = uint8_t(ne);
+    }
+
+    EdgeMap boundaryHash(nB);
+    for (uint32_t i = 0; i < nB; i++) boundaryHash.put(raw.bnodes[2 * i], raw.bnodes[2 * i + 1], i + 1);
+    // Edge discovery without a global hash: every cell-edge visit v (cell i, local
+    // edge j, v = eoff[i] + j) is bucketed under its lower node, in visit order.  The
+    // first visit of a node pair is the first bucket entry with the same upper node,
+    // which is exactly the first-visit order the reference's map produces.
+    std::vector<uint32_t> bstart(size_t(nNodes) + 1, 0);
+    FillVec<uint32_t> bhi(maxEdges), bvis(maxEdges);
+    for (uint32_t i = 0; i < nElem; i++) {
+        const uint32_t ne = raw.eoff[i + 1] - raw.eoff[i];
         const uint32_t* el = raw.enodes.data() + raw.eoff[i];
-        double area = 0.0, perim = 0.0, c0 = 0.0, c1 = 0.0;
         for (uint32_t j = 0; j < ne; j++) {
             const uint32_t n0 = el[j] - 1, n1 = el[(j + 1) % ne] - 1;
-            uint32_t edgeidx;
-            const uint32_t bval = boundaryHash.get(n0, n1);
-            if (bval == 0) {
-                const uint32_t found = edgeHash.get(n0, n1);
-                if (found != 0) {
-                    edgeidx = found - 1;
-                    m.eCell1[edgeidx] = i;
-                    m.fluxMult[size_t(i) * MAXE + j] = -1;
-                } else {
-                    const double ddx = nd[2 * n0] - nd[2 * n1], ddy = nd[2 * n0 + 1] - nd[2 * n1 + 1];
-                    const double len = std::sqrt(ddx * ddx + ddy * ddy);
-                    const double vx = nd[2 * n1 + 1] - nd[2 * n0 + 1], vy = nd[2 * n0] - nd[2 * n1];
-                    const double mag = std::sqrt(vx * vx + vy * vy);
-                    edgeidx = uint32_t(m.len.size());
-                    m.eNode0.push_back(n0); m.eNode1.push_back(n1);
-                    m.eCell0.push_back(i); m.eCell1.push_back(0);
-                    m.len.push_back(len);
-                    m.nx.push_back(vx / mag); m.ny.push_back(vy / mag);
-                    m.mx.push_back(0.5 * (nd[2 * n0] + nd[2 * n1]));
-                    m.my.push_back(0.5 * (nd[2 * n0 + 1] + nd[2 * n1 + 1]));
-                    m.eBGroup.push_back(-1);
-                    edgeHash.put(n0, n1, edgeidx + 1);
-                    m.interiorEdges.push_back(edgeidx);
-                }
-            } else {
-                const double ddx = nd[2 * n0] - nd[2 * n1], ddy = nd[2 * n0 + 1] - nd[2 * n1 + 1];
-                const double len = std::sqrt(ddx * ddx + ddy * ddy);
-                const double inv = 1.0 / len;
-                const uint32_t bNodeIdx = bval - 1;
-                int32_t g = 0;
-                for (uint32_t t = 0; t + 1 < nTags; t++)
-                    if (bNodeIdx >= raw.bgroupStart[t] && bNodeIdx < raw.bgroupStart[t + 1]) g = int32_t(t);
-                if (nTags > 0 && bNodeIdx >= raw.bgroupStart[nTags - 1]) g = int32_t(nTags) - 1;
-                edgeidx = uint32_t(m.len.size());
-                m.eNode0.push_back(n0); m.eNode1.push_back(n1);
-                m.eCell0.push_back(i); m.eCell1.push_back(0);
-                m.len.push_back(len);
-                m.nx.push_back(inv * (nd[2 * n1 + 1] - nd[2 * n0 + 1]));
-                m.ny.push_back(inv * (nd[2 * n0] - nd[2 * n1]));
-                m.mx.push_back(0.5 * (nd[2 * n0] + nd[2 * n1]));
-                m.my.push_back(0.5 * (nd[2 * n0 + 1] + nd[2 * n1 + 1]));
-                m.eBGroup.push_back(g);
-                if (nTags > 0) m.bGroups[g].push_back(edgeidx);
-                m.boundaryEdges.push_back(edgeidx);
-            }
-            m.cellEdges[size_t(i) * MAXE + j] = edgeidx;
-            perim += m.len[edgeidx];
-            const double x0 = nd[2 * n0], x1 = nd[2 * n1], y0 = nd[2 * n0 + 1], y1 = nd[2 * n1 + 1];
-            area += x0 * y1 - x1 * y0;
-            c0 += (x0 + x1) * (x0 * y1 - x1 * y0);
-            c1 += (y0 + y1) * (x0 * y1 - x1 * y0);
+            bstart[size_t(n0 < n1 ? n0 : n1) + 1]++;
         }
-        area *= 0.5;
-        if (!(area > 0)) { err = "Computed cell " + std::to_string(i) + " area negative"; return false; }
-        m.area[i] = area;
-        m.perim[i] = perim;
-        m.d[i] = (2 * area) / perim;
-        const double f = 1 / (6 * area);
-        m.cx[i] = c0 * f;
-        m.cy[i] = c1 * f;
     }
-    m.nEdges = uint32_t(m.len.size());
+    for (uint32_t k = 0; k < nNodes; k++) bstart[k + 1] += bstart[k];
+    {
+        std::vector<uint32_t> cur(bstart.begin(), bstart.end() - 1);
+        for (uint32_t i = 0; i < nElem; i++) {
+            const uint32_t ne = raw.eoff[i + 1] - raw.eoff[i];
+            const uint32_t* el = raw.enodes.data() + raw.eoff[i];
+            for (uint32_t j = 0; j < ne; j++) {
+                const uint32_t n0 = el[j] - 1, n1 = el[(j + 1) % ne] - 1;
+                const uint32_t lo = n0 < n1 ? n0 : n1, hi = n0 < n1 ? n1 : n0;
+                const uint32_t q = cur[lo]++;
+                bhi[q] = hi;
+                bvis[q] = raw.eoff[i] + j;
+            }
+        }
+    }
+    auto firstVisit = [&](uint32_t n0, uint32_t n1) {
+        const uint32_t lo = n0 < n1 ? n0 : n1, hi = n0 < n1 ? n1 : n0;
+        for (uint32_t q = bstart[lo];; q++)
+            if (bhi[q] == hi) return bvis[q];
+    };
 
-    // ghost cells (mesh.d:329-381)
+    // A: classify every visit (boundary line / first visit of an interior edge / repeat)
+    enum : uint8_t { REPEAT = 0, FIRST = 1, BOUNDARY = 2 };
+    // Written where read: kind for every visit, firstOf for non-boundary visits,
+    // bvalOf for boundary visits, edgeOfVisit for non-repeat visits.
+    FillVec<uint8_t> kind(maxEdges);
+    FillVec<uint32_t> firstOf(maxEdges), edgeOfVisit(maxEdges), bvalOf(maxEdges);
+    parallelFor(nElem, [&](uint32_t b, uint32_t e) {
+        for (uint32_t i = b; i < e; i++) {
+            const uint32_t ne = raw.eoff[i + 1] - raw.eoff[i];
+            const uint32_t* el = raw.enodes.data() + raw.eoff[i];
+            for (uint32_t j = 0; j < ne; j++) {
+                const uint32_t n0 = el[j] - 1, n1 = el[(j + 1) % ne] - 1, v = raw.eoff[i] + j;
+                const uint32_t bval = boundaryHash.get(n0, n1);
+                if (bval) { kind[v] = BOUNDARY; bvalOf[v] = bval; continue; }
+                const uint32_t f = firstVisit(n0, n1);
+                kind[v] = f == v ? FIRST : REPEAT;
+                firstOf[v] = f;
+            }
+        }
+    });
+    // B: edge numbers in visit order, boundary groups
+    uint32_t nEdges = 0;
+    for (uint64_t v = 0; v < maxEdges; v++)
+        if (kind[v] != REPEAT) edgeOfVisit[v] = nEdges++;
+    m.nEdges = nEdges;
+    // Every edge has exactly one non-repeat visit, which writes all its fields below.
+    m.eNode0.resize(nEdges); m.eNode1.resize(nEdges); m.eCell0.resize(nEdges); m.eCell1.resize(nEdges);
+    m.len.resize(nEdges); m.nx.resize(nEdges); m.ny.resize(nEdges);
+    m.mx.resize(nEdges); m.my.resize(nEdges); m.eBGroup.resize(nEdges);
+    m.bGroups.assign(nTags, {});
+    for (uint64_t v = 0; v < maxEdges; v++) {
+        if (kind[v] == FIRST) {
+            m.eBGroup[edgeOfVisit[v]] = -1;
+            m.interiorEdges.push_back(edgeOfVisit[v]);
+        } else if (kind[v] == BOUNDARY) {
+            const uint32_t edgeidx = edgeOfVisit[v], bNodeIdx = bvalOf[v] - 1;
+            int32_t g = 0;
+            for (uint32_t t = 0; t + 1 < nTags; t++)
+                if (bNodeIdx >= raw.bgroupStart[t] && bNodeIdx < raw.bgroupStart[t + 1]) g = int32_t(t);
+            if (nTags > 0 && bNodeIdx >= raw.bgroupStart[nTags - 1]) g = int32_t(nTags) - 1;
+            m.eBGroup[edgeidx] = g;
+            if (nTags > 0) m.bGroups[g].push_back(edgeidx);
+            m.boundaryEdges.push_back(edgeidx);
+        }
+    }
+
+    // C: per cell, its edges (geometry of the edges it creates) and its own geometry.
+    // The per-cell arrays are sized without a fill: this loop and the ghost and LSQ
+    // loops below write every slot, so each page is first touched by its thread.
     const uint32_t nBE = uint32_t(m.boundaryEdges.size());
     m.nCells = nElem + nBE;
-    m.area.resize(m.nCells, 0.0); m.d.resize(m.nCells, 0.0); m.perim.resize(m.nCells, 0.0);
-    m.cx.resize(m.nCells, 0.0); m.cy.resize(m.nCells, 0.0);
-    for (uint32_t b = 0; b < nBE; b++) {
+    m.nEdgesCell.resize(nElem);
+    m.cellEdges.resize(size_t(nElem) * MAXE);
+    m.fluxMult.resize(size_t(nElem) * MAXE);
+    m.nbr.resize(size_t(nElem) * MAXE);
+    m.gradMat.resize(size_t(nElem) * 2 * MAXE);
+    m.area.resize(m.nCells); m.d.resize(m.nCells); m.perim.resize(m.nCells);
+    m.cx.resize(m.nCells); m.cy.resize(m.nCells);
+    std::atomic<uint32_t> badArea(~0u);
+    parallelFor(nElem, [&](uint32_t b, uint32_t e) {
+        for (uint32_t i = b; i < e; i++) {
+            const uint32_t ne = raw.eoff[i + 1] - raw.eoff[i];
+            m.nEdgesCell[i] = uint8_t(ne);
+            const uint32_t* el = raw.enodes.data() + raw.eoff[i];
+            for (uint32_t j = ne; j < uint32_t(MAXE); j++) {
+                m.cellEdges[size_t(i) * MAXE + j] = 0;
+                m.fluxMult[size_t(i) * MAXE + j] = 1;
+            }
+            double area = 0.0, perim = 0.0, c0 = 0.0, c1 = 0.0;
+            for (uint32_t j = 0; j < ne; j++) {
+                const uint32_t n0 = el[j] - 1, n1 = el[(j + 1) % ne] - 1, v = raw.eoff[i] + j;
+                const double ddx = nd[2 * n0] - nd[2 * n1], ddy = nd[2 * n0 + 1] - nd[2 * n1 + 1];
+                const double len = std::sqrt(ddx * ddx + ddy * ddy);   // the same for either direction
+                uint32_t edgeidx;
+                if (kind[v] == REPEAT) {
+                    edgeidx = edgeOfVisit[firstOf[v]];
+                    m.fluxMult[size_t(i) * MAXE + j] = -1;
+                } else {
+                    edgeidx = edgeOfVisit[v];
+                    m.fluxMult[size_t(i) * MAXE + j] = 1;
+                    m.eNode0[edgeidx] = n0; m.eNode1[edgeidx] = n1;
+                    m.eCell0[edgeidx] = i;
+                    m.eCell1[edgeidx] = 0;   // set below: the last other visitor, or the ghost
+                    m.len[edgeidx] = len;
+                    if (kind[v] == FIRST) {
+                        const double vx = nd[2 * n1 + 1] - nd[2 * n0 + 1], vy = nd[2 * n0] - nd[2 * n1];
+                        const double mag = std::sqrt(vx * vx + vy * vy);
+                        m.nx[edgeidx] = vx / mag; m.ny[edgeidx] = vy / mag;
+                    } else {
+                        const double inv = 1.0 / len;
+                        m.nx[edgeidx] = inv * (nd[2 * n1 + 1] - nd[2 * n0 + 1]);
+                        m.ny[edgeidx] = inv * (nd[2 * n0] - nd[2 * n1]);
+                    }
+                    m.mx[edgeidx] = 0.5 * (nd[2 * n0] + nd[2 * n1]);
+                    m.my[edgeidx] = 0.5 * (nd[2 * n0 + 1] + nd[2 * n1 + 1]);
+                }
+                m.cellEdges[size_t(i) * MAXE + j] = edgeidx;
+                perim += len;
+                const double x0 = nd[2 * n0], x1 = nd[2 * n1], y0 = nd[2 * n0 + 1], y1 = nd[2 * n1 + 1];
+                area += x0 * y1 - x1 * y0;
+                c0 += (x0 + x1) * (x0 * y1 - x1 * y0);
+                c1 += (y0 + y1) * (x0 * y1 - x1 * y0);
+            }
+            area *= 0.5;
+            if (!(area > 0)) {
+                uint32_t cur = badArea.load();
+                while (i < cur && !badArea.compare_exchange_weak(cur, i)) {}
+                continue;
+            }
+            m.area[i] = area;
+            m.perim[i] = perim;
+            m.d[i] = (2 * area) / perim;
+            const double f = 1 / (6 * area);
+            m.cx[i] = c0 * f;
+            m.cy[i] = c1 * f;
+        }
+    });
+    if (badArea.load() != ~0u) { err = "Computed cell " + std::to_string(badArea.load()) + " area negative"; return false; }
+    // D: the other cell of each interior edge; the last visitor wins, as in the reference's loop
+    for (uint32_t i = 0; i < nElem; i++)
+        for (uint32_t v = raw.eoff[i]; v < raw.eoff[i + 1]; v++)
+            if (kind[v] == REPEAT) m.eCell1[edgeOfVisit[firstOf[v]]] = i;
+
+    // ghost cells (mesh.d:329-381)
+    parallelFor(nBE, [&](uint32_t b0, uint32_t b1) {
+    for (uint32_t b = b0; b < b1; b++) {
         const uint32_t e = m.boundaryEdges[b];
         const uint32_t o = m.eCell0[e];
         const double xa = nd[2 * m.eNode0[e]], ya = nd[2 * m.eNode0[e] + 1];
@@ -415,13 +596,16 @@ bool buildMesh(const RawMesh& raw, BuiltMesh& m, std::string& err)
             xr = -(2.0 * dd - x1);
             yr = m.cy[o];
         }
+        m.area[nElem + b] = 0.0; m.d[nElem + b] = 0.0; m.perim[nElem + b] = 0.0;
         m.cx[nElem + b] = xr;
         m.cy[nElem + b] = yr;
         m.eCell1[e] = nElem + b;
     }
+    });
 
     // neighbours + LSQ matrix (mesh.d:481-642)
-    for (uint32_t i = 0; i < nElem; i++) {
+    parallelFor(nElem, [&](uint32_t i0, uint32_t i1) {
+    for (uint32_t i = i0; i < i1; i++) {
         const uint32_t ne = m.nEdgesCell[i];
         double A[MAXE][2] = {};
         for (uint32_t j = 0; j < ne; j++) {
@@ -431,6 +615,7 @@ bool buildMesh(const RawMesh& raw, BuiltMesh& m, std::string& err)
             A[j][0] = m.cx[nb] - m.cx[i];
             A[j][1] = m.cy[nb] - m.cy[i];
         }
+        for (uint32_t j = ne; j < uint32_t(MAXE); j++) m.nbr[size_t(i) * MAXE + j] = 0;
         double b[2][2];
         for (int r = 0; r < 2; r++)
             for (int c = 0; c < 2; c++) {
@@ -445,6 +630,7 @@ bool buildMesh(const RawMesh& raw, BuiltMesh& m, std::string& err)
             for (int k = 0; k < MAXE; k++)
                 m.gradMat[size_t(i) * 2 * MAXE + r * MAXE + k] = inv[r][0] * A[k][0] + inv[r][1] * A[k][1];
     }
+    });
     return true;
 }
 

@@ -61,6 +61,22 @@ def test_generated_mesh_roundtrip_through_gmsh_file(maker, tmp_path):
     _compare_geometry(b, om)
 
 
+@pytest.mark.parametrize("maker", ["tri", "quad"])
+def test_threaded_build_matches_oracle(maker, tmp_path):
+    """Above 65536 cells the geometry precompute runs on several threads and fills
+    its arrays without a zero pass; the result must not change by a bit."""
+    raw = {"tri": lambda: meshgen.mms_square(190, 0.1, 2),
+           "quad": lambda: meshgen.cylinder_ogrid(384, 192, stretch=1.02)}[maker]()
+    om = oracle.Mesh(raw)
+    assert om.n_interior > 65536
+    b = build_mesh(raw)
+    assert (b.n_cells, b.n_interior, b.n_edges) == (om.n_cells, om.n_interior, om.n_edges)
+    _compare_geometry(b, om)
+    path = str(tmp_path / "m.msh")
+    meshgen.write_gmsh(raw, path)
+    _compare_geometry(import_mesh(path), om)
+
+
 def test_negative_area_rejected():
     raw = meshgen.mms_square(4, 0.0, 1)
     raw.elem_nodes = raw.elem_nodes.reshape(-1, 3)[:, ::-1].reshape(-1).copy()   # clockwise
