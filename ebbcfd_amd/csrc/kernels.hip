@@ -185,6 +185,9 @@ DEV void ld4(const double4* p, uint32_t i, double o[4])
     o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
 }
 DEV void st4(double4* p, uint32_t i, const double o[4]) { p[i] = make_double4(o[0], o[1], o[2], o[3]); }
+// gradients and limiters are read once, by the next kernel: stored non-temporally so
+// they do not displace the neighbour data that kernel's gathers reuse from L2
+typedef double ntd2 __attribute__((ext_vector_type(2)));
 
 // ghost state from the owner state (solve.d:520-689); the owner is the only writer
 DEV void ghostState(const BFace& b, const double qo[4], double qg[4], const double qgOld[4])
@@ -491,11 +494,11 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
     __syncthreads();
     const uint32_t base = bid * BLK, n = (m.N - base) < BLK ? (m.N - base) : BLK;
     double2* gd = reinterpret_cast<double2*>(a.grad + size_t(base) * 8);
-    for (uint32_t c = tid; c < 4 * n; c += BLK) gd[c] = make_double2(sg[c / 4][2 * (c % 4)], sg[c / 4][2 * (c % 4) + 1]);
+    for (uint32_t c = tid; c < 4 * n; c += BLK) __builtin_nontemporal_store((ntd2){sg[c / 4][2 * (c % 4)], sg[c / 4][2 * (c % 4) + 1]}, reinterpret_cast<ntd2*>(gd) + c);
     if (LWS) {
         double2* ld = reinterpret_cast<double2*>(a.lim + size_t(base) * LWS);
         constexpr uint32_t PR = LWS / 2;
-        for (uint32_t c = tid; c < PR * n; c += BLK) ld[c] = make_double2(sl[c / PR][2 * (c % PR)], sl[c / PR][2 * (c % PR) + 1]);
+        for (uint32_t c = tid; c < PR * n; c += BLK) __builtin_nontemporal_store((ntd2){sl[c / PR][2 * (c % PR)], sl[c / PR][2 * (c % PR) + 1]}, reinterpret_cast<ntd2*>(ld) + c);
     }
     // MODE 1 (unlimited stage): every limiter is 1; the face kernels take LP = 2 and read none
 }
@@ -777,7 +780,7 @@ DEV double warpMax(double v)
 template <int KIND>
 DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t i, const double r[4],
                    double sAve, const double* qinI, const double* met, double& dtCand, double& rCand,
-                   double* qNew = nullptr, double* qOld = nullptr)
+                   double* qNew = nullptr, double* qOld = nullptr, const double* qnI = nullptr)
 {
     const double cper = met ? met[0] : m.cper[i], cd = met ? met[1] : m.cd[i], nia = met ? met[2] : m.nia[i];
     sAve /= cper;
@@ -807,7 +810,10 @@ DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, ui
     const double dt = a.sc->dt;
     const double h = p.localTimestep ? hc : dt;
     double qn[4], o[4];
-    if (KIND != SK_RESIDUAL) ld4(a.qn, i, qn);
+    if (KIND != SK_RESIDUAL) {
+        if (qnI) { qn[0] = qnI[0]; qn[1] = qnI[1]; qn[2] = qnI[2]; qn[3] = qnI[3]; }
+        else ld4(a.qn, i, qn);
+    }
     if (KIND == SK_RESIDUAL) {
         st4(a.R, i, R);
     } else if (KIND == SK_EULER) {
@@ -1186,12 +1192,21 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
     const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x);
     const uint4 d = t.desc[tile];
     const uint32_t nC = d.x, nF = d.y, tid = threadIdx.x;
+    // the first cell round's ids are loaded ahead of the faces; the next round's face
+    // record is loaded one round ahead, so each round waits for one memory latency
+    // (cell data) instead of two (record, then cell data)
+    uint32_t giNext = tid < nC ? t.ids[d.z + tid] : 0u;
+    uint4 rNext = make_uint4(0, 0, 0, 0);
+    uint32_t n1Next = 0;
+    if (tid < nF) { rNext = t.fref[d.w + tid]; n1Next = t.fn1[d.w + tid]; }
     for (uint32_t k = tid; k < nF; k += BLK) {
-        const uint4 r = t.fref[d.w + k];
+        const uint4 r = rNext;
+        const uint32_t n1 = n1Next;
+        if (k + BLK < nF) { rNext = t.fref[d.w + k + BLK]; n1Next = t.fn1[d.w + k + BLK]; }
         const uint32_t f = r.x;
         double2 n, mid;
         double len;
-        edgeGeometry(f < m.Fi, m.node[r.w], m.node[t.fn1[d.w + k]], n, mid, len);
+        edgeGeometry(f < m.Fi, m.node[r.w], m.node[n1], n, mid, len);
         double qL[4], qR[4], F[4], sMax;
         uint64_t errKey;
         if (f < m.Fi) {
@@ -1231,6 +1246,21 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
         slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
         slot[4] = len * sMax;
     }
+    // each cell round's per-cell inputs are loaded one round ahead (the first round's
+    // before the barrier, which waits for LDS only)
+    double metN[3] = {0, 0, 0}, qnN[4] = {0, 0, 0, 0}, qiN[4] = {0, 0, 0, 0};
+    const bool wantQn = a.kind != SK_RESIDUAL, wantQi = a.kind == SK_TVD_2;
+    uint32_t cfN[S] = {};
+    auto loadCellIn = [&](uint32_t c) {
+        metN[0] = m.cper[c]; metN[1] = m.cd[c]; metN[2] = m.nia[c];
+        if (wantQn) ld4(a.qn, c, qnN);
+        if (wantQi) ld4(a.qin, c, qiN);
+    };
+    auto loadCf = [&](uint32_t k) {
+#pragma unroll
+        for (int j = 0; j < S; j++) cfN[j] = t.cf[size_t(d.z + k) * S + j];
+    };
+    if (tid < nC) { loadCellIn(giNext); loadCf(tid); }
     if (tid < 5) fsd[5 * nF + tid] = 0.0;   // padding face
     ldsBarrier();
     double dtCand = INFINITY, rCand = 0.0;
@@ -1239,30 +1269,42 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
         const uint32_t k = k0 + tid;
         const bool have = k < nC;
         double qNew[4] = {0, 0, 0, 0}, qOld[4] = {0, 0, 0, 0};
-        uint32_t gi = 0;
+        uint32_t gi = giNext;
+        double met[3], qnC[4], qiC[4];
+#pragma unroll
+        for (int x = 0; x < 3; x++) met[x] = metN[x];
+#pragma unroll
+        for (int x = 0; x < 4; x++) { qnC[x] = qnN[x]; qiC[x] = qiN[x]; }
+        uint32_t cfC[S];
+#pragma unroll
+        for (int j = 0; j < S; j++) cfC[j] = cfN[j];
+        if (k + BLK < nC) {
+            giNext = t.ids[d.z + k + BLK];
+            loadCellIn(giNext);
+            loadCf(k + BLK);
+        }
         if (have) {
             double r[4] = {0.0, 0.0, 0.0, 0.0};
             double sAve = 0;
 #pragma unroll
             for (int j = 0; j < S; j++) {
-                const uint32_t fe = t.cf[size_t(d.z + k) * S + j];
+                const uint32_t fe = cfC[j];
                 const double* lf = fsd + 5 * (fe & 0x7fffu);
                 if (fe >> 15) { r[0] = r[0] - lf[0]; r[1] = r[1] - lf[1]; r[2] = r[2] - lf[2]; r[3] = r[3] - lf[3]; }
                 else { r[0] = r[0] + lf[0]; r[1] = r[1] + lf[1]; r[2] = r[2] + lf[2]; r[3] = r[3] + lf[3]; }
                 sAve += lf[4];
             }
-            gi = t.ids[d.z + k];
             switch (a.kind) {
-            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
-            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
-            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
-            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
-            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
-            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
-            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
-            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
-            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand, qNew, qOld); break;
-            default: cellStage<SK_RESIDUAL>(m, p, a, gi, r, sAve, nullptr, nullptr, dtCand, rCand); break;
+            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
+            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
+            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
+            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
+            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
+            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
+            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
+            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
+            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
+            default: cellStage<SK_RESIDUAL>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
             }
         }
         if (a.rmsBlocks) rmsBlockLeaf(have, qNew, qOld, have ? m.refId[gi] : 0u, a.rmsBlocks);
