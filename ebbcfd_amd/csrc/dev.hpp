@@ -179,8 +179,9 @@ struct FaceTileArgs {
     const uint32_t* ids;           // own cells' device ids
     const uint16_t* cf;            // S per own cell: local face, bit 15 set where the cell is not the owner;
                                    // nF (an all-zero slot) for padding
-    const uint4* fref;             // per face: global face id, cell a, cell b (ghost for boundary faces), node 0
-    const uint32_t* fn1;           // per face: node 1 (normal, midpoint and length are recomputed from the nodes)
+    const uint4* fref;             // per face: cell a, cell b, node 0, node 1 (normal, midpoint and length are
+                                   // recomputed from the nodes); boundary face b has the ghost N + b as cell b
+    const uint32_t* fid;           // per face: global face id, read only to report a NaN flux
 };
 
 // Launchers (kernels.hip).  All run on `stream`.

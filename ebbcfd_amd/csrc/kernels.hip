@@ -1197,51 +1197,52 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
     // (cell data) instead of two (record, then cell data)
     uint32_t giNext = tid < nC ? t.ids[d.z + tid] : 0u;
     uint4 rNext = make_uint4(0, 0, 0, 0);
-    uint32_t n1Next = 0;
-    if (tid < nF) { rNext = t.fref[d.w + tid]; n1Next = t.fn1[d.w + tid]; }
+    if (tid < nF) rNext = t.fref[d.w + tid];
     for (uint32_t k = tid; k < nF; k += BLK) {
-        const uint4 r = rNext;
-        const uint32_t n1 = n1Next;
-        if (k + BLK < nF) { rNext = t.fref[d.w + k + BLK]; n1Next = t.fn1[d.w + k + BLK]; }
-        const uint32_t f = r.x;
+        const uint4 r = rNext;   // cell a, cell b, node 0, node 1
+        if (k + BLK < nF) rNext = t.fref[d.w + k + BLK];
+        const bool interior = r.y < m.N;
         double2 n, mid;
         double len;
-        edgeGeometry(f < m.Fi, m.node[r.w], m.node[n1], n, mid, len);
+        edgeGeometry(interior, m.node[r.z], m.node[r.w], n, mid, len);
         double qL[4], qR[4], F[4], sMax;
-        uint64_t errKey;
-        if (f < m.Fi) {
+        bool isB = false;
+        uint32_t b = 0;
+        if (interior) {
             double qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
-            ld4(a.qin, r.y, qa);
-            ld4(a.qin, r.z, qb);
-            if (ORDER > 1 || VISC) { loadGrad(a.grad, r.y, ga); loadGrad(a.grad, r.z, gb); }
+            ld4(a.qin, r.x, qa);
+            ld4(a.qin, r.y, qb);
+            if (ORDER > 1 || VISC) { loadGrad(a.grad, r.x, ga); loadGrad(a.grad, r.y, gb); }
             double2 ca = make_double2(0, 0), cb = make_double2(0, 0);
             if (ORDER > 1) {
-                loadLim<LP>(a.lim, r.y, lxa, lya);
-                loadLim<LP>(a.lim, r.z, lxb, lyb);
-                ca = m.cen[r.y]; cb = m.cen[r.z];
+                loadLim<LP>(a.lim, r.x, lxa, lya);
+                loadLim<LP>(a.lim, r.y, lxb, lyb);
+                ca = m.cen[r.x]; cb = m.cen[r.y];
             }
             interiorFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, n, mid, p, qL, qR, F, sMax);
-            errKey = uint64_t(m.Fb) + f;
         } else {
-            const uint32_t b = f - m.Fi;
+            isB = true;
+            b = r.y - m.N;   // the ghost of boundary face b is cell N + b
             double qo[4], qg[4], gO[8], lx[4], ly[4];
-            ld4(a.qin, r.y, qo);
-            ld4(a.qin, r.z, qg);
-            if (ORDER > 1 || VISC) loadGrad(a.grad, r.y, gO);
+            ld4(a.qin, r.x, qo);
+            ld4(a.qin, r.y, qg);
+            if (ORDER > 1 || VISC) loadGrad(a.grad, r.x, gO);
             double2 co = make_double2(0, 0);
             if (ORDER > 1) {
-                loadLim<LP>(a.lim, r.y, lx, ly);
-                co = m.cen[r.y];
+                loadLim<LP>(a.lim, r.x, lx, ly);
+                co = m.cen[r.x];
             }
             boundaryFace<FLUX, VISC, ORDER, LP>(qo, gO, lx, ly, co, qg, m.bface[b], n, mid, p, qL, qR, F, sMax);
-            errKey = b;
             // boundary faces (each in exactly one tile) feed the force integral
+            const uint32_t f = m.Fi + b;
             *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) =
                 make_double4(len * F[0], len * F[1], len * F[2], len * F[3]);
             a.lenS[f] = len * sMax;
         }
+        // boundary edges are checked first in the reference: their keys come first
         if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
-            atomicMin(&a.sc->errEdge, (uint64_t(a.stageId) << 32) | errKey);
+            atomicMin(&a.sc->errEdge,
+                      (uint64_t(a.stageId) << 32) | (isB ? uint64_t(b) : uint64_t(m.Fb) + t.fid[d.w + k]));
         double* slot = fsd + 5 * k;
         slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
         slot[4] = len * sMax;

@@ -145,7 +145,7 @@ struct ebb_solver {
     bool useTiles = false;
     uint32_t tileCells = 0;
     DBuf<uint4> d_ftDesc, d_ftRef;
-    DBuf<uint32_t> d_ftIds, d_ftN1;
+    DBuf<uint32_t> d_ftIds, d_ftFid;
     DBuf<uint16_t> d_ftCf;
     DBuf<double2> d_node;
     FaceTileArgs ftiles{};
@@ -383,11 +383,11 @@ int buildFaceTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& 
     HIPCHK(trackUpload(s, s->d_ftRef, h.fref));
     HIPCHK(trackUpload(s, s->d_ftIds, h.ids));
     HIPCHK(trackUpload(s, s->d_ftCf, h.cf));
-    HIPCHK(trackUpload(s, s->d_ftN1, h.fn1));
+    HIPCHK(trackUpload(s, s->d_ftFid, h.fid));
     FaceTileArgs ft{};
     ft.nTiles = h.nTiles;
     ft.ldsBytes = lds;
-    ft.desc = s->d_ftDesc.p; ft.ids = s->d_ftIds.p; ft.cf = s->d_ftCf.p; ft.fref = s->d_ftRef.p; ft.fn1 = s->d_ftN1.p;
+    ft.desc = s->d_ftDesc.p; ft.ids = s->d_ftIds.p; ft.cf = s->d_ftCf.p; ft.fref = s->d_ftRef.p; ft.fid = s->d_ftFid.p;
     s->ftiles = ft;
     s->useFaceTiles = true;
     return EBB_OK;
@@ -1360,7 +1360,7 @@ void ebb_solver_destroy(ebb_solver* s)
     s->d_src.release(); s->d_qA.release(); s->d_qB.release(); s->d_T.release(); s->d_T2.release();
     s->d_acc.release(); s->d_R.release(); s->d_estate.release();
     s->d_tDesc.release(); s->d_tBlob.release();
-    s->d_ftDesc.release(); s->d_ftRef.release(); s->d_ftIds.release(); s->d_ftCf.release(); s->d_ftN1.release(); s->d_node.release();
+    s->d_ftDesc.release(); s->d_ftRef.release(); s->d_ftIds.release(); s->d_ftCf.release(); s->d_ftFid.release(); s->d_node.release();
     s->d_devOf.release(); s->d_rmsBlocks.release();
     if (s->d_sc) (void)hipFree(s->d_sc);
     if (s->h_sc) (void)hipHostFree(s->h_sc);

@@ -285,8 +285,8 @@ bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const st
         for (uint32_t k = 0; k < nF; k++) {
             const uint32_t f = fl[k];
             const uint32_t ca = L.fcell[f].x, cb = f < L.Fi ? L.fcell[f].y : N + (f - L.Fi);
-            h.fref.push_back(make_uint4(f, ca, cb, L.fnode[f].x));
-            h.fn1.push_back(L.fnode[f].y);
+            h.fref.push_back(make_uint4(ca, cb, L.fnode[f].x, L.fnode[f].y));
+            h.fid.push_back(f);
         }
         h.maxF = std::max(h.maxF, nF);
         h.sumF += nF;

@@ -41,7 +41,7 @@ struct TileHost {
 
 struct FaceTileHost {
     std::vector<uint4> desc, fref;
-    std::vector<uint32_t> ids, fn1;
+    std::vector<uint32_t> ids, fid;
     std::vector<uint16_t> cf;
     uint32_t nTiles = 0, maxF = 0;
     uint64_t sumF = 0;
