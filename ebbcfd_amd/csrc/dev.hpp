@@ -109,6 +109,7 @@ struct StageArgs {
     int first;             // first stage of a step (resets Rmax)
     int last;              // last stage of a step (NaN scan of the new state)
     int checkActive;       // honour sc->active (time loop) or always run
+    int deferReduce;       // leave the partials for the step's RMS kernel instead of reducing them now
     int recordErrors;
     uint32_t stageId;      // global stage counter for error ordering
     uint32_t stepId;
@@ -210,6 +211,10 @@ struct DiagArgs {
     int64_t histBase;        // step index of hist[0]
     int64_t histCap;
     int64_t maxSteps;        // stop after this many completed steps (<0: none)
+    // stage partials left by the step's stages (deferReduce): stage s holds nPart
+    // (min dt candidate, max |R|) pairs at partial + s * pstride; nStages 0: none
+    const double* partial;
+    uint32_t nPart, pstride, nStages;
     DevScalars* sc;
 };
 constexpr int HIST_FIELDS = 10;

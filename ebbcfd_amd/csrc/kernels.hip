@@ -1342,7 +1342,7 @@ void launchFaceTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, 
     if (S == 3) faceTileS<3>(m, p, a, t, st);
     else if (S == 4) faceTileS<4>(m, p, a, t, st);
     else faceTileS<6>(m, p, a, t, st);
-    hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nTiles, a.sc, a.checkActive);
+    if (!a.deferReduce) hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nTiles, a.sc, a.checkActive);
 }
 
 template <int S, int FLUX, int VISC, int LP>
@@ -1371,7 +1371,7 @@ void launchStageTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a,
     if (S == 3) tileS<3>(m, p, a, t, st);
     else if (S == 4) tileS<4>(m, p, a, t, st);
     else tileS<6>(m, p, a, t, st);
-    hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nBlocks, a.sc, a.checkActive);
+    if (!a.deferReduce) hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nBlocks, a.sc, a.checkActive);
 }
 
 // newDt = min over the stage's workgroup partials; Rmax = max with the step's running value
@@ -1466,6 +1466,33 @@ DEV void chunkLevels(double (*v)[RMS_CHUNK_BLOCKS], uint32_t cnt, double* out, u
 
 // The boundary force (mesh.d:645-671): len*flux of the force faces summed in the
 // reference's edge order by one lane, from LDS-staged values; NT threads stage.
+// The stage reductions of a time-loop step, done here instead of after each stage:
+// newDt is the last stage's minimum (every stage overwrites it), Rmax the maximum
+// over the step's stages; min and max are exact in any order.
+template <int NT>
+DEV void stageReduce(const DiagArgs& d, double* lds)
+{
+    double a0 = INFINITY, b0 = 0.0;
+    for (uint32_t s = 0; s < d.nStages; s++) {
+        const double* pp = d.partial + size_t(s) * d.pstride;
+        const bool lastStage = s + 1 == d.nStages;
+        for (uint32_t k = threadIdx.x; k < d.nPart; k += NT) {
+            if (lastStage) a0 = fmin(a0, pp[2 * k]);
+            b0 = fmax(b0, pp[2 * k + 1]);
+        }
+    }
+    a0 = warpMin(a0);
+    b0 = warpMax(b0);
+    const uint32_t w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) { lds[2 * w] = a0; lds[2 * w + 1] = b0; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t k = 1; k < NT / 64; k++) { a0 = fmin(a0, lds[2 * k]); b0 = fmax(b0, lds[2 * k + 1]); }
+        d.sc->newDtBits = (unsigned long long)__double_as_longlong(a0);
+        const double prev = __longlong_as_double((long long)d.sc->rmaxBits);
+        if (b0 > prev) d.sc->rmaxBits = (unsigned long long)__double_as_longlong(b0);
+    }
+}
 template <int NT>
 DEV void forceSum(const DiagArgs& d, double* fb)
 {
@@ -1503,7 +1530,8 @@ __global__ __launch_bounds__(RBLK) void k_rms_chunks(DiagArgs d)
 {
     if (!d.sc->active) return;
     __shared__ double v[8][RMS_CHUNK_BLOCKS];
-    if (blockIdx.x == gridDim.x - 1) { forceSum<RBLK>(d, &v[0][0]); return; }
+    if (blockIdx.x == gridDim.x - 2) { forceSum<RBLK>(d, &v[0][0]); return; }
+    if (blockIdx.x == gridDim.x - 1) { if (d.nStages) stageReduce<RBLK>(d, &v[0][0]); return; }
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint64_t c0 = uint64_t(blockIdx.x) * RMS_CHUNK_BLOCKS;
     const uint32_t cnt = uint32_t((nb - c0) < RMS_CHUNK_BLOCKS ? (nb - c0) : RMS_CHUNK_BLOCKS);
@@ -1537,6 +1565,7 @@ __global__ __launch_bounds__(RFB) void k_rms_blocks(DiagArgs d, uint32_t nfull)
     __shared__ double v[8][RMS_CHUNK_BLOCKS];
     const uint64_t nb = uint64_t(d.L) / 16;
     if (blockIdx.x == nfull + 1) { forceSum<RFB>(d, &v[0][0]); return; }
+    if (blockIdx.x == nfull + 2) { if (d.nStages) stageReduce<RFB>(d, &v[0][0]); return; }
     const uint64_t c0 = uint64_t(blockIdx.x) * RMS_CHUNK_BLOCKS;
     auto leaf = [&](uint64_t b, double o[8]) {
         if (b * 16 < d.N) {
@@ -1801,7 +1830,7 @@ void launchUpdate(const DevMesh& m, const PhysParams& p, const StageArgs& a, int
     else if (S == 4) updS<4>(m, p, a, st);
     else updS<6>(m, p, a, st);
     const uint32_t g = nblk(m.N);
-    if (g) hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, g, a.sc, a.checkActive);
+    if (g && !a.deferReduce) hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, g, a.sc, a.checkActive);
 }
 
 void launchRmsChunks(const DiagArgs& d, hipStream_t st)
@@ -1809,11 +1838,12 @@ void launchRmsChunks(const DiagArgs& d, hipStream_t st)
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint32_t nc = uint32_t((nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS);
     if (d.blockSums) {
-        // full chunks, the partial chunk (its workgroup returns at once when there is none), the force
+        // full chunks, the partial chunk (its workgroup returns at once when there is none), the
+        // force, the stage reductions
         const uint32_t nfull = uint32_t(nb / RMS_CHUNK_BLOCKS);
-        hipLaunchKernelGGL(k_rms_blocks, dim3(nfull + 2), dim3(RFB), 0, st, d, nfull);
+        hipLaunchKernelGGL(k_rms_blocks, dim3(nfull + 3), dim3(RFB), 0, st, d, nfull);
     } else {
-        hipLaunchKernelGGL(k_rms_chunks, dim3(nc + 1), dim3(RBLK), 0, st, d);
+        hipLaunchKernelGGL(k_rms_chunks, dim3(nc + 2), dim3(RBLK), 0, st, d);
     }
 }
 

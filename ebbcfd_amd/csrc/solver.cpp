@@ -168,6 +168,9 @@ struct ebb_solver {
     int lastLimit = 1;
     bool haveStage = false;
     uint32_t stageCounter = 0;
+    // stage partials: pairs per stage (the stage kernel's workgroups), region stride,
+    // and the stage of the current time-loop step
+    uint32_t nPart = 0, pstride = 0, stageInStep = 0;
     int64_t stepsTotal = 0;
     double t0 = 0, dt0 = 0;
     std::vector<double> history;     // HIST_FIELDS per completed step of ebb_run
@@ -215,6 +218,13 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
     a.rmsBlocks = (last && checkActive && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
     a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
     a.checkActive = checkActive; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = stepId;
+    if (checkActive) {
+        // time loop: each stage leaves its partials in its own region; the step's RMS
+        // kernel reduces them (nothing in the step reads newDt or Rmax before that)
+        a.deferReduce = 1;
+        a.partial = s->d_partial.p + size_t(s->stageInStep) * s->pstride;
+        s->stageInStep++;
+    }
     if (s->useTiles && !estate && kind != SK_RESIDUAL) {
         launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
     } else if (s->useFaceTiles && !estate && kind != SK_RESIDUAL) {
@@ -236,6 +246,7 @@ void stepStages(ebb_solver* s, int checkActive, uint32_t stepId)
     const Config& c = s->cfg;
     const int msl = c.multistageLimiting ? 1 : 0, lts = c.localTimestep ? 1 : 0;
     double4 *q = s->qcur(), *nx = s->qnext(), *T = s->d_T.p, *T2 = s->d_T2.p;
+    s->stageInStep = 0;
     switch (c.integratorKind()) {
     case IntEuler:
         stage(s, SK_EULER, q, q, nx, 1, 1, 1, 1, checkActive, nullptr, nullptr, stepId);
@@ -280,6 +291,7 @@ DiagArgs diagArgs(ebb_solver* s, const double4* qnew, const double4* qold, int64
     d.dynamicDt = s->cfg.dynamicDt; d.cflAdjust = s->cfg.cflAdjust;
     d.hist = s->d_hist.p; d.histBase = histBase; d.histCap = s->histCap;
     d.maxSteps = maxSteps;
+    d.partial = s->d_partial.p; d.nPart = s->nPart; d.pstride = s->pstride; d.nStages = s->stageInStep;
     d.sc = s->d_sc;
     return d;
 }
@@ -584,8 +596,10 @@ int buildDevice(ebb_solver* s)
     // leaf blocks need 16 aligned consecutive reference cells per 16 lanes of the last stage
     s->rmsFused = !s->useTiles && ((s->useFaceTiles || !s->refOf.empty()) ? c.tileOrder == TILE_BLOCKS : true);
     if (s->rmsFused) HIPCHK((hipError_t)trackAlloc(s, s->d_rmsBlocks, size_t((N + 15) / 16) * 8));
-    HIPCHK((hipError_t)trackAlloc(s, s->d_partial,
-                                  2 * std::max<size_t>({(N + 255) / 256, s->tiles.nBlocks, s->ftiles.nTiles}) + 2));
+    // the stage kernel's workgroup count: one (min dt, max |R|) pair each
+    s->nPart = s->useTiles ? s->tiles.nBlocks : (s->useFaceTiles ? s->ftiles.nTiles : uint32_t((N + 255) / 256));
+    s->pstride = 2 * uint32_t(std::max<size_t>({(N + 255) / 256, s->tiles.nBlocks, s->ftiles.nTiles})) + 2;
+    HIPCHK((hipError_t)trackAlloc(s, s->d_partial, size_t(s->pstride) * 4));   // up to 4 stages per step
     HIPCHK((hipError_t)trackAlloc(s, s->d_hist, size_t(s->histCap) * HIST_FIELDS));
     const uint64_t nb = uint64_t(C) / 16;
     const uint64_t nchunks = (nb + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
