@@ -43,6 +43,22 @@ struct DevScalars {
     double rms[4];
     double force[2];
     double fsum[2];                 // boundary force sums of the current step, before the AoA rotation
+    int32_t forcePending;           // 1: the last step's boundary force is still to be summed
+    int32_t pad2;
+    int64_t forceSlot;              // its history row (index into the history buffer), -1: none
+};
+
+// The ordered boundary-force sum of a time-loop step (mesh.d:645-671), when it is
+// left to the next gradient kernel: one extra workgroup of that kernel sums it while
+// the others compute gradients, so its long dependent chain of additions is off the
+// step's critical path.  The face fluxes it reads are only rewritten by the tile
+// kernel that follows.
+struct ForceArgs {
+    const uint32_t* faces;   // force-boundary faces in bGroups order
+    uint32_t n;              // 0: no force boundary (no extra workgroup)
+    const double* lenF;      // len*flux per face
+    double rot[4];           // freestream AoA rotation (row-major)
+    double* hist;            // history rows (HIST_FIELDS doubles each)
 };
 
 struct DevMesh {
@@ -113,6 +129,7 @@ struct StageArgs {
     int recordErrors;
     uint32_t stageId;      // global stage counter for error ordering
     uint32_t stepId;
+    ForceArgs force;       // gradient kernels: the pending boundary force, if any
 };
 
 // Tiles for the fused stage kernel: each tile is a compact group of interior
@@ -215,11 +232,14 @@ struct DiagArgs {
     // (min dt candidate, max |R|) pairs at partial + s * pstride; nStages 0: none
     const double* partial;
     uint32_t nPart, pstride, nStages;
+    int deferForce;          // 1: the force is left pending for the next gradient kernel
     DevScalars* sc;
 };
 constexpr int HIST_FIELDS = 10;
 constexpr uint32_t RMS_CHUNK_BLOCKS = 1024;     // 16-element blocks per workgroup
 void launchRmsChunks(const DiagArgs& d, hipStream_t stream);
+// sums a pending boundary force (end of a time-loop batch); no-op when none is pending
+void launchPendingForce(const ForceArgs& f, DevScalars* sc, hipStream_t stream);
 void launchFinalize(const DiagArgs& d, hipStream_t stream);
 void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStream_t stream);
 

@@ -29,6 +29,7 @@ namespace ebb {
 #define DEV __device__ __forceinline__
 constexpr double G_ = 1.4;
 constexpr int BLK = 256;
+constexpr int FORCE_LDS = 1024;   // force faces staged per pass by a gradient kernel's force workgroup
 
 enum { BT_FULL = 0, BT_CONSTP, BT_INVWALL, BT_VISCWALL, BT_SYM, BT_TPIN, BT_DIRICHLET };
 
@@ -438,14 +439,94 @@ DEV void ghostFromOwner(const BFace& bf, const double qo[4], const double4* qbuf
     ghostState(bf, qo, qg, old);
 }
 
+// The boundary force sum of mesh.d:645-671: (len*flux) momentum components of the
+// force faces added in order, two interleaved chains, by thread 0 (its result).
+// fp: LDS staging of CAP (x, y) pairs.
+template <int NT, int CAP>
+DEV void orderedForce(const uint32_t* faces, uint32_t nF, const double* lenF, double2* fp, double& f0, double& f1)
+{
+    f0 = 0; f1 = 0;
+    for (uint32_t k0 = 0; k0 < nF; k0 += CAP) {
+        const uint32_t cnt = (nF - k0) < uint32_t(CAP) ? (nF - k0) : uint32_t(CAP);
+        __syncthreads();
+        // all of a thread's face ids, then all of their fluxes, are loaded before any is
+        // used: two round trips per pass instead of two per face
+        constexpr int FS = (CAP + NT - 1) / NT;
+        uint32_t fi[FS];
+#pragma unroll
+        for (int u = 0; u < FS; u++) {
+            const uint32_t x = threadIdx.x + uint32_t(u) * NT;
+            fi[u] = x < cnt ? faces[k0 + x] : 0u;
+        }
+        double2 fv[FS];
+#pragma unroll
+        for (int u = 0; u < FS; u++) {
+            const uint32_t x = threadIdx.x + uint32_t(u) * NT;
+            fv[u] = x < cnt ? make_double2(lenF[size_t(fi[u]) * 4 + 1], lenF[size_t(fi[u]) * 4 + 2])
+                            : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < FS; u++) {
+            const uint32_t x = threadIdx.x + uint32_t(u) * NT;
+            if (x < cnt) fp[x] = fv[u];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // the two ordered chains interleave; 32 LDS reads are in flight ahead of the adds
+            uint32_t x = 0;
+            for (; x + 32 <= cnt; x += 32) {
+                double2 v[32];
+#pragma unroll
+                for (int u = 0; u < 32; u++) v[u] = fp[x + u];
+#pragma unroll
+                for (int u = 0; u < 32; u++) { f0 += v[u].x; f1 += v[u].y; }
+            }
+            for (; x < cnt; x++) { const double2 v = fp[x]; f0 += v.x; f1 += v.y; }
+        }
+    }
+}
+
+// a pending force (k_finalize left it): rotated drag/lift into the step's history row
+// and the scalars, as k_finalize would have written them
+template <int NT, int CAP>
+DEV void pendingForce(const ForceArgs& f, DevScalars* sc, double2* fp)
+{
+    if (!sc->forcePending) return;
+    double f0, f1;
+    orderedForce<NT, CAP>(f.faces, f.n, f.lenF, fp, f0, f1);
+    if (threadIdx.x == 0) {
+        const double ld0 = 0.0 + f.rot[0] * f0 + f.rot[1] * f1;
+        const double ld1 = 0.0 + f.rot[2] * f0 + f.rot[3] * f1;
+        const int64_t slot = sc->forceSlot;
+        if (slot >= 0) {
+            double* h = f.hist + slot * HIST_FIELDS;
+            h[7] = ld0; h[8] = ld1;
+        }
+        sc->force[0] = ld0; sc->force[1] = ld1;
+        sc->forcePending = 0;
+    }
+}
+
 // MODE 0: ghost states only (order 1); 1: unlimited gradient (no limiter stored);
 // 2: scalar limiter; 3: scalar limiter followed by the LP vector limiter (lim
 // stored as x,y pairs).
 template <int S, int MODE>
 __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, StageArgs a)
 {
+    // with a force boundary, workgroup 0 (dispatched first) sums a pending boundary force
+    // of the previous step, whether or not the loop is still active; the others are the
+    // cell blocks
+    constexpr int LWS = MODE == 3 ? 8 : (MODE == 2 ? 4 : 0);
+    __shared__ double sg[BLK][9];   // staging of the gradient rows (and of the force faces)
+    __shared__ double sl[BLK][LWS + 1];
+    static_assert(BLK * 9 >= 2 * FORCE_LDS, "force staging exceeds the gradient staging rows");
+    const uint32_t fw = a.force.n ? 1u : 0u;
+    if (fw && blockIdx.x == 0) {
+        pendingForce<BLK, FORCE_LDS>(a.force, a.sc, reinterpret_cast<double2*>(&sg[0][0]));
+        return;
+    }
     if (a.checkActive && !a.sc->active) return;
-    const uint32_t bid = xcdBlock(blockIdx.x, gridDim.x);
+    const uint32_t bid = xcdBlock(blockIdx.x - fw, gridDim.x - fw);
     if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
     const uint32_t i = bid * BLK + threadIdx.x;
     if (MODE == 0 && i >= m.N) return;
@@ -476,9 +557,6 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
     if (MODE == 0) return;
     // stage the rows in LDS and store the workgroup's contiguous run of cells with
     // lane-consecutive 16-byte stores
-    constexpr int LWS = MODE == 3 ? 8 : (MODE == 2 ? 4 : 0);
-    __shared__ double sg[BLK][9];
-    __shared__ double sl[BLK][LWS + 1];
     const uint32_t tid = threadIdx.x;
     if (have) {
 #pragma unroll
@@ -495,7 +573,7 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
     const uint32_t base = bid * BLK, n = (m.N - base) < BLK ? (m.N - base) : BLK;
     double2* gd = reinterpret_cast<double2*>(a.grad + size_t(base) * 8);
     for (uint32_t c = tid; c < 4 * n; c += BLK) __builtin_nontemporal_store((ntd2){sg[c / 4][2 * (c % 4)], sg[c / 4][2 * (c % 4) + 1]}, reinterpret_cast<ntd2*>(gd) + c);
-    if (LWS) {
+    if constexpr (LWS > 0) {
         double2* ld = reinterpret_cast<double2*>(a.lim + size_t(base) * LWS);
         constexpr uint32_t PR = LWS / 2;
         for (uint32_t c = tid; c < PR * n; c += BLK) __builtin_nontemporal_store((ntd2){sl[c / PR][2 * (c % PR)], sl[c / PR][2 * (c % PR) + 1]}, reinterpret_cast<ntd2*>(ld) + c);
@@ -1428,7 +1506,7 @@ DEV void rmsElement(const DiagArgs& d, uint64_t e, double out[8])
 }
 
 constexpr int RBLK = 1024;   // threads per chunk workgroup (element path)
-constexpr int FTILE = 2048;   // force faces staged per pass
+constexpr int FTILE = 2048;   // force faces staged per pass (RMS kernels)
 
 // Pairwise levels of one chunk: v[q][0..cnt) holds its 16-element block sums.
 // Level l's odd leftover is stored as tree l of the chunk (the binary counter's
@@ -1436,6 +1514,11 @@ constexpr int FTILE = 2048;   // force faces staged per pass
 template <int NT>
 DEV void chunkLevels(double (*v)[RMS_CHUNK_BLOCKS], uint32_t cnt, double* out, uint32_t* maskOut)
 {
+    // NT/8 threads per quantity; slot u of a thread is pair l + u * G (registers, no
+    // divisions by the level's length)
+    constexpr uint32_t G = NT / 8;
+    constexpr int PER = int((RMS_CHUNK_BLOCKS / 2 + G - 1) / G);
+    const uint32_t qq = threadIdx.x / G, l = threadIdx.x % G;
     uint32_t c = cnt;
     int level = 0;
     uint32_t mask = 0;
@@ -1445,17 +1528,17 @@ DEV void chunkLevels(double (*v)[RMS_CHUNK_BLOCKS], uint32_t cnt, double* out, u
             mask |= 1u << level;
         }
         const uint32_t half = c / 2;
-        double tmp[(8 * RMS_CHUNK_BLOCKS / 2 + NT - 1) / NT];
-        int n = 0;
-        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += NT) {
-            const uint32_t q = idx / half, j = idx % half;
-            tmp[n++] = v[q][2 * j] + v[q][2 * j + 1];
+        double tmp[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const uint32_t j = l + uint32_t(u) * G;
+            tmp[u] = j < half ? v[qq][2 * j] + v[qq][2 * j + 1] : 0.0;
         }
         __syncthreads();
-        n = 0;
-        for (uint32_t idx = threadIdx.x; idx < 8 * half; idx += NT) {
-            const uint32_t q = idx / half, j = idx % half;
-            v[q][j] = tmp[n++];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const uint32_t j = l + uint32_t(u) * G;
+            if (j < half) v[qq][j] = tmp[u];
         }
         __syncthreads();
         c = half;
@@ -1472,13 +1555,25 @@ DEV void chunkLevels(double (*v)[RMS_CHUNK_BLOCKS], uint32_t cnt, double* out, u
 template <int NT>
 DEV void stageReduce(const DiagArgs& d, double* lds)
 {
+    // each pass issues all of a thread's loads before using any (the partials were
+    // written on all XCDs: every load is a round trip past this XCD's L2)
+    constexpr int SLOTS = 24;
     double a0 = INFINITY, b0 = 0.0;
     for (uint32_t s = 0; s < d.nStages; s++) {
-        const double* pp = d.partial + size_t(s) * d.pstride;
+        const double2* pp = reinterpret_cast<const double2*>(d.partial + size_t(s) * d.pstride);
         const bool lastStage = s + 1 == d.nStages;
-        for (uint32_t k = threadIdx.x; k < d.nPart; k += NT) {
-            if (lastStage) a0 = fmin(a0, pp[2 * k]);
-            b0 = fmax(b0, pp[2 * k + 1]);
+        for (uint32_t base = 0; base < d.nPart; base += NT * SLOTS) {
+            double2 pv[SLOTS];
+#pragma unroll
+            for (int u = 0; u < SLOTS; u++) {
+                const uint32_t k = base + threadIdx.x + uint32_t(u) * NT;
+                pv[u] = k < d.nPart ? pp[k] : make_double2(INFINITY, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < SLOTS; u++) {
+                if (lastStage) a0 = fmin(a0, pv[u].x);
+                b0 = fmax(b0, pv[u].y);
+            }
         }
     }
     a0 = warpMin(a0);
@@ -1493,34 +1588,17 @@ DEV void stageReduce(const DiagArgs& d, double* lds)
         if (b0 > prev) d.sc->rmaxBits = (unsigned long long)__double_as_longlong(b0);
     }
 }
+// force workgroup of the RMS kernels: the step's force sums for k_finalize (unless
+// they are left to the next gradient kernel)
 template <int NT>
 DEV void forceSum(const DiagArgs& d, double* fb)
 {
-    double f0 = 0, f1 = 0;
-    double2* fp = reinterpret_cast<double2*>(fb);   // (x, y) pairs: one LDS read per face
-    for (uint32_t k0 = 0; k0 < d.nForce; k0 += FTILE) {
-        const uint32_t cnt = (d.nForce - k0) < uint32_t(FTILE) ? (d.nForce - k0) : uint32_t(FTILE);
-        __syncthreads();
-        for (uint32_t x = threadIdx.x; x < cnt; x += NT) {
-            const uint32_t f = d.forceFaces[k0 + x];
-            fp[x] = make_double2(d.lenF[size_t(f) * 4 + 1], d.lenF[size_t(f) * 4 + 2]);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            // the two ordered chains interleave; 32 LDS reads are in flight ahead of the adds
-            uint32_t x = 0;
-            for (; x + 32 <= cnt; x += 32) {
-                double2 v[32];
-#pragma unroll
-                for (int u = 0; u < 32; u++) v[u] = fp[x + u];
-#pragma unroll
-                for (int u = 0; u < 32; u++) { f0 += v[u].x; f1 += v[u].y; }
-            }
-            for (; x < cnt; x++) { const double2 v = fp[x]; f0 += v.x; f1 += v.y; }
-        }
-    }
+    if (d.deferForce) return;
+    double f0, f1;
+    orderedForce<NT, FTILE>(d.forceFaces, d.nForce, d.lenF, reinterpret_cast<double2*>(fb), f0, f1);
     if (threadIdx.x == 0) { d.sc->fsum[0] = f0; d.sc->fsum[1] = f1; }
 }
+
 
 // RMS sums from the states (element path): one workgroup per chunk of
 // RMS_CHUNK_BLOCKS 16-element blocks; 16 consecutive lanes own one block, and
@@ -1619,7 +1697,6 @@ constexpr int FIN_CHUNKS = 1024;     // chunk trees merged in LDS per quantity
 __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
 {
     DevScalars* sc = d.sc;
-    if (!sc->active) return;
     __shared__ double v[8][FIN_CHUNKS];
     __shared__ double trees[8][48];
     __shared__ int ntree[8];
@@ -1627,13 +1704,38 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
     const int tid = threadIdx.x;
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint64_t nfull = nb / RMS_CHUNK_BLOCKS, part = nb % RMS_CHUNK_BLOCKS;
+    // Every global read of the kernel is issued here, before the first is used (the
+    // active flag included): the previous kernels wrote this data on all XCDs, so each
+    // read is a round trip past this XCD's L2, and a chain of them set the kernel's length.
+    // FIN_T/8 threads per quantity; slot u of a thread is chunk l + u * G.
+    constexpr uint32_t G = FIN_T / 8;
+    constexpr int PER = int(FIN_CHUNKS / G), HPER = int(FIN_CHUNKS / 2 / G);
+    const uint32_t qq = uint32_t(tid) / G, l = uint32_t(tid) % G;
+    const uint32_t n = nfull <= FIN_CHUNKS ? uint32_t(nfull) : 0u;
+    double tv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const uint32_t k = l + uint32_t(u) * G;
+        tv[u] = k < n ? d.chunkTrees[size_t(k) * 128 + qq * 16 + 10] : 0.0;
+    }
+    uint32_t pmask = 0;   // the partial chunk's level trees (thread q < 8: quantity q)
+    double ptree[16];
+#pragma unroll
+    for (int b2 = 0; b2 < 16; b2++) ptree[b2] = 0.0;
+    if (tid < 8 && part) {
+        pmask = d.chunkMask[nfull * 8 + tid];
+#pragma unroll
+        for (int b2 = 0; b2 < 16; b2++) ptree[b2] = d.chunkTrees[nfull * 128 + tid * 16 + b2];
+    }
+    const DevScalars s0 = *sc;
+    if (!s0.active) return;
     if (tid < 8) ntree[tid] = 0;
     __syncthreads();
     if (nfull <= FIN_CHUNKS) {
-        const uint32_t n = uint32_t(nfull);
-        for (uint32_t x = tid; x < 8 * n; x += FIN_T) {
-            const uint32_t q = x / n, k = x % n;
-            v[q][k] = d.chunkTrees[size_t(k) * 128 + q * 16 + 10];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const uint32_t k = l + uint32_t(u) * G;
+            if (k < n) v[qq][k] = tv[u];
         }
         __syncthreads();
         uint32_t c = n;
@@ -1645,17 +1747,17 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
                 lmask |= 1u << level;
             }
             const uint32_t half = c / 2;
-            double tmp[(8 * FIN_CHUNKS / 2 + FIN_T - 1) / FIN_T];
-            int m = 0;
-            for (uint32_t x = tid; x < 8 * half; x += FIN_T) {
-                const uint32_t q = x / half, jj = x % half;
-                tmp[m++] = v[q][2 * jj] + v[q][2 * jj + 1];
+            double tmp[HPER];
+#pragma unroll
+            for (int u = 0; u < HPER; u++) {
+                const uint32_t j = l + uint32_t(u) * G;
+                tmp[u] = j < half ? v[qq][2 * j] + v[qq][2 * j + 1] : 0.0;
             }
             __syncthreads();
-            m = 0;
-            for (uint32_t x = tid; x < 8 * half; x += FIN_T) {
-                const uint32_t q = x / half, jj = x % half;
-                v[q][jj] = tmp[m++];
+#pragma unroll
+            for (int u = 0; u < HPER; u++) {
+                const uint32_t j = l + uint32_t(u) * G;
+                if (j < half) v[qq][j] = tmp[u];
             }
             __syncthreads();
             c = half;
@@ -1687,9 +1789,9 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
         int idx = ntree[q];
         double* st = trees[q];
         if (part) {
-            const uint32_t mask = d.chunkMask[nfull * 8 + q];
-            for (int l = 15; l >= 0; l--)
-                if (mask & (1u << l)) st[idx++] = d.chunkTrees[nfull * 128 + q * 16 + l];
+#pragma unroll
+            for (int b2 = 15; b2 >= 0; b2--)
+                if (pmask & (1u << b2)) st[idx++] = ptree[b2];
         }
         const uint64_t rem = uint64_t(d.L) - nb * 16;
         for (uint64_t i = 0; i < rem; i++) {
@@ -1709,33 +1811,47 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
     }
     __syncthreads();
     if (tid != 0) return;
-    const double f0 = sc->fsum[0], f1 = sc->fsum[1];   // summed by the RMS kernel's force workgroup
+    const double f0 = s0.fsum[0], f1 = s0.fsum[1];   // summed by the RMS kernel's force workgroup (unless deferred)
     double rms[4];
     for (int q = 0; q < 4; q++) rms[q] = sqrt(double(d.N) * tot[q]) / tot[4 + q];
-    const double newDt = __longlong_as_double((long long)sc->newDtBits);
-    const double Rmax = __longlong_as_double((long long)sc->rmaxBits);
-    if (d.dynamicDt) sc->dt = newDt;
+    const double newDt = __longlong_as_double((long long)s0.newDtBits);
+    const double Rmax = __longlong_as_double((long long)s0.rmaxBits);
+    const double dt = d.dynamicDt ? newDt : s0.dt;
     const double ld0 = 0.0 + d.rot[0] * f0 + d.rot[1] * f1;
     const double ld1 = 0.0 + d.rot[2] * f0 + d.rot[3] * f1;
     const double residMax = dmax(dmax(dmax(rms[0], rms[1]), rms[2]), rms[3]);
+    int64_t inc = s0.incIters;
+    double CFL = s0.CFL;
     if (d.cflAdjust) {
-        if (sc->residLast < residMax) sc->incIters++;
-        else if (sc->incIters >= 1) sc->incIters--;
-        if (sc->incIters >= 2000) { sc->CFL *= 0.99; sc->incIters = 0; }
+        if (s0.residLast < residMax) inc++;
+        else if (inc >= 1) inc--;
+        if (inc >= 2000) { CFL *= 0.99; inc = 0; }
     }
-    sc->residLast = residMax;
-    const int64_t slot = sc->steps - d.histBase;
+    const int64_t slot = s0.steps - d.histBase;
     if (slot >= 0 && slot < d.histCap) {
         double* h = d.hist + slot * HIST_FIELDS;
-        h[0] = sc->t; h[1] = sc->dt; h[2] = rms[0]; h[3] = rms[1]; h[4] = rms[2]; h[5] = rms[3];
-        h[6] = Rmax; h[7] = ld0; h[8] = ld1; h[9] = sc->CFL;
+        h[0] = s0.t; h[1] = dt; h[2] = rms[0]; h[3] = rms[1]; h[4] = rms[2]; h[5] = rms[3];
+        h[6] = Rmax; h[9] = CFL;
+        if (!d.deferForce) { h[7] = ld0; h[8] = ld1; }
     }
+    const double t1 = s0.t + dt;
+    const int64_t steps1 = s0.steps + 1;
+    sc->dt = dt;
+    sc->incIters = inc;
+    sc->CFL = CFL;
+    sc->residLast = residMax;
     for (int q = 0; q < 4; q++) sc->rms[q] = rms[q];
-    sc->force[0] = ld0; sc->force[1] = ld1;
-    sc->t += sc->dt;
-    sc->steps++;
-    if (!(sc->t < d.tEnd) || (d.maxSteps >= 0 && sc->steps >= d.maxSteps) ||
-        sc->errEdge != ~0ull || sc->errCell != ~0ull)
+    if (d.deferForce) {
+        // drag and lift of this step are summed by the next gradient kernel (or at the end of
+        // the batch), which writes them into this row and sc->force
+        sc->forcePending = 1;
+        sc->forceSlot = (slot >= 0 && slot < d.histCap) ? slot : -1;
+    } else {
+        sc->force[0] = ld0; sc->force[1] = ld1;
+    }
+    sc->t = t1;
+    sc->steps = steps1;
+    if (!(t1 < d.tEnd) || (d.maxSteps >= 0 && steps1 >= d.maxSteps) || s0.errEdge != ~0ull || s0.errCell != ~0ull)
         sc->active = 0;
 }
 
@@ -1755,6 +1871,10 @@ __global__ void k_init_scalars(DevScalars* sc, double dt, double t, double CFL)
     sc->pad = 0;
     for (int v = 0; v < 4; v++) sc->rms[v] = 0;
     sc->force[0] = sc->force[1] = 0;
+    sc->fsum[0] = sc->fsum[1] = 0;
+    sc->forcePending = 0;
+    sc->pad2 = 0;
+    sc->forceSlot = -1;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1763,7 +1883,7 @@ static uint32_t nblk(uint64_t n) { return uint32_t((n + BLK - 1) / BLK); }
 template <int S>
 static void gradS(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
 {
-    const uint32_t g = nblk(m.N);
+    const uint32_t g = nblk(m.N) + (a.force.n ? 1u : 0u);   // + the force workgroup
     if (g == 0) return;
     const int mode = p.order == 1 ? 0 : (!(p.limited && a.limit) ? 1 : (p.lpThresh > 0 ? 3 : 2));
     switch (mode) {
@@ -1845,6 +1965,17 @@ void launchRmsChunks(const DiagArgs& d, hipStream_t st)
     } else {
         hipLaunchKernelGGL(k_rms_chunks, dim3(nc + 2), dim3(RBLK), 0, st, d);
     }
+}
+
+__global__ __launch_bounds__(BLK) void k_pending_force(ForceArgs f, DevScalars* sc)
+{
+    __shared__ double2 fsh[FORCE_LDS];
+    pendingForce<BLK, FORCE_LDS>(f, sc, fsh);
+}
+
+void launchPendingForce(const ForceArgs& f, DevScalars* sc, hipStream_t st)
+{
+    if (f.n) hipLaunchKernelGGL(k_pending_force, dim3(1), dim3(BLK), 0, st, f, sc);
 }
 
 void launchFinalize(const DiagArgs& d, hipStream_t st)

@@ -207,6 +207,27 @@ hipError_t trackUpload(ebb_solver* s, DBuf<T>& b, const std::vector<T>& v)
     return e;
 }
 
+// Rotation of the boundary force into drag and lift (app/solver.d:178-193).
+void aoaRotation(const ebb_solver* s, double rot[4])
+{
+    const long double a = (long double)s->cfg.ic[1] * (3.14159265358979323846264338327950288L / 180.0L);
+    rot[0] = double(cosl(a)); rot[1] = -double(sinl(a)); rot[2] = double(sinl(a)); rot[3] = double(cosl(a));
+}
+
+// Paths whose stages start with the gradient kernel leave each step's force sum to the
+// next gradient kernel (the fused path sums it in the RMS kernel).
+bool deferForce(const ebb_solver* s) { return !s->useTiles && s->d_forceFaces.n > 0; }
+
+ForceArgs forceArgs(const ebb_solver* s)
+{
+    ForceArgs f{};
+    f.faces = s->d_forceFaces.p; f.n = uint32_t(s->d_forceFaces.n);
+    f.lenF = s->d_lenF.p;
+    aoaRotation(s, f.rot);
+    f.hist = s->d_hist.p;
+    return f;
+}
+
 // one residual evaluation + stage update
 void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qout, int limit, int dtUpdate,
            int first, int last, int checkActive, double4* R, double4* estate, uint32_t stepId)
@@ -224,6 +245,8 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
         a.deferReduce = 1;
         a.partial = s->d_partial.p + size_t(s->stageInStep) * s->pstride;
         s->stageInStep++;
+        // the gradient kernels of the time loop also sum the previous step's force
+        if (deferForce(s)) a.force = forceArgs(s);
     }
     if (s->useTiles && !estate && kind != SK_RESIDUAL) {
         launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
@@ -285,13 +308,13 @@ DiagArgs diagArgs(ebb_solver* s, const double4* qnew, const double4* qold, int64
     d.chunkTrees = s->d_chunkTrees.p; d.chunkMask = s->d_chunkMask.p;
     d.forceFaces = s->d_forceFaces.p; d.nForce = uint32_t(s->d_forceFaces.n);
     d.lenF = s->d_lenF.p;
-    const long double a = (long double)s->cfg.ic[1] * (3.14159265358979323846264338327950288L / 180.0L);
-    d.rot[0] = double(cosl(a)); d.rot[1] = -double(sinl(a)); d.rot[2] = double(sinl(a)); d.rot[3] = double(cosl(a));
+    aoaRotation(s, d.rot);
     d.tEnd = s->cfg.tEnd;
     d.dynamicDt = s->cfg.dynamicDt; d.cflAdjust = s->cfg.cflAdjust;
     d.hist = s->d_hist.p; d.histBase = histBase; d.histCap = s->histCap;
     d.maxSteps = maxSteps;
     d.partial = s->d_partial.p; d.nPart = s->nPart; d.pstride = s->pstride; d.nStages = s->stageInStep;
+    d.deferForce = deferForce(s) ? 1 : 0;
     d.sc = s->d_sc;
     return d;
 }
@@ -1019,6 +1042,8 @@ int ebb_run(ebb_solver* s, int64_t max_steps, double* t, double* dt, int64_t* st
                 launchFinalize(d, s->stream);
                 s->cur ^= 1;
             }
+            // the batch's last step (or the one that ended the loop) still has its force pending
+            if (deferForce(s)) launchPendingForce(forceArgs(s), s->d_sc, s->stream);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(s->h_hist, s->d_hist.p, sizeof(double) * size_t(batch) * HIST_FIELDS,
                                   hipMemcpyDeviceToHost, s->stream));
