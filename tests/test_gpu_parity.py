@@ -129,6 +129,19 @@ def test_run_in_batches_equals_one_run():
     np.testing.assert_array_equal(s1.q, s2.q)
 
 
+def test_run_across_history_batches_matches_oracle():
+    """One run longer than a history batch (256 steps): the batch boundary inside the run
+    sums the last step's boundary force before the host copies the rows."""
+    raw, cfg = channel_case(integrator="RK2_TVD", viscosity=True, walls="viscidWall")
+    om, s = _pair(raw, cfg)
+    hist_o, t_o, dt_o = om.run(300)
+    hist_g, t_g, dt_g = s.run(300)
+    assert len(hist_g) == 300
+    np.testing.assert_array_equal(hist_g, hist_o)
+    np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
+    assert t_g == t_o and dt_g == dt_o
+
+
 def test_stops_at_tend():
     raw, cfg = channel_case(integrator="Euler", dynamicDt=False, dt=1.0e-3, tEnd=0.0105)
     om, s = _pair(raw, cfg)
@@ -137,6 +150,9 @@ def test_stops_at_tend():
     assert len(hist_g) == len(hist_o) == 11
     assert t_g == t_o
     np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
+    # every row, drag and lift included: the loop stops mid-batch, and the last step's
+    # force is summed after the loop has stopped
+    np.testing.assert_array_equal(hist_g, hist_o)
 
 
 def test_nan_edge_reported_like_reference():
