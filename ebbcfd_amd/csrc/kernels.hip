@@ -1771,8 +1771,9 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
             ntree[tid] = t;
         }
     } else if (tid < 8) {
-        // very large meshes: replay the binary counter over the chunk trees directly
-        double st[48];
+        // very large meshes: replay the binary counter over the chunk trees directly, on
+        // the quantity's LDS stack (no scratch)
+        double* st = trees[tid];
         int idx = 0;
         for (uint64_t k = 0; k < nfull; k++) {
             st[idx] = d.chunkTrees[k * 128 + tid * 16 + 10];
@@ -1780,7 +1781,6 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
             while (idx > last) { st[idx - 1] += st[idx]; --idx; }
             ++idx;
         }
-        for (int t = 0; t < idx; t++) trees[tid][t] = st[t];
         ntree[tid] = idx;
     }
     __syncthreads();
@@ -1797,7 +1797,10 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
         for (uint64_t i = 0; i < rem; i++) {
             double o[8];
             rmsElement(d, nb * 16 + i, o);
-            st[idx] = o[q];
+            double oq = o[0];   // o[q] by selects, so o stays in registers
+#pragma unroll
+            for (int k = 1; k < 8; k++) oq = q == k ? o[k] : oq;
+            st[idx] = oq;
             const int last = idx - __builtin_ctzll(i + 1);
             while (idx > last) { st[idx - 1] += st[idx]; --idx; }
             ++idx;
