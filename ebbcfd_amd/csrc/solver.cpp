@@ -380,7 +380,20 @@ int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& st
     if (want != -1 && (want < 1 || want > 1024)) return fail(s, EBB_ERR_ARG, "tileCells must be -1 or in 1..1024");
     std::vector<uint32_t> order;
     uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 ? 768 : 128);
-    if (c.stagePath == 2 && want <= 0) {
+    if (c.stagePath == 1 && want <= 0) {
+        // automatic face tiles: the largest multiple of 16 up to 768 cells whose busiest
+        // tile keeps its face results (40 B per face) within a third of the CU's 160 KB
+        // of LDS, so three workgroups stay resident (768 fits triangles; quads and larger
+        // polygons have more faces per cell)
+        constexpr uint32_t LDS_TARGET = 53248;
+        for (;;) {
+            tileOrder(ref, T, c.tileOrder, order, start);
+            const uint32_t need = 40 * (maxTileFaces(ref, order, start) + 1);
+            if (need <= LDS_TARGET || T <= 16) break;
+            const uint32_t next = uint32_t(uint64_t(T) * LDS_TARGET / need) / 16 * 16;
+            T = std::max<uint32_t>(16, std::min(next, T - 16));
+        }
+    } else if (c.stagePath == 2 && want <= 0) {
         const int lp = (c.order > 1 && c.limited && c.lpThresh > 0) ? 1 : 0;
         for (;; T -= 16) {
             tileOrder(ref, T, c.tileOrder, order, start);

@@ -250,6 +250,26 @@ void tileOrder(const Layout& L, uint32_t T, int kind, std::vector<uint32_t>& ord
     else growTiles(L, T, order, start);
 }
 
+uint32_t maxTileFaces(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start)
+{
+    const uint32_t S = L.S, F = L.Fi + L.Fb;
+    if (start.size() < 2) return 0;
+    std::vector<uint32_t> fstamp(F, ~0u);
+    uint32_t best = 0;
+    for (uint32_t t = 0; t + 1 < start.size(); t++) {
+        uint32_t nF = 0;
+        for (uint32_t k = start[t]; k < start[t + 1]; k++)
+            for (uint32_t j = 0; j < S; j++) {
+                const uint32_t f = L.cface[size_t(order[k]) * S + j] & 0x7fffffffu;
+                if (f == F || fstamp[f] == t) continue;
+                fstamp[f] = t;
+                nF++;
+            }
+        best = std::max(best, nF);
+    }
+    return best;
+}
+
 bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
                    FaceTileHost& h, std::string& err)
 {

@@ -54,6 +54,10 @@ struct FaceTileHost {
 // block last): tile t is cells order[start[t] .. start[t+1]).
 void tileOrder(const Layout& L, uint32_t T, int kind, std::vector<uint32_t>& order, std::vector<uint32_t>& start);
 
+// Largest number of distinct faces of one tile (the face tile kernel's LDS holds 40 bytes
+// per face).
+uint32_t maxTileFaces(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start);
+
 // Face tiles: each cell group with the faces of its cells.
 bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
                    FaceTileHost& h, std::string& err);

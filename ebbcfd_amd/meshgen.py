@@ -13,7 +13,8 @@ Workloads:
                            seeded random diagonal; tags "Wall" and "Farfield";
   * ``channel_bump``    -- channel [0,3]x[0,1] with a sin^2 bump of height h on
                            the lower wall between x=1 and x=2; tags "Inflow",
-                           "Outflow", "Lower", "Upper".
+                           "Outflow", "Lower", "Upper" (``channel_bump_quads``: the
+                           same domain in quadrilaterals).
 Random numbers come from numpy's PCG64 seeded with ``seed``.
 """
 from dataclasses import dataclass, field
@@ -128,6 +129,32 @@ def channel_bump(nx=96, ny=32, height=0.05, seed=1, jitter=0.1):
     upper = np.stack([idx[-1, 1:], idx[-1, :-1]], axis=1)
     inflow = np.stack([idx[1:, 0], idx[:-1, 0]], axis=1)
     return _tri_mesh(nodes, tris, [inflow, outflow, lower, upper], ["Inflow", "Outflow", "Lower", "Upper"])
+
+
+def channel_bump_quads(nx=96, ny=32, height=0.05, seed=1, jitter=0.1):
+    """The channel_bump domain as nx*ny quadrilaterals (no diagonal split)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs = np.linspace(0.0, 3.0, nx + 1)
+    s = np.linspace(0.0, 1.0, ny + 1)
+    X, S = np.meshgrid(xs, s)
+    yl = np.where((X >= 1.0) & (X <= 2.0), height * np.sin(np.pi * (X - 1.0)) ** 2, 0.0)
+    Y = yl + S * (1.0 - yl)
+    if jitter > 0:
+        hx, hy = 3.0 / nx, 1.0 / ny
+        X[1:-1, 1:-1] += rng.uniform(-jitter * hx, jitter * hx, size=(ny - 1, nx - 1)) * (1.0 - yl[1:-1, 1:-1])
+        Y[1:-1, 1:-1] += rng.uniform(-jitter * hy, jitter * hy, size=(ny - 1, nx - 1)) * (1.0 - yl[1:-1, 1:-1])
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+    idx = np.arange((nx + 1) * (ny + 1)).reshape(ny + 1, nx + 1)
+    quads = np.stack([idx[:-1, :-1], idx[:-1, 1:], idx[1:, 1:], idx[1:, :-1]], axis=-1).reshape(-1, 4)  # counter-clockwise
+    groups = [np.stack([idx[1:, 0], idx[:-1, 0]], axis=1), np.stack([idx[:-1, -1], idx[1:, -1]], axis=1),
+              np.stack([idx[0, :-1], idx[0, 1:]], axis=1), np.stack([idx[-1, 1:], idx[-1, :-1]], axis=1)]
+    ne = quads.shape[0]
+    starts = np.cumsum([0] + [g.shape[0] for g in groups[:-1]]).astype(np.uint64)
+    return RawMesh(nodes=np.ascontiguousarray(nodes, dtype=np.float64),
+                   elem_offsets=(np.arange(ne + 1, dtype=np.uint64) * 4).astype(np.uint32),
+                   elem_nodes=(quads.astype(np.uint32) + 1).reshape(-1),
+                   bnodes=np.concatenate(groups).astype(np.uint32), btags=["Inflow", "Outflow", "Lower", "Upper"],
+                   bgroup_start=starts)
 
 
 def channel_bump_cells(target_cells, aspect=3.0):

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from ebbcfd_amd import Solver, build_mesh
+from ebbcfd_amd import Solver, build_mesh, meshgen
 
 from helpers import channel_case, cylinder_case, mms_case, oracle_cfg, reference_fixture
 
@@ -32,6 +32,8 @@ CASES = {
     "mms_viscous_rk4": lambda: mms_case(viscosity=True),
     "mms_fixture_tri_lp": lambda: reference_fixture("ms_refine_tri_pert_0"),
     "mms_fixture_quad_lp": lambda: reference_fixture("ms_refine_1"),
+    "quads_rk2tvd_visc": lambda: (meshgen.channel_bump_quads(24, 8),
+                                  channel_case(integrator="RK2_TVD", viscosity=True)[1]),
 }
 
 
@@ -141,3 +143,23 @@ def test_renumbered_interfaces_match_split(path, tmp_path):
     tiled2 = Solver(build_mesh(raw), with_path(cfg, path, 17, "morton", "tiles"))
     tiled2.load_solution(str(tmp_path / "t.esln"))
     np.testing.assert_array_equal(tiled2.q, tiled.q)
+
+
+def test_quad_mesh_takes_face_tiles_by_default():
+    """Quads have ~2.3 faces per cell: the automatic tile size shrinks until a tile's face
+    results fit the LDS of three resident workgroups, instead of falling back to the split
+    kernels, and the results stay those of the split kernels and the oracle."""
+    _, cfg = channel_case(integrator="RK2_TVD", viscosity=True)
+    raw = meshgen.channel_bump_quads(200, 60)
+    tiled = Solver(build_mesh(raw), cfg)
+    split = Solver(build_mesh(raw), with_path(cfg, "split"))
+    h_t, _, _ = tiled.run(10)
+    h_s, _, _ = split.run(10)
+    np.testing.assert_array_equal(h_t, h_s)
+    np.testing.assert_array_equal(tiled.q, split.q)
+    om = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    h_o, _, _ = om.run(10)
+    np.testing.assert_array_equal(h_t, h_o)
+    np.testing.assert_array_equal(tiled.q, om.q[: om.n_interior])
+    tt = tiled.time_steps(1)
+    assert tt["fused_ms"] > 0 and tt["faces_ms"] == 0
