@@ -9,6 +9,7 @@ from ebbcfd_amd import meshgen
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MMS_DIR = os.path.join(ROOT, "tests", "golden", "ms_refinement")
+FLAT_DIR = os.path.join(ROOT, "tests", "golden", "flatPlate")
 
 
 def base_config(**over):
@@ -68,6 +69,15 @@ def reference_fixture(name):
     raw = parse_gmsh(os.path.join(MMS_DIR, name + ".msh"))
     cfg = json.load(open(os.path.join(MMS_DIR, name + ".json")))
     cfg["extensions"] = {"mms": True}
+    return raw, cfg
+
+
+def flat_plate_fixture(k=0):
+    """The reference's viscous flat-plate case (flatPlate/flatPlate_<k>.msh + .json): quads,
+    Roe flux, Euler steps, fullState / viscidWall / constP / symmetry boundaries."""
+    from oracle.gmsh import parse_gmsh
+    raw = parse_gmsh(os.path.join(FLAT_DIR, "flatPlate_%d.msh" % k))
+    cfg = json.load(open(os.path.join(FLAT_DIR, "flatPlate_%d.json" % k)))
     return raw, cfg
 
 

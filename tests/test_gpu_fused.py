@@ -15,7 +15,7 @@ import pytest
 import oracle
 from ebbcfd_amd import Solver, build_mesh, meshgen
 
-from helpers import channel_case, cylinder_case, mms_case, oracle_cfg, reference_fixture
+from helpers import channel_case, cylinder_case, flat_plate_fixture, mms_case, oracle_cfg, reference_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -32,6 +32,7 @@ CASES = {
     "mms_viscous_rk4": lambda: mms_case(viscosity=True),
     "mms_fixture_tri_lp": lambda: reference_fixture("ms_refine_tri_pert_0"),
     "mms_fixture_quad_lp": lambda: reference_fixture("ms_refine_1"),
+    "flat_plate_fixture": lambda: flat_plate_fixture(0),
     "quads_rk2tvd_visc": lambda: (meshgen.channel_bump_quads(24, 8),
                                   channel_case(integrator="RK2_TVD", viscosity=True)[1]),
 }

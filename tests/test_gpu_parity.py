@@ -13,7 +13,7 @@ import pytest
 import oracle
 from ebbcfd_amd import Solver, build_mesh
 
-from helpers import channel_case, cylinder_case, mms_case, oracle_cfg, perturb, reference_fixture
+from helpers import channel_case, cylinder_case, flat_plate_fixture, mms_case, oracle_cfg, perturb, reference_fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +45,7 @@ RESIDUAL_CASES = {
     "mms_dirichlet_viscous": lambda: mms_case(viscosity=True),
     "mms_fixture_lp": lambda: reference_fixture("ms_refine_tri_pert_1"),
     "mms_fixture_quads_lp": lambda: reference_fixture("ms_refine_1"),
+    "flat_plate_fixture": lambda: flat_plate_fixture(0),
 }
 
 
@@ -87,6 +88,8 @@ STEP_CASES = {
     "cyl_ns_rk2tvd": lambda: cylinder_case(),
     "mms_rk4": lambda: mms_case(),
     "mms_fixture_euler_lp": lambda: reference_fixture("ms_refine_tri_pert_0"),
+    "flat_plate_fixture_0": lambda: flat_plate_fixture(0),
+    "flat_plate_fixture_1": lambda: flat_plate_fixture(1),
 }
 
 

@@ -15,7 +15,7 @@ from oracle.gmsh import parse_gmsh
 from ebbcfd_amd import _lib, build_mesh, import_mesh, load_config, meshgen
 from ebbcfd_amd.solver import Config
 
-from helpers import MMS_DIR, ROOT
+from helpers import FLAT_DIR, MMS_DIR, ROOT
 
 
 def test_library_exports_every_declared_symbol():
@@ -41,7 +41,8 @@ def _compare_geometry(mesh, om):
         np.testing.assert_array_equal(e[k], oe[k], err_msg=k)
 
 
-@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(MMS_DIR, "*.msh"))), ids=os.path.basename)
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(MMS_DIR, "*.msh")) + glob.glob(os.path.join(FLAT_DIR, "*.msh"))),
+                         ids=os.path.basename)
 def test_gmsh_reader_and_geometry_match_oracle(path):
     m = import_mesh(path)
     om = oracle.Mesh(parse_gmsh(path))
