@@ -46,6 +46,8 @@ struct DevScalars {
     int32_t forcePending;           // 1: the last step's boundary force is still to be summed
     int32_t pad2;
     int64_t forceSlot;              // its history row (index into the history buffer), -1: none
+    unsigned long long errBC;       // (stage << 32) | boundary face whose total-conditions inflow has no
+                                    // non-negative Mach root (solve.d:648-671), min wins; ~0 = none
 };
 
 // The ordered boundary-force sum of a time-loop step (mesh.d:645-671), when it is

@@ -191,7 +191,9 @@ DEV void st4(double4* p, uint32_t i, const double o[4]) { p[i] = make_double4(o[
 typedef double ntd2 __attribute__((ext_vector_type(2)));
 
 // ghost state from the owner state (solve.d:520-689); the owner is the only writer
-DEV void ghostState(const BFace& b, const double qo[4], double qg[4], const double qgOld[4])
+// false where the reference would stop: a total-conditions inflow with no non-negative Mach
+// root (solve.d:661-670)
+DEV bool ghostState(const BFace& b, const double qo[4], double qg[4], const double qgOld[4])
 {
     switch (b.type) {
     case BT_DIRICHLET:
@@ -250,9 +252,11 @@ DEV void ghostState(const BFace& b, const double qo[4], double qg[4], const doub
         const double M1 = (-B + sqrt(B * B - 4 * A * Cc)) / (2.0 * A);
         const double M2 = (-B - sqrt(B * B - 4 * A * Cc)) / (2.0 * A);
         double Mb = NAN;
+        bool rootFound = true;
         if (M1 >= 0 && M2 >= 0) Mb = fmin(M1, M2);
         else if (M1 >= 0 && M2 < 0) Mb = M1;
         else if (M1 < 0 && M2 >= 0) Mb = M2;
+        else rootFound = false;   // the reference stops here (enforce)
         const double Tb = Tt / (1 + 0.5 * (G_ - 1) * (Mb * Mb));
         const double pb = Pt * pow(Tb / Tt, G_ / (G_ - 1));
         const double rhob = pb / (Rg * Tb);
@@ -261,12 +265,13 @@ DEV void ghostState(const BFace& b, const double qo[4], double qg[4], const doub
         const double mag = sqrt(vb0 * vb0 + vb1 * vb1);
         qg[0] = pb / (G_ - 1) + 0.5 * rhob * (mag * mag);
         qg[1] = qgOld[1]; qg[2] = qgOld[2]; qg[3] = qgOld[3];
-        break;
+        return rootFound;
     }
     default:  // FullState
         qg[0] = qo[0]; qg[1] = qo[1]; qg[2] = qo[2]; qg[3] = qo[3];
         break;
     }
+    return true;
 }
 
 // solve.d:27-173 LP vector limiter (all-inequality simplex, at most 16 pivots)
@@ -432,11 +437,11 @@ DEV void gradLimit(const double qi[4], const double qn[S][4], const double* gm, 
 }
 
 // the owner's ghost state for boundary record b, with TempPresInflow's untouched components
-DEV void ghostFromOwner(const BFace& bf, const double qo[4], const double4* qbuf, uint32_t gIdx, double qg[4])
+DEV bool ghostFromOwner(const BFace& bf, const double qo[4], const double4* qbuf, uint32_t gIdx, double qg[4])
 {
     double old[4] = {0, 0, 0, 0};
     if (bf.type == BT_TPIN) ld4(qbuf, gIdx, old);
-    ghostState(bf, qo, qg, old);
+    return ghostState(bf, qo, qg, old);
 }
 
 // The boundary force sum of mesh.d:645-671: (len*flux) momentum components of the
@@ -541,7 +546,8 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
         for (int j = 0; j < S; j++) {
             nb[j] = m.nbr[i * S + j];
             if (nb[j] >= m.N) {
-                ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qn[j]);
+                if (!ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qn[j]) && a.recordErrors)
+                    atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
                 st4(a.qin, nb[j], qn[j]);
             } else {
                 ld4(a.qin, nb[j], qn[j]);
@@ -1096,7 +1102,8 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
             for (uint32_t k = nQ + tid; k < nL; k += TBLK) {
                 const uint32_t g = ids[k], own = gown[k - nQ];
                 double qg[4];
-                ghostFromOwner(m.bface[g - m.N], qs + 4 * own, a.qin, g, qg);
+                if (!ghostFromOwner(m.bface[g - m.N], qs + 4 * own, a.qin, g, qg) && a.recordErrors)
+                    atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (g - m.N));
 #pragma unroll
                 for (int c = 0; c < 4; c++) qs[4 * k + c] = qg[c];
                 if (own < nC) st4(a.qin, g, qg);   // one writer per ghost: the tile owning its cell
@@ -1854,7 +1861,8 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
     }
     sc->t = t1;
     sc->steps = steps1;
-    if (!(t1 < d.tEnd) || (d.maxSteps >= 0 && steps1 >= d.maxSteps) || s0.errEdge != ~0ull || s0.errCell != ~0ull)
+    if (!(t1 < d.tEnd) || (d.maxSteps >= 0 && steps1 >= d.maxSteps) || s0.errEdge != ~0ull || s0.errCell != ~0ull ||
+        s0.errBC != ~0ull)
         sc->active = 0;
 }
 
@@ -1864,6 +1872,7 @@ __global__ void k_init_scalars(DevScalars* sc, double dt, double t, double CFL)
     sc->rmaxBits = 0;
     sc->errEdge = ~0ull;
     sc->errCell = ~0ull;
+    sc->errBC = ~0ull;
     sc->dt = dt;
     sc->t = t;
     sc->CFL = CFL;

@@ -329,6 +329,15 @@ int readScalars(ebb_solver* s)
 int checkErrors(ebb_solver* s)
 {
     const DevScalars& h = *s->h_sc;
+    // the reference updates the ghosts before its edge loop and checks the cells after the
+    // step: a boundary failure comes first unless a NaN edge came in an earlier stage
+    if (h.errBC != ~0ull && (h.errEdge == ~0ull || (h.errBC >> 32) <= (h.errEdge >> 32))) {
+        const uint64_t key = h.errBC & 0xFFFFFFFFull;
+        s->errKind = EBB_ERR_BC;
+        s->errIndex = int64_t(s->b.boundaryEdges[key]);
+        s->errMsg = "Both computed mach numbers are negative";
+        return EBB_ERR_BC;
+    }
     if (h.errEdge != ~0ull) {
         const uint64_t key = h.errEdge & 0xFFFFFFFFull;
         int64_t edge = key < s->Fb ? int64_t(s->b.boundaryEdges[key]) : int64_t(s->b.interiorEdges[key - s->Fb]);
@@ -354,6 +363,7 @@ int clearErrors(ebb_solver* s)
     HIPCHK(hipStreamSynchronize(s->stream));
     s->h_sc->errEdge = ~0ull;
     s->h_sc->errCell = ~0ull;
+    s->h_sc->errBC = ~0ull;
     HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return EBB_OK;
@@ -981,6 +991,7 @@ int ebb_residual(ebb_solver* s, int limit, int dt_update, double* R, double* new
     if (rmax && *rmax > 0) std::memcpy(&s->h_sc->rmaxBits, rmax, 8);
     s->h_sc->errEdge = ~0ull;
     s->h_sc->errCell = ~0ull;
+    s->h_sc->errBC = ~0ull;
     HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
     stage(s, SK_RESIDUAL, s->qcur(), s->qcur(), nullptr, limit, dt_update, 0, 0, 0, s->d_R.p, s->d_estate.p, 0);
     HIPCHK(hipGetLastError());
@@ -1003,6 +1014,7 @@ int ebb_step(ebb_solver* s, double* dt, double* rmax)
     s->h_sc->dt = *dt;
     s->h_sc->errEdge = ~0ull;
     s->h_sc->errCell = ~0ull;
+    s->h_sc->errBC = ~0ull;
     HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
     stepStages(s, 0, uint32_t(s->stepsTotal));
     HIPCHK(hipGetLastError());
@@ -1028,7 +1040,7 @@ int ebb_run(ebb_solver* s, int64_t max_steps, double* t, double* dt, int64_t* st
     { int rc_ = readScalars(s); if (rc_) return rc_; }
     int64_t done = 0;
     int rc = EBB_OK;
-    if (s->h_sc->errEdge != ~0ull || s->h_sc->errCell != ~0ull) return fail(s, EBB_ERR_STATE, "solver stopped on an error");
+    if (s->h_sc->errEdge != ~0ull || s->h_sc->errCell != ~0ull || s->h_sc->errBC != ~0ull) return fail(s, EBB_ERR_STATE, "solver stopped on an error");
     if (!(s->h_sc->t < s->cfg.tEnd)) goto out;
     {
         // (re)activate: the device stops itself at tEnd, after maxSteps or on an error

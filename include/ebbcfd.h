@@ -29,7 +29,8 @@ enum {
     EBB_ERR_IO = 2,         /* file could not be read or written */
     EBB_ERR_PARSE = 3,      /* malformed mesh or config */
     EBB_ERR_MESH = 4,       /* mesh topology/geometry rejected (e.g. negative area) */
-    EBB_ERR_BC = 5,         /* missing or inconsistent boundary condition */
+    EBB_ERR_BC = 5,         /* missing or inconsistent boundary condition, or a tpInflow boundary
+                               with no non-negative Mach root (source/solve.d:661-670) */
     EBB_ERR_DEVICE = 6,     /* HIP runtime error or no device */
     EBB_ERR_EDGE_NAN = 7,   /* NaN flux on an edge (EdgeException, source/solve.d:806) */
     EBB_ERR_CELL_NAN = 8,   /* NaN state in a cell (CellException, app/solver.d:148) */

@@ -180,3 +180,51 @@ def test_constant_pressure_outflow_matches_oracle():
     hg, _, _ = s.run(20)
     qo = om.q[: om.n_interior]
     assert np.all(np.abs(s.q - qo) <= 1e-11 * np.abs(qo).max(axis=0))
+
+
+def _total_conditions_inflow_case(aoa, tt_factor):
+    """Channel with a total-temperature/total-pressure inflow (tpInflow, solve.d:625-683) built
+    from the M=0.5 freestream; tt_factor scales the total temperature."""
+    raw = meshgen.channel_bump(24, 8, 0.05, seed=3)
+    M = 0.5
+    q = [M, 0.0, 1.0, 1.0]
+    p = 1.0 / (1.4 * M * M)
+    T = p / 8.3145
+    Tt = T * (1 + 0.2 * M * M) * tt_factor
+    Pt = p * (1 + 0.2 * M * M) ** 3.5
+    bcs = [{"tag": "Inflow", "type": "tpInflow", "q": [Tt, Pt, aoa]}, {"tag": "Outflow", "type": "fullState", "q": q},
+           {"tag": "Lower", "type": "inviscidWall", "q": q}, {"tag": "Upper", "type": "inviscidWall", "q": q}]
+    return raw, base_config(initialConditions=q, boudaryConditions=bcs, forceBoundary="Lower", integrator="RK2")
+
+
+def test_total_conditions_inflow_matches_oracle():
+    """The ghost keeps its old momentum and energy and gets only component 0 (the reference
+    writes q[g][0] four times); pow() in the ghost state allows an ulp or so."""
+    raw, cfg = _total_conditions_inflow_case(180.0, 1.2)
+    om = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    s = Solver(build_mesh(raw), cfg)
+    R_o, nd_o, _ = om.residual()
+    R_g, nd_g, _ = s.ufvm_solver()
+    np.testing.assert_allclose(R_g, R_o, rtol=1e-12, atol=1e-12 * np.abs(R_o).max())
+    np.testing.assert_allclose(s.q_all(), om.q, rtol=1e-13, atol=0)
+    ho, _, _ = om.run(20)
+    hg, _, _ = s.run(20)
+    qo = om.q[: om.n_interior]
+    assert np.all(np.abs(s.q - qo) <= 1e-11 * np.abs(qo).max(axis=0))
+
+
+def test_total_conditions_inflow_without_root_stops_like_reference():
+    """No non-negative Mach root: the reference stops (enforce, solve.d:661-670); the device
+    reports EBB_ERR_BC with the same reference edge, from the residual and from the loop."""
+    raw, cfg = _total_conditions_inflow_case(0.0, 1.0)
+    om = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    with pytest.raises(oracle.OracleError) as eo:
+        om.residual()
+    s = Solver(build_mesh(raw), cfg)
+    with pytest.raises(Exception) as eg:
+        s.ufvm_solver()
+    assert eg.value.code == 5 and eg.value.index == eo.value.index
+    s2 = Solver(build_mesh(raw), cfg)
+    with pytest.raises(Exception) as er:
+        s2.run(5)
+    assert er.value.code == 5 and er.value.index == eo.value.index
