@@ -81,6 +81,23 @@ def flat_plate_fixture(k=0):
     return raw, cfg
 
 
+def tiny_mesh(kind):
+    """Meshes whose every cell is a boundary cell: one triangle, two triangles, one quad;
+    one far-field tag."""
+    if kind == "tri1":
+        nodes, elems = [[0, 0], [1, 0], [0, 1]], [[1, 2, 3]]
+    elif kind == "tri2":
+        nodes, elems = [[0, 0], [1, 0], [1, 1], [0, 1]], [[1, 2, 3], [1, 3, 4]]
+    else:
+        nodes, elems = [[0, 0], [1, 0], [1, 1], [0, 1]], [[1, 2, 3, 4]]
+    nn = len(nodes)
+    offs = np.cumsum([0] + [len(e) for e in elems]).astype(np.uint32)
+    bnodes = np.array([[k, (k + 1) % nn] for k in range(nn)], np.uint32)
+    return meshgen.RawMesh(nodes=np.array(nodes, float), elem_offsets=offs,
+                           elem_nodes=np.array([v for e in elems for v in e], np.uint32), bnodes=bnodes,
+                           btags=["Farfield"], bgroup_start=np.zeros(1, np.uint64))
+
+
 def perturb(q, scale=0.02, seed=0):
     rng = np.random.Generator(np.random.PCG64(seed))
     return q * (1.0 + scale * rng.uniform(-1, 1, size=q.shape))

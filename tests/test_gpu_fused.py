@@ -15,7 +15,8 @@ import pytest
 import oracle
 from ebbcfd_amd import Solver, build_mesh, meshgen
 
-from helpers import channel_case, cylinder_case, flat_plate_fixture, mms_case, oracle_cfg, reference_fixture
+from helpers import (base_config, channel_case, cylinder_case, flat_plate_fixture, mms_case, oracle_cfg, perturb,
+                     reference_fixture, tiny_mesh)
 
 pytestmark = pytest.mark.gpu
 
@@ -164,3 +165,26 @@ def test_quad_mesh_takes_face_tiles_by_default():
     np.testing.assert_array_equal(tiled.q, om.q[: om.n_interior])
     tt = tiled.time_steps(1)
     assert tt["fused_ms"] > 0 and tt["faces_ms"] == 0
+
+
+@pytest.mark.parametrize("path", ["split", "faceTiles", "fused"])
+@pytest.mark.parametrize("kind", ["tri1", "tri2", "quad1"])
+def test_tiny_meshes_match_oracle(kind, path):
+    """One or two cells, all on the boundary: a tile shorter than a 16-cell block and an
+    RMS sum made of the element tail only."""
+    raw = tiny_mesh(kind)
+    q = [0.3, 10.0, 1.0, 1.0]
+    cfg = with_path(base_config(initialConditions=q, boudaryConditions=[{"tag": "Farfield", "type": "fullState", "q": q}],
+                                forceBoundary="Farfield", integrator="RK2_TVD"), path)
+    om = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    s = Solver(build_mesh(raw), cfg)
+    qp = perturb(om.q[: om.n_interior], 0.05, seed=1)
+    qq = om.q
+    qq[: om.n_interior] = qp
+    om.q = qq
+    s.q = qp
+    h_o, t_o, _ = om.run(5)
+    h_g, t_g, _ = s.run(5)
+    np.testing.assert_array_equal(h_g, h_o)
+    np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
+    assert t_g == t_o
