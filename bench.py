@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n-theta", type=int, default=2048)
     ap.add_argument("--n-r", type=int, default=1024)
+    ap.add_argument("--stretch", type=float, default=None,
+                    help="radial growth ratio (default: the generator's, 1.006 at 512+ radial cells)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--time-steps", type=int, default=20, help="steps timed per kernel with HIP events")
     ap.add_argument("--stage-path", default="faceTiles", choices=("split", "faceTiles", "fused"),
@@ -177,7 +179,8 @@ def main():
     from ebbcfd_amd import Solver, build_mesh, meshgen
 
     log("generating O-grid %d x %d" % (args.n_theta, args.n_r))
-    raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1)
+    kw = {} if args.stretch is None else {"stretch": args.stretch}
+    raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1, **kw)
     cfg = cylinder_config()
     cfg["extensions"] = {"stagePath": args.stage_path, "tileCells": args.tile_cells, "tileOrder": args.tile_order,
                          "cellOrder": args.cell_order}
@@ -217,6 +220,8 @@ def main():
         launch_bytes = F * FACE_BYTES_PER_FACE + N * FACE_BYTES_PER_CELL
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     workload = "cylinder_ns_ogrid_%dx%dx2" % (args.n_theta, args.n_r)
+    if args.stretch is not None:
+        workload += "_stretch%g" % args.stretch
     traffic = MEASURED_TRAFFIC.get((kernel, workload, args.stage_path, args.tile_cells, args.tile_order,
                                     args.cell_order))
 
