@@ -81,6 +81,47 @@ def flat_plate_fixture(k=0):
     return raw, cfg
 
 
+def hanging_node_mesh(nc=6, ny=4):
+    """Channel [0,2]x[0,1]: nc x ny coarse quads on x < 1, 2nc x 2ny fine quads on x > 1.
+    The coarse cells on x = 1 carry the fine column's midpoint as a hanging node, which makes
+    them pentagons (5 edges, so the device uses 6 slots per cell).  Tags as channel_case."""
+    hc, hy = 1.0 / nc, 1.0 / ny
+    nodes = []
+    C = {}
+    for i in range(nc):                      # coarse columns x < 1
+        for j in range(ny + 1):
+            C[i, j] = len(nodes)
+            nodes.append([i * hc, j * hy])
+    Fn = {}
+    for i in range(2 * nc + 1):              # fine columns from x = 1
+        for j in range(2 * ny + 1):
+            Fn[i, j] = len(nodes)
+            nodes.append([1.0 + i * hc / 2, j * hy / 2])
+    elems = []
+    for i in range(nc):
+        for j in range(ny):
+            if i < nc - 1:
+                elems.append([C[i, j], C[i + 1, j], C[i + 1, j + 1], C[i, j + 1]])
+            else:
+                elems.append([C[i, j], Fn[0, 2 * j], Fn[0, 2 * j + 1], Fn[0, 2 * j + 2], C[i, j + 1]])
+    for i in range(2 * nc):
+        for j in range(2 * ny):
+            elems.append([Fn[i, j], Fn[i + 1, j], Fn[i + 1, j + 1], Fn[i, j + 1]])
+    inflow = [[C[0, j + 1], C[0, j]] for j in range(ny)]
+    outflow = [[Fn[2 * nc, j], Fn[2 * nc, j + 1]] for j in range(2 * ny)]
+    bottom = [[C[i, 0], C[i + 1, 0]] for i in range(nc - 1)] + [[C[nc - 1, 0], Fn[0, 0]]] + \
+             [[Fn[i, 0], Fn[i + 1, 0]] for i in range(2 * nc)]
+    top = [[C[i + 1, ny], C[i, ny]] for i in range(nc - 1)] + [[Fn[0, 2 * ny], C[nc - 1, ny]]] + \
+          [[Fn[i + 1, 2 * ny], Fn[i, 2 * ny]] for i in range(2 * nc)]
+    groups = [inflow, outflow, bottom, top]
+    offs = np.cumsum([0] + [len(e) for e in elems]).astype(np.uint32)
+    starts = np.cumsum([0] + [len(g) for g in groups[:-1]]).astype(np.uint64)
+    return meshgen.RawMesh(nodes=np.array(nodes, float), elem_offsets=offs,
+                           elem_nodes=np.array([v + 1 for e in elems for v in e], np.uint32),
+                           bnodes=np.array([p for g in groups for p in g], np.uint32),
+                           btags=["Inflow", "Outflow", "Lower", "Upper"], bgroup_start=starts)
+
+
 def tiny_mesh(kind):
     """Meshes whose every cell is a boundary cell: one triangle, two triangles, one quad;
     one far-field tag."""

@@ -15,8 +15,8 @@ import pytest
 import oracle
 from ebbcfd_amd import Solver, build_mesh, meshgen
 
-from helpers import (base_config, channel_case, cylinder_case, flat_plate_fixture, mms_case, oracle_cfg, perturb,
-                     reference_fixture, tiny_mesh)
+from helpers import (base_config, channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mms_case,
+                     oracle_cfg, perturb, reference_fixture, tiny_mesh)
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +34,7 @@ CASES = {
     "mms_fixture_tri_lp": lambda: reference_fixture("ms_refine_tri_pert_0"),
     "mms_fixture_quad_lp": lambda: reference_fixture("ms_refine_1"),
     "flat_plate_fixture": lambda: flat_plate_fixture(0),
+    "pentagons_rk2tvd_visc": lambda: (hanging_node_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
     "quads_rk2tvd_visc": lambda: (meshgen.channel_bump_quads(24, 8),
                                   channel_case(integrator="RK2_TVD", viscosity=True)[1]),
 }

@@ -15,7 +15,7 @@ from oracle.gmsh import parse_gmsh
 from ebbcfd_amd import _lib, build_mesh, import_mesh, load_config, meshgen
 from ebbcfd_amd.solver import Config
 
-from helpers import FLAT_DIR, MMS_DIR, ROOT
+from helpers import FLAT_DIR, MMS_DIR, ROOT, hanging_node_mesh
 
 
 def test_library_exports_every_declared_symbol():
@@ -76,6 +76,16 @@ def test_threaded_build_matches_oracle(maker, tmp_path):
     path = str(tmp_path / "m.msh")
     meshgen.write_gmsh(raw, path)
     _compare_geometry(import_mesh(path), om)
+
+
+def test_pentagon_mesh_geometry_matches_oracle():
+    """Coarse quads next to a refined column: the interface cells have a hanging node and
+    five edges."""
+    raw = hanging_node_mesh()
+    b = build_mesh(raw)
+    om = oracle.Mesh(raw)
+    assert b.max_edges == 5
+    _compare_geometry(b, om)
 
 
 def test_negative_area_rejected():
