@@ -122,6 +122,28 @@ def hanging_node_mesh(nc=6, ny=4):
                            btags=["Inflow", "Outflow", "Lower", "Upper"], bgroup_start=starts)
 
 
+def mixed_mesh(nx=24, ny=8, frac=0.5, seed=5):
+    """The channel_bump quads with a seeded random subset (about ``frac``) split into two
+    triangles along a random diagonal: triangles and quads side by side, so the device
+    uses 4 slots per cell and pads the triangles.  Tags as channel_case."""
+    raw = meshgen.channel_bump_quads(nx, ny, seed=seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    quads = raw.elem_nodes.reshape(-1, 4)
+    split = rng.uniform(size=quads.shape[0]) < frac
+    diag = rng.integers(0, 2, size=quads.shape[0])
+    elems = []
+    for q, s, dg in zip(quads.tolist(), split, diag):
+        if not s:
+            elems.append(q)
+        elif dg:   # diagonal 0-2
+            elems += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+        else:      # diagonal 1-3
+            elems += [[q[0], q[1], q[3]], [q[1], q[2], q[3]]]
+    raw.elem_offsets = np.cumsum([0] + [len(e) for e in elems]).astype(np.uint32)
+    raw.elem_nodes = np.array([v for e in elems for v in e], np.uint32)
+    return raw
+
+
 def tiny_mesh(kind):
     """Meshes whose every cell is a boundary cell: one triangle, two triangles, one quad;
     one far-field tag."""

@@ -15,7 +15,7 @@ import pytest
 import oracle
 from ebbcfd_amd import Solver, build_mesh, meshgen
 
-from helpers import (base_config, channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mms_case,
+from helpers import (base_config, channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mixed_mesh, mms_case,
                      oracle_cfg, perturb, reference_fixture, tiny_mesh)
 
 pytestmark = pytest.mark.gpu
@@ -35,6 +35,8 @@ CASES = {
     "mms_fixture_quad_lp": lambda: reference_fixture("ms_refine_1"),
     "flat_plate_fixture": lambda: flat_plate_fixture(0),
     "pentagons_rk2tvd_visc": lambda: (hanging_node_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
+    "mixed_rk2tvd_visc": lambda: (mixed_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
+    "mixed_rk4_lp": lambda: (mixed_mesh(seed=8), channel_case(integrator="RK4", lpThresh=1.0)[1]),
     "quads_rk2tvd_visc": lambda: (meshgen.channel_bump_quads(24, 8),
                                   channel_case(integrator="RK2_TVD", viscosity=True)[1]),
 }

@@ -13,7 +13,7 @@ import pytest
 import oracle
 from ebbcfd_amd import Solver, build_mesh
 
-from helpers import (channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mms_case, oracle_cfg, perturb,
+from helpers import (channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mixed_mesh, mms_case, oracle_cfg, perturb,
                      reference_fixture)
 
 pytestmark = pytest.mark.gpu
@@ -48,6 +48,8 @@ RESIDUAL_CASES = {
     "mms_fixture_quads_lp": lambda: reference_fixture("ms_refine_1"),
     "flat_plate_fixture": lambda: flat_plate_fixture(0),
     "pentagons_viscous": lambda: (hanging_node_mesh(), channel_case(flux="roeFlux", viscosity=True)[1]),
+    "mixed_viscous": lambda: (mixed_mesh(), channel_case(flux="roeFlux", viscosity=True)[1]),
+    "mixed_lp": lambda: (mixed_mesh(), channel_case(flux="rusanovFlux", lpThresh=1.0)[1]),
 }
 
 
@@ -93,6 +95,8 @@ STEP_CASES = {
     "flat_plate_fixture_0": lambda: flat_plate_fixture(0),
     "flat_plate_fixture_1": lambda: flat_plate_fixture(1),
     "pentagons_rk2tvd_viscous": lambda: (hanging_node_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
+    "mixed_rk2tvd_viscous": lambda: (mixed_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
+    "mixed_rk4_lp": lambda: (mixed_mesh(seed=8), channel_case(integrator="RK4", lpThresh=1.0)[1]),
 }
 
 

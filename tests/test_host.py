@@ -15,7 +15,7 @@ from oracle.gmsh import parse_gmsh
 from ebbcfd_amd import _lib, build_mesh, import_mesh, load_config, meshgen
 from ebbcfd_amd.solver import Config
 
-from helpers import FLAT_DIR, MMS_DIR, ROOT, hanging_node_mesh
+from helpers import FLAT_DIR, MMS_DIR, ROOT, hanging_node_mesh, mixed_mesh
 
 
 def test_library_exports_every_declared_symbol():
@@ -85,6 +85,16 @@ def test_pentagon_mesh_geometry_matches_oracle():
     b = build_mesh(raw)
     om = oracle.Mesh(raw)
     assert b.max_edges == 5
+    _compare_geometry(b, om)
+
+
+def test_mixed_mesh_geometry_matches_oracle():
+    """Triangles and quadrilaterals side by side in one mesh."""
+    raw = mixed_mesh()
+    assert sorted(set(np.diff(raw.elem_offsets).tolist())) == [3, 4]
+    b = build_mesh(raw)
+    om = oracle.Mesh(raw)
+    assert b.max_edges == 4
     _compare_geometry(b, om)
 
 
