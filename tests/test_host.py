@@ -27,6 +27,32 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
+def _declarations(text):
+    """name -> parameter count of every `ebb_*` function declared in C or D text."""
+    out = {}
+    for m in re.finditer(r"\b(ebb_\w+)\s*\(", text):
+        depth, k = 1, m.end()
+        while depth:   # D writes const(char)*: match the parentheses
+            depth += {"(": 1, ")": -1}.get(text[k], 0)
+            k += 1
+        if not re.match(r"\s*;", text[k:]):
+            continue   # a call, not a declaration
+        params = text[m.end():k - 1]
+        out[m.group(1)] = 0 if params.strip() in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_d_binding_declares_every_entry_point():
+    """The extern(C) D module in INTEGRATION.md names every function of the C ABI with
+    the header's parameter count."""
+    header = _declarations(open(os.path.join(ROOT, "include", "ebbcfd.h")).read())
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    d_block = doc.split("```d\nmodule ebb.gpu;")[1].split("```")[0]
+    d = _declarations(d_block)
+    assert set(header) == set(_lib.SYMBOLS)
+    assert d == header, {k: (header.get(k), d.get(k)) for k in set(header) | set(d) if header.get(k) != d.get(k)}
+
+
 def _compare_geometry(mesh, om):
     g, og = mesh.cell_geometry(), om.cells()
     N = om.n_interior
