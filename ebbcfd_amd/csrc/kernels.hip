@@ -540,25 +540,49 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
     if (have) {
         double qi[4];
         ld4(a.qin, i, qi);
-        double qn[S][4];
         uint32_t nb[S];
 #pragma unroll
-        for (int j = 0; j < S; j++) {
-            nb[j] = m.nbr[i * S + j];
-            if (nb[j] >= m.N) {
-                if (!ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qn[j]) && a.recordErrors)
-                    atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
-                st4(a.qin, nb[j], qn[j]);
-            } else {
-                ld4(a.qin, nb[j], qn[j]);
-            }
-        }
-        if (MODE == 0) return;
-        double2 cn[S];
-        bool real[S];
+        for (int j = 0; j < S; j++) nb[j] = m.nbr[i * S + j];
+        if constexpr (MODE == 0) {
+            // order 1: the ghost states are the kernel's only output
 #pragma unroll
-        for (int j = 0; j < S; j++) { cn[j] = m.cen[nb[j]]; real[j] = nb[j] != i; }
-        gradLimit<S, MODE>(qi, qn, nullptr, 0, m.cen[i], cn, real, a.limit, p.lpThresh, g, lx, ly);
+            for (int j = 0; j < S; j++)
+                if (nb[j] >= m.N) {
+                    double qg[4];
+                    if (!ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qg) && a.recordErrors)
+                        atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
+                    st4(a.qin, nb[j], qg);
+                }
+            return;
+        } else {
+            // Every neighbour row is loaded before any ghost is written, so the loads of
+            // all neighbours are in flight together.  A ghost slot belongs to this cell
+            // alone (the owner of its boundary face), so its row still holds the stage
+            // buffer's previous ghost state, which TempPresInflow keeps in the components
+            // it does not write.
+            double qn[S][4];
+            double2 cn[S];
+#pragma unroll
+            for (int j = 0; j < S; j++) { ld4(a.qin, nb[j], qn[j]); cn[j] = m.cen[nb[j]]; }
+            const double2 ci = m.cen[i];
+#pragma unroll
+            for (int j = 0; j < S; j++)
+                if (nb[j] >= m.N) {
+                    const BFace& bf = m.bface[nb[j] - m.N];
+                    double old[4] = {0, 0, 0, 0};
+                    if (bf.type == BT_TPIN) {
+#pragma unroll
+                        for (int k = 0; k < 4; k++) old[k] = qn[j][k];
+                    }
+                    if (!ghostState(bf, qi, qn[j], old) && a.recordErrors)
+                        atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
+                    st4(a.qin, nb[j], qn[j]);
+                }
+            bool real[S];
+#pragma unroll
+            for (int j = 0; j < S; j++) real[j] = nb[j] != i;
+            gradLimit<S, MODE>(qi, qn, nullptr, 0, ci, cn, real, a.limit, p.lpThresh, g, lx, ly);
+        }
     }
     if (MODE == 0) return;
     // stage the rows in LDS and store the workgroup's contiguous run of cells with
