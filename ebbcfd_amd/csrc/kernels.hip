@@ -1301,10 +1301,11 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
     const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x);
     const uint4 d = t.desc[tile];
     const uint32_t nC = d.x, nF = d.y, tid = threadIdx.x;
-    // the first cell round's ids are loaded ahead of the faces; the next round's face
-    // record is loaded one round ahead, so each round waits for one memory latency
+    // the first two cell rounds' ids are loaded ahead of the faces; the next round's
+    // face record is loaded one round ahead, so each round waits for one memory latency
     // (cell data) instead of two (record, then cell data)
     uint32_t giNext = tid < nC ? t.ids[d.z + tid] : 0u;
+    uint32_t giNext2 = tid + BLK < nC ? t.ids[d.z + tid + BLK] : 0u;
     uint4 rNext = make_uint4(0, 0, 0, 0);
     if (tid < nF) rNext = t.fref[d.w + tid];
     for (uint32_t k = tid; k < nF; k += BLK) {
@@ -1357,7 +1358,7 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
         slot[4] = len * sMax;
     }
     // each cell round's per-cell inputs are loaded one round ahead (the first round's
-    // before the barrier, which waits for LDS only)
+    // before the barrier, which waits for LDS only), from ids loaded two rounds ahead
     double metN[3] = {0, 0, 0}, qnN[4] = {0, 0, 0, 0}, qiN[4] = {0, 0, 0, 0};
     const bool wantQn = a.kind != SK_RESIDUAL, wantQi = a.kind == SK_TVD_2;
     uint32_t cfN[S] = {};
@@ -1389,7 +1390,8 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
 #pragma unroll
         for (int j = 0; j < S; j++) cfC[j] = cfN[j];
         if (k + BLK < nC) {
-            giNext = t.ids[d.z + k + BLK];
+            giNext = giNext2;
+            if (k + 2 * BLK < nC) giNext2 = t.ids[d.z + k + 2 * BLK];
             loadCellIn(giNext);
             loadCf(k + BLK);
         }
