@@ -1312,37 +1312,34 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
         const uint4 r = rNext;   // cell a, cell b, node 0, node 1
         if (k + BLK < nF) rNext = t.fref[d.w + k + BLK];
         const bool interior = r.y < m.N;
+        // every load of the face is issued before its arithmetic, with no branch between
+        // them, so the round waits for one memory latency.  Side b of a boundary face is
+        // its ghost (cell N + b), which has a state and a centroid but no gradient or
+        // limiter row: the owner's row is read in its place and not used.
+        const double2 p0 = m.node[r.z], p1 = m.node[r.w];
+        const uint32_t rb = interior ? r.y : r.x;
+        double qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
+        ld4(a.qin, r.x, qa);
+        ld4(a.qin, r.y, qb);
+        if (ORDER > 1 || VISC) { loadGrad(a.grad, r.x, ga); loadGrad(a.grad, rb, gb); }
+        double2 ca = make_double2(0, 0), cb = make_double2(0, 0);
+        if (ORDER > 1) {
+            loadLim<LP>(a.lim, r.x, lxa, lya);
+            loadLim<LP>(a.lim, rb, lxb, lyb);
+            ca = m.cen[r.x]; cb = m.cen[r.y];
+        }
         double2 n, mid;
         double len;
-        edgeGeometry(interior, m.node[r.z], m.node[r.w], n, mid, len);
+        edgeGeometry(interior, p0, p1, n, mid, len);
         double qL[4], qR[4], F[4], sMax;
         bool isB = false;
         uint32_t b = 0;
         if (interior) {
-            double qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
-            ld4(a.qin, r.x, qa);
-            ld4(a.qin, r.y, qb);
-            if (ORDER > 1 || VISC) { loadGrad(a.grad, r.x, ga); loadGrad(a.grad, r.y, gb); }
-            double2 ca = make_double2(0, 0), cb = make_double2(0, 0);
-            if (ORDER > 1) {
-                loadLim<LP>(a.lim, r.x, lxa, lya);
-                loadLim<LP>(a.lim, r.y, lxb, lyb);
-                ca = m.cen[r.x]; cb = m.cen[r.y];
-            }
             interiorFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, n, mid, p, qL, qR, F, sMax);
         } else {
             isB = true;
             b = r.y - m.N;   // the ghost of boundary face b is cell N + b
-            double qo[4], qg[4], gO[8], lx[4], ly[4];
-            ld4(a.qin, r.x, qo);
-            ld4(a.qin, r.y, qg);
-            if (ORDER > 1 || VISC) loadGrad(a.grad, r.x, gO);
-            double2 co = make_double2(0, 0);
-            if (ORDER > 1) {
-                loadLim<LP>(a.lim, r.x, lx, ly);
-                co = m.cen[r.x];
-            }
-            boundaryFace<FLUX, VISC, ORDER, LP>(qo, gO, lx, ly, co, qg, m.bface[b], n, mid, p, qL, qR, F, sMax);
+            boundaryFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, m.bface[b], n, mid, p, qL, qR, F, sMax);
             // boundary faces (each in exactly one tile) feed the force integral
             const uint32_t f = m.Fi + b;
             *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) =
