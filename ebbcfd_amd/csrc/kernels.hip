@@ -885,11 +885,17 @@ DEV double warpMax(double v)
 // its summed face fluxes r and sum of len*sMax
 // met: (perimeter, d, -1/area) of the cell, or null to read them from the mesh
 // qNew/qOld (optional): the new state and q^n when this stage completes a step
+// qnI, accI (optional): the cell's q^n and RK accumulator rows already loaded
 template <int KIND>
 DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t i, const double r[4],
                    double sAve, const double* qinI, const double* met, double& dtCand, double& rCand,
-                   double* qNew = nullptr, double* qOld = nullptr, const double* qnI = nullptr)
+                   double* qNew = nullptr, double* qOld = nullptr, const double* qnI = nullptr,
+                   const double* accI = nullptr)
 {
+    auto loadAcc = [&](double v[4]) {
+        if (accI) { v[0] = accI[0]; v[1] = accI[1]; v[2] = accI[2]; v[3] = accI[3]; }
+        else ld4(a.acc, i, v);
+    };
     const double cper = met ? met[0] : m.cper[i], cd = met ? met[1] : m.cd[i], nia = met ? met[2] : m.nia[i];
     sAve /= cper;
     const double CFL = a.sc->CFL;
@@ -935,7 +941,7 @@ DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, ui
         st4(a.qout, i, o);
     } else if (KIND == SK_RK2_2) {
         double k1[4];
-        ld4(a.acc, i, k1);
+        loadAcc(k1);
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 2.0) * (k1[k] + R[k]);
         st4(a.qout, i, o);
@@ -957,7 +963,7 @@ DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, ui
         st4(a.qout, i, o);
     } else if (KIND == SK_RK4_2 || KIND == SK_RK4_3) {
         double ac[4];
-        ld4(a.acc, i, ac);
+        loadAcc(ac);
 #pragma unroll
         for (int k = 0; k < 4; k++) ac[k] = ac[k] + 2.0 * R[k];
         const double c = (KIND == SK_RK4_2) ? (dt / 2.0) : dt;
@@ -967,7 +973,7 @@ DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, ui
         st4(a.qout, i, o);
     } else if (KIND == SK_RK4_4) {
         double ac[4];
-        ld4(a.acc, i, ac);
+        loadAcc(ac);
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 6.0) * (ac[k] + R[k]);
         st4(a.qout, i, o);
@@ -1356,13 +1362,19 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
     }
     // each cell round's per-cell inputs are loaded one round ahead (the first round's
     // before the barrier, which waits for LDS only), from ids loaded two rounds ahead
-    double metN[3] = {0, 0, 0}, qnN[4] = {0, 0, 0, 0}, qiN[4] = {0, 0, 0, 0};
+    // (a load issued after the next round's would wait for it: loads complete in order)
+    double metN[3] = {0, 0, 0}, qnN[4] = {0, 0, 0, 0}, qiN[4] = {0, 0, 0, 0}, acN[4] = {0, 0, 0, 0};
+    uint32_t refN = 0;
     const bool wantQn = a.kind != SK_RESIDUAL, wantQi = a.kind == SK_TVD_2;
+    const bool wantAcc = a.kind == SK_RK2_2 || a.kind == SK_RK4_2 || a.kind == SK_RK4_3 || a.kind == SK_RK4_4;
+    const bool wantRef = a.rmsBlocks != nullptr;
     uint32_t cfN[S] = {};
     auto loadCellIn = [&](uint32_t c) {
         metN[0] = m.cper[c]; metN[1] = m.cd[c]; metN[2] = m.nia[c];
         if (wantQn) ld4(a.qn, c, qnN);
         if (wantQi) ld4(a.qin, c, qiN);
+        if (wantAcc) ld4(a.acc, c, acN);
+        if (wantRef) refN = m.refId[c];
     };
     auto loadCf = [&](uint32_t k) {
 #pragma unroll
@@ -1378,11 +1390,12 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
         const bool have = k < nC;
         double qNew[4] = {0, 0, 0, 0}, qOld[4] = {0, 0, 0, 0};
         uint32_t gi = giNext;
-        double met[3], qnC[4], qiC[4];
+        double met[3], qnC[4], qiC[4], acC[4];
 #pragma unroll
         for (int x = 0; x < 3; x++) met[x] = metN[x];
 #pragma unroll
-        for (int x = 0; x < 4; x++) { qnC[x] = qnN[x]; qiC[x] = qiN[x]; }
+        for (int x = 0; x < 4; x++) { qnC[x] = qnN[x]; qiC[x] = qiN[x]; acC[x] = acN[x]; }
+        const uint32_t refC = refN;
         uint32_t cfC[S];
 #pragma unroll
         for (int j = 0; j < S; j++) cfC[j] = cfN[j];
@@ -1404,19 +1417,19 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
                 sAve += lf[4];
             }
             switch (a.kind) {
-            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
-            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
-            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
-            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
-            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
-            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
-            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
-            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
-            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC); break;
+            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
+            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
+            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
+            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
             default: cellStage<SK_RESIDUAL>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
             }
         }
-        if (a.rmsBlocks) rmsBlockLeaf(have, qNew, qOld, have ? m.refId[gi] : 0u, a.rmsBlocks);
+        if (a.rmsBlocks) rmsBlockLeaf(have, qNew, qOld, have ? refC : 0u, a.rmsBlocks);
     }
     blockPartial(dtCand, rCand, a.partial);
 }
