@@ -668,13 +668,18 @@ DEV void interiorFace(const double qa[4], const double ga[8], const double lxa[4
 #pragma unroll
         for (int k = 0; k < 4; k++) { qL[k] = qa[k]; qR[k] = qb[k]; }
     }
-    numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+    // the face gradient is formed before the convective flux, so the two cell gradients
+    // are not live across it
+    double dq[8];
     if (VISC) {
-        double qa2[4], dq[8], Fv[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) qa2[k] = 0.5 * (qL[k] + qR[k]);
 #pragma unroll
         for (int k = 0; k < 8; k++) dq[k] = 0.5 * (ga[k] + gb[k]);
+    }
+    numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+    if (VISC) {
+        double qa2[4], Fv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) qa2[k] = 0.5 * (qL[k] + qR[k]);
         viscousFlux(p.Pr, p.mu, qa2, dq, n.x, n.y, Fv);
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
