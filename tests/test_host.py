@@ -131,6 +131,56 @@ def test_negative_area_rejected():
         build_mesh(raw)
 
 
+def test_c_abi_error_codes(tmp_path):
+    """The status codes of include/ebbcfd.h on bad input, through the C ABI itself:
+    null handles and pointers (1), unreadable files (2), malformed input (3), meshes that
+    buildMesh rejects (4)."""
+    import ctypes as C
+    L = _lib.lib()
+    vp = C.c_void_p()
+    d = C.c_double()
+    assert L.ebb_mesh_load(None, C.byref(vp)) == 1
+    assert L.ebb_mesh_load(str(tmp_path / "missing.msh").encode(), C.byref(vp)) == 2
+    (tmp_path / "mesh.txt").write_text("not a mesh")
+    assert L.ebb_mesh_load(str(tmp_path / "mesh.txt").encode(), C.byref(vp)) == 3   # neither .msh nor .gri
+    (tmp_path / "bad.msh").write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n3\n1 0 0\n")
+    assert L.ebb_mesh_load(str(tmp_path / "bad.msh").encode(), C.byref(vp)) == 3     # truncated node block
+    assert L.ebb_config_parse(b"{not json", C.byref(vp)) == 3
+    assert L.ebb_config_parse(None, C.byref(vp)) == 1
+    assert L.ebb_config_get(None, b"CFL", C.byref(d)) == 1
+    assert L.ebb_mesh_info(None, (C.c_int64 * 8)()) == 1
+    assert L.ebb_mesh_tile_stats(None, 128, 0, (C.c_double * 8)()) == 1
+    assert L.ebb_solver_create(None, None, 0, C.byref(vp)) == 1
+    for name in ("ebb_state_upload", "ebb_state_download", "ebb_run_reset"):
+        assert getattr(L, name)(None, *([None] if name != "ebb_run_reset" else [0.0, 0.0])) == 1
+    assert L.ebb_history(None, 0, 0, None) == 1
+
+    def create(nodes, eoff, enodes, bnodes):
+        nodes = np.ascontiguousarray(nodes, np.float64)
+        eoff = np.ascontiguousarray(eoff, np.uint32)
+        enodes = np.ascontiguousarray(enodes, np.uint32)
+        bnodes = np.ascontiguousarray(bnodes, np.uint32).reshape(-1, 2)
+        tags = (C.c_char_p * 1)(b"Farfield")
+        starts = np.zeros(1, np.uint64)
+        P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        h = C.c_void_p()
+        rc = L.ebb_mesh_create(nodes.shape[0], P(nodes, C.c_double), eoff.size - 1, P(eoff, C.c_uint32),
+                               P(enodes, C.c_uint32), bnodes.shape[0], P(bnodes, C.c_uint32), 1, tags,
+                               P(starts, C.c_uint64), C.byref(h))
+        if rc == 0:
+            L.ebb_mesh_destroy(h)
+        return rc
+
+    square = [[0, 0], [1, 0], [1, 1], [0, 1]]
+    ring = [[0, 1], [1, 2], [2, 3], [3, 0]]
+    assert create(square, [0, 3, 6], [1, 2, 3, 1, 3, 4], ring) == 0
+    assert create(square, [0, 3, 6], [1, 2, 3, 1, 3, 5], ring) == 4      # node 5 of 4 (1-based)
+    assert create(square, [0, 3, 6], [1, 2, 3, 1, 3, 0], ring) == 4      # node 0 (elements are 1-based)
+    assert create(square, [0, 3, 6], [1, 2, 3, 1, 3, 4], [[0, 1], [1, 7]]) == 4   # boundary node out of range
+    assert create(square, [0, 2, 6], [1, 2, 3, 1, 3, 4], ring) == 4      # a two-node element
+    assert create(square, [0, 3, 6], [3, 2, 1, 1, 3, 4], ring) == 4      # clockwise: negative area
+
+
 REF_CFG = json.load(open(os.path.join(MMS_DIR, "ms_refine_tri_0.json")))
 
 
