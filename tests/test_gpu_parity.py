@@ -180,3 +180,29 @@ def test_nan_edge_reported_like_reference():
         s.ufvm_solver()
     assert eg.value.code == 7
     assert eg.value.index == eo.value.index
+
+
+def test_run_after_an_error_needs_a_reset():
+    """The reference aborts on a NaN (app/solver.d:395-423).  Here the loop stops, further
+    runs are refused (EBB_ERR_STATE), and ebb_run_reset plus a new state starts it again
+    exactly as a fresh solver would."""
+    raw, cfg = channel_case(integrator="RK2")
+    s = Solver(build_mesh(raw), cfg)
+    q0 = s.q
+    q = q0.copy()
+    q[17, 0] = -1.0
+    s.q = q
+    with pytest.raises(Exception) as e1:
+        s.run(3)
+    assert e1.value.code in (7, 8)
+    with pytest.raises(Exception) as e2:
+        s.run(3)
+    assert e2.value.code == 9
+    s.run_reset()
+    s.q = q0
+    h, t, dt = s.run(5)
+    fresh = Solver(build_mesh(raw), cfg)
+    hf, tf, dtf = fresh.run(5)
+    np.testing.assert_array_equal(h, hf)
+    assert t == tf and dt == dtf
+    np.testing.assert_array_equal(s.q, fresh.q)
