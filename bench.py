@@ -150,7 +150,8 @@ def main():
     ap.add_argument("--n-r", type=int, default=1024)
     ap.add_argument("--stretch", type=float, default=None,
                     help="radial growth ratio (default: the generator's, 1.006 at 512+ radial cells)")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=4,
+                    help="oracle steps timed for the CPU baseline, about 13 s on one core (0: skip)")
     ap.add_argument("--time-steps", type=int, default=20, help="steps timed per kernel with HIP events")
     ap.add_argument("--stage-path", default="faceTiles", choices=("split", "faceTiles", "fused"),
                     help="kernels of one stage: gradient/face/update, gradient + face/update tiles, or one fused kernel")
