@@ -168,6 +168,10 @@ struct ebb_solver {
     int lastLimit = 1;
     bool haveStage = false;
     uint32_t stageCounter = 0;
+    // the step of each stage launched since the error words were last cleared (stage
+    // stageBase + k ran in step stageStep[k]): edge and boundary errors carry their stage
+    uint32_t stageBase = 0;
+    std::vector<int64_t> stageStep;
     // stage partials: pairs per stage (the stage kernel's workgroups), region stride,
     // and the stage of the current time-loop step
     uint32_t nPart = 0, pstride = 0, stageInStep = 0;
@@ -239,6 +243,7 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
     a.rmsBlocks = (last && checkActive && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
     a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
     a.checkActive = checkActive; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = stepId;
+    s->stageStep.push_back(stepId);
     if (checkActive) {
         // time loop: each stage leaves its partials in its own region; the step's RMS
         // kernel reduces them (nothing in the step reads newDt or Rmax before that)
@@ -326,6 +331,19 @@ int readScalars(ebb_solver* s)
     return EBB_OK;
 }
 
+// a new window of stages whose errors the next checkErrors reports
+static void beginErrorWindow(ebb_solver* s)
+{
+    s->stageBase = s->stageCounter;
+    s->stageStep.clear();
+}
+
+static int64_t stepOfStage(const ebb_solver* s, uint32_t stage)
+{
+    const uint32_t k = stage - s->stageBase;
+    return k < s->stageStep.size() ? s->stageStep[k] : -1;
+}
+
 int checkErrors(ebb_solver* s)
 {
     const DevScalars& h = *s->h_sc;
@@ -335,6 +353,7 @@ int checkErrors(ebb_solver* s)
         const uint64_t key = h.errBC & 0xFFFFFFFFull;
         s->errKind = EBB_ERR_BC;
         s->errIndex = int64_t(s->b.boundaryEdges[key]);
+        s->errStep = stepOfStage(s, uint32_t(h.errBC >> 32));
         s->errMsg = "Both computed mach numbers are negative";
         return EBB_ERR_BC;
     }
@@ -343,6 +362,7 @@ int checkErrors(ebb_solver* s)
         int64_t edge = key < s->Fb ? int64_t(s->b.boundaryEdges[key]) : int64_t(s->b.interiorEdges[key - s->Fb]);
         s->errKind = EBB_ERR_EDGE_NAN;
         s->errIndex = edge;
+        s->errStep = stepOfStage(s, uint32_t(h.errEdge >> 32));
         s->errMsg = key < s->Fb ? "Got NaN on boundary edge" : "Got NaN on interior edge";
         return EBB_ERR_EDGE_NAN;
     }
@@ -361,6 +381,7 @@ int clearErrors(ebb_solver* s)
     // keep t/dt/CFL/counters, clear the error words and reactivate
     HIPCHK(hipMemcpyAsync(s->h_sc, s->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    beginErrorWindow(s);
     s->h_sc->errEdge = ~0ull;
     s->h_sc->errCell = ~0ull;
     s->h_sc->errBC = ~0ull;
@@ -989,6 +1010,7 @@ int ebb_residual(ebb_solver* s, int limit, int dt_update, double* R, double* new
     { int rc_ = readScalars(s); if (rc_) return rc_; }
     s->h_sc->rmaxBits = 0;
     if (rmax && *rmax > 0) std::memcpy(&s->h_sc->rmaxBits, rmax, 8);
+    beginErrorWindow(s);
     s->h_sc->errEdge = ~0ull;
     s->h_sc->errCell = ~0ull;
     s->h_sc->errBC = ~0ull;
@@ -1012,6 +1034,7 @@ int ebb_step(ebb_solver* s, double* dt, double* rmax)
     { int rc_ = readScalars(s); if (rc_) return rc_; }
     DevScalars saved = *s->h_sc;
     s->h_sc->dt = *dt;
+    beginErrorWindow(s);
     s->h_sc->errEdge = ~0ull;
     s->h_sc->errCell = ~0ull;
     s->h_sc->errBC = ~0ull;
@@ -1056,6 +1079,7 @@ int ebb_run(ebb_solver* s, int64_t max_steps, double* t, double* dt, int64_t* st
         while (limit < 0 || done < limit) {
             const int64_t batch = limit < 0 ? s->histCap : std::min<int64_t>(s->histCap, limit - done);
             const int64_t base = s->h_sc->steps;
+            beginErrorWindow(s);   // no error is pending: the previous batch would have stopped
             const int64_t stopAt = base + batch;
             int c0 = s->cur;
             for (int64_t k = 0; k < batch; k++) {
@@ -1362,6 +1386,7 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
             a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
             a.rmsBlocks = (x.last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
             a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
+            s->stageStep.push_back(s->stepsTotal);
             if (s->useTiles) {
                 launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
                 HIPCHK(hipEventRecord(ev[k++], s->stream));

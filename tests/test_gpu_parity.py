@@ -182,6 +182,47 @@ def test_nan_edge_reported_like_reference():
     assert eg.value.index == eo.value.index
 
 
+def test_error_step_is_the_step_that_failed():
+    """A fixed time step too large for the channel makes the run diverge after a few steps:
+    the device stops in the step the oracle stops in, on the same edge, and reports that
+    step (the number of steps completed before it)."""
+    raw, cfg = channel_case(integrator="Euler", dynamicDt=False, dt=0.03)
+    om = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    done = 0
+    with pytest.raises(oracle.OracleError) as eo:
+        while done < 100:
+            om.run(1)
+            done += 1
+    assert done >= 2
+    s = Solver(build_mesh(raw), cfg)
+    with pytest.raises(Exception) as eg:
+        s.run(100)
+    assert eg.value.code == 7 and eg.value.index == eo.value.index
+    assert eg.value.step == done
+
+
+def test_solvers_release_device_memory():
+    """Ten solvers created, run (with edge states, which allocate on first use) and
+    destroyed leave the device's free memory where it was."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    raw, cfg = channel_case(nx=200, ny=100, integrator="RK2_TVD")   # 40,000 cells
+    mesh = build_mesh(raw)
+    s = Solver(mesh, cfg)
+    s.run(2)
+    del s
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    f0 = free.value
+    for _ in range(10):
+        s = Solver(mesh, cfg)
+        s.run(2)
+        s.edge_states()
+        del s
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    assert f0 - free.value < (64 << 20), (f0, free.value)
+
+
 def test_run_after_an_error_needs_a_reset():
     """The reference aborts on a NaN (app/solver.d:395-423).  Here the loop stops, further
     runs are refused (EBB_ERR_STATE), and ebb_run_reset plus a new state starts it again
@@ -194,7 +235,7 @@ def test_run_after_an_error_needs_a_reset():
     s.q = q
     with pytest.raises(Exception) as e1:
         s.run(3)
-    assert e1.value.code in (7, 8)
+    assert e1.value.code in (7, 8) and e1.value.step == 0   # the first step, for edges and cells alike
     with pytest.raises(Exception) as e2:
         s.run(3)
     assert e2.value.code == 9
