@@ -1,0 +1,50 @@
+// D binding of the C ABI in ebbcfd.h (libebbcfd.so), for D host code that keeps the
+// reference's mesh.d / solve.d / integrators.d and calls the MI355X path instead of
+// ufvmSolver.  Link with "libs": ["ebbcfd"] and "lflags": ["-L<dir of libebbcfd.so>"].
+// Types map as uint32_t -> uint, int64_t -> long, uint64_t -> ulong, int8_t -> byte,
+// uint8_t -> ubyte; handles are opaque structs.
+module ebb.gpu;
+extern(C) @nogc nothrow:
+
+struct ebb_mesh; struct ebb_config; struct ebb_solver;
+
+int ebb_mesh_create(uint nNodes, const(double)* xy, uint nElem, const(uint)* elemOffsets,
+                    const(uint)* elemNodes, uint nBnodes, const(uint)* bnodes, uint nTags,
+                    const(char*)* tags, const(ulong)* bgroupStart, ebb_mesh** mesh);
+int ebb_mesh_load(const(char)* path, ebb_mesh** mesh);
+int ebb_config_parse(const(char)* json, ebb_config** cfg);
+int ebb_solver_create(const(ebb_mesh)* mesh, const(ebb_config)* cfg, int device, ebb_solver** s);
+int ebb_state_upload(ebb_solver* s, const(double)* q);        // q: nInterior x 4, cell order of mesh.d
+int ebb_state_download(ebb_solver* s, double* q);
+int ebb_residual(ebb_solver* s, int limit, int dtUpdate, double* R, double* newDt, double* Rmax);
+int ebb_step(ebb_solver* s, double* dt, double* Rmax);
+int ebb_run(ebb_solver* s, long maxSteps, double* t, double* dt, long* stepsDone);
+int ebb_history(ebb_solver* s, long first, long count, double* rows);   // 10 doubles per step
+int ebb_save_solution(ebb_solver* s, const(char)* path);
+int ebb_load_solution(ebb_solver* s, const(char)* path);
+int ebb_last_error(ebb_solver* s, int* kind, long* index, long* step, char* msg, size_t len);
+void ebb_solver_destroy(ebb_solver* s);
+void ebb_mesh_destroy(ebb_mesh* m);
+void ebb_config_destroy(ebb_config* c);
+
+// the rest of include/ebbcfd.h: queries, diagnostics and file output
+const(char)* ebb_error_string(int code);
+int ebb_mesh_info(const(ebb_mesh)* m, long* info);                          // 8 entries
+int ebb_mesh_cell_geometry(const(ebb_mesh)* m, double* area, double* d, double* perim, double* centroid,
+                           double* gradMat, uint* edges, byte* fluxMult, uint* neighbors, ubyte* nEdges);
+int ebb_mesh_edge_geometry(const(ebb_mesh)* m, uint* nodes, uint* cells, double* len, double* normal,
+                           double* mid, int* bgroup);
+int ebb_mesh_tile_stats(const(ebb_mesh)* m, int tileCells, int order, double* out8);
+int ebb_config_get(const(ebb_config)* c, const(char)* key, double* value);
+int ebb_config_set(ebb_config* c, const(char)* key, double value);
+int ebb_config_mesh_file(const(ebb_config)* c, char* buf, size_t len);
+int ebb_state_download_all(ebb_solver* s, double* q);                      // interior + ghosts
+int ebb_run_state(ebb_solver* s, double* t, double* dt, double* cfl, long* steps);
+int ebb_run_reset(ebb_solver* s, double t, double dt);
+long ebb_history_length(ebb_solver* s);
+int ebb_edge_states(ebb_solver* s, double* eq, double* flux, double* smax); // Edge.q[2], flux, sMax
+int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim);
+int ebb_save_solution_as(ebb_solver* s, const(char)* path, double t, double dt);
+int ebb_write_forces(ebb_solver* s, const(char)* path, long first, long count, int append);
+int ebb_time_steps(ebb_solver* s, int steps, double* out6);
+long ebb_solver_device_bytes(const(ebb_solver)* s);

@@ -43,12 +43,10 @@ def _declarations(text):
 
 
 def test_d_binding_declares_every_entry_point():
-    """The extern(C) D module in INTEGRATION.md names every function of the C ABI with
-    the header's parameter count."""
+    """The extern(C) D module include/ebbcfd.d names every function of the C ABI with the
+    header's parameter count."""
     header = _declarations(open(os.path.join(ROOT, "include", "ebbcfd.h")).read())
-    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    d_block = doc.split("```d\nmodule ebb.gpu;")[1].split("```")[0]
-    d = _declarations(d_block)
+    d = _declarations(open(os.path.join(ROOT, "include", "ebbcfd.d")).read())
     assert set(header) == set(_lib.SYMBOLS)
     assert d == header, {k: (header.get(k), d.get(k)) for k in set(header) | set(d) if header.get(k) != d.get(k)}
 
