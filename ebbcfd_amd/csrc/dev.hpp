@@ -192,10 +192,17 @@ __host__ __device__ inline TileDyn tileDyn(uint32_t nG, uint32_t nL, int LW)
 // computed by both, identically) and then updates its cells.  Gradients,
 // limiters and ghost states come from the gradient kernel.  Per tile, the face
 // references are stored contiguously.
+// Lanes of a face tile workgroup.  Tile t's cells start at t * cellStride and its faces at
+// t * faceStride, and the arrays carry 2 * FACE_TILE_LANES padding entries after the last
+// tile, so a workgroup's first loads (two rounds of cell ids, one of face records) need not
+// wait for the descriptor and stay in bounds.
+constexpr uint32_t FACE_TILE_LANES = 256;
+
 struct FaceTileArgs {
     uint32_t nTiles;
     uint32_t ldsBytes;
-    const uint4* desc;             // per tile: nC, nF, first cell, first face
+    uint32_t cellStride, faceStride;
+    const uint4* desc;             // per tile: nC, nF, first cell (t * cellStride), first face (t * faceStride)
     const uint32_t* ids;           // own cells' device ids
     const uint16_t* cf;            // S per own cell: local face, bit 15 set where the cell is not the owner;
                                    // nF (an all-zero slot) for padding

@@ -29,6 +29,7 @@ namespace ebb {
 #define DEV __device__ __forceinline__
 constexpr double G_ = 1.4;
 constexpr int BLK = 256;
+static_assert(uint32_t(BLK) == FACE_TILE_LANES, "face tile padding assumes one lane per workgroup thread");
 constexpr int FORCE_LDS = 1024;   // force faces staged per pass by a gradient kernel's force workgroup
 
 enum { BT_FULL = 0, BT_CONSTP, BT_INVWALL, BT_VISCWALL, BT_SYM, BT_TPIN, BT_DIRICHLET };
@@ -1309,16 +1310,17 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
 {
     if (a.checkActive && !a.sc->active) return;
     extern __shared__ double fsd[];
-    const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x);
-    const uint4 d = t.desc[tile];
-    const uint32_t nC = d.x, nF = d.y, tid = threadIdx.x;
-    // the first two cell rounds' ids are loaded ahead of the faces; the next round's
+    const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x), tid = threadIdx.x;
+    // The first two cell rounds' ids are loaded ahead of the faces; the next round's
     // face record is loaded one round ahead, so each round waits for one memory latency
-    // (cell data) instead of two (record, then cell data)
-    uint32_t giNext = tid < nC ? t.ids[d.z + tid] : 0u;
-    uint32_t giNext2 = tid + BLK < nC ? t.ids[d.z + tid + BLK] : 0u;
-    uint4 rNext = make_uint4(0, 0, 0, 0);
-    if (tid < nF) rNext = t.fref[d.w + tid];
+    // (cell data) instead of two (record, then cell data).  The tile's arrays start at
+    // fixed strides, so these first loads are issued beside the descriptor's instead of
+    // after it (lanes past the tile's counts read padding and do not use it).
+    uint32_t giNext = t.ids[size_t(tile) * t.cellStride + tid];
+    uint32_t giNext2 = t.ids[size_t(tile) * t.cellStride + tid + BLK];
+    uint4 rNext = t.fref[size_t(tile) * t.faceStride + tid];
+    const uint4 d = t.desc[tile];
+    const uint32_t nC = d.x, nF = d.y;
     for (uint32_t k = tid; k < nF; k += BLK) {
         const uint4 r = rNext;   // cell a, cell b, node 0, node 1
         if (k + BLK < nF) rNext = t.fref[d.w + k + BLK];

@@ -466,6 +466,8 @@ int buildFaceTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& 
     FaceTileArgs ft{};
     ft.nTiles = h.nTiles;
     ft.ldsBytes = lds;
+    ft.cellStride = h.cellStride;
+    ft.faceStride = h.faceStride;
     ft.desc = s->d_ftDesc.p; ft.ids = s->d_ftIds.p; ft.cf = s->d_ftCf.p; ft.fref = s->d_ftRef.p; ft.fid = s->d_ftFid.p;
     s->ftiles = ft;
     s->useFaceTiles = true;

@@ -311,6 +311,25 @@ bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const st
         h.maxF = std::max(h.maxF, nF);
         h.sumF += nF;
     }
+    // fixed strides (see FaceTileArgs): tile t moves to t * cellStride and t * faceStride,
+    // padding slots hold cell 0 and an all-zero face record (loaded, never used)
+    uint32_t CS = 0;
+    for (uint32_t t = 0; t < nT; t++) CS = std::max(CS, h.desc[t].x);
+    const uint32_t FS = std::max(h.maxF, 1u), pad = 2 * FACE_TILE_LANES;
+    std::vector<uint32_t> ids(size_t(nT) * CS + pad, 0u), fid(size_t(nT) * FS + pad, 0u);
+    std::vector<uint16_t> cf(size_t(nT) * CS * S + size_t(pad) * S, uint16_t(0));
+    std::vector<uint4> fref(size_t(nT) * FS + pad, make_uint4(0, 0, 0, 0));
+    for (uint32_t t = 0; t < nT; t++) {
+        const uint4 d = h.desc[t];
+        std::copy(h.ids.begin() + d.z, h.ids.begin() + d.z + d.x, ids.begin() + size_t(t) * CS);
+        std::copy(h.cf.begin() + size_t(d.z) * S, h.cf.begin() + size_t(d.z + d.x) * S, cf.begin() + size_t(t) * CS * S);
+        std::copy(h.fref.begin() + d.w, h.fref.begin() + d.w + d.y, fref.begin() + size_t(t) * FS);
+        std::copy(h.fid.begin() + d.w, h.fid.begin() + d.w + d.y, fid.begin() + size_t(t) * FS);
+        h.desc[t] = make_uint4(d.x, d.y, t * CS, t * FS);
+    }
+    h.ids.swap(ids); h.cf.swap(cf); h.fref.swap(fref); h.fid.swap(fid);
+    h.cellStride = CS;
+    h.faceStride = FS;
     return true;
 }
 

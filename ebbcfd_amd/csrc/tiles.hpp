@@ -44,6 +44,7 @@ struct FaceTileHost {
     std::vector<uint32_t> ids, fid;
     std::vector<uint16_t> cf;
     uint32_t nTiles = 0, maxF = 0;
+    uint32_t cellStride = 0, faceStride = 0;   // tile t: cells from t * cellStride, faces from t * faceStride
     uint64_t sumF = 0;
 };
 
