@@ -42,7 +42,7 @@ FACE_BYTES_PER_CELL = 144.0     # state 32 + gradient 64 + limiter 32 + centroid
 # commands are in profiles/hbm_traffic_cylinder_ns_4M.md.  Re-measured whenever
 # the kernel changes; reported only for the default configuration it was
 # measured on.
-MEASURED_TRAFFIC = {("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference"): 1.3699e9}
+MEASURED_TRAFFIC = {("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference"): 1.3674e9}
 
 
 def log(msg):
