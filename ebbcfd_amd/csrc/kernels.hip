@@ -1308,7 +1308,6 @@ DEV void edgeGeometry(bool interior, double2 p0, double2 p1, double2& n, double2
 template <int S, int FLUX, int VISC, int ORDER, int LP>
 __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
 {
-    if (a.checkActive && !a.sc->active) return;
     extern __shared__ double fsd[];
     const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x), tid = threadIdx.x;
     // The first two cell rounds' ids are loaded ahead of the faces; the next round's
@@ -1320,6 +1319,8 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
     uint32_t giNext2 = t.ids[size_t(tile) * t.cellStride + tid + BLK];
     uint4 rNext = t.fref[size_t(tile) * t.faceStride + tid];
     const uint4 d = t.desc[tile];
+    // the loop's active flag is read after those loads are issued (all in bounds)
+    if (a.checkActive && !a.sc->active) return;
     const uint32_t nC = d.x, nF = d.y;
     for (uint32_t k = tid; k < nF; k += BLK) {
         const uint4 r = rNext;   // cell a, cell b, node 0, node 1
