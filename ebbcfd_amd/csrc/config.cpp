@@ -34,6 +34,8 @@ struct JVal {
 struct Parser {
     const std::string& t;
     size_t p = 0;
+    int depth = 0;                       // nesting of the value being parsed
+    static constexpr int MAX_DEPTH = 64; // configs nest 3 deep; deeper input is rejected, not recursed into
     std::string err;
     explicit Parser(const std::string& s) : t(s) {}
     void ws() { while (p < t.size() && std::isspace((unsigned char)t[p])) p++; }
@@ -63,6 +65,12 @@ struct Parser {
     }
     bool val(JVal& v)
     {
+        if (depth >= MAX_DEPTH) return fail("nesting too deep");
+        struct Level {
+            int& d;
+            explicit Level(int& x) : d(x) { ++d; }
+            ~Level() { --d; }
+        } level(depth);
         ws();
         if (p >= t.size()) return fail("unexpected end");
         char c = t[p];

@@ -19,6 +19,7 @@
 #include <fstream>
 #include <sstream>
 #include <algorithm>
+#include <cerrno>
 #include <atomic>
 #include <thread>
 
@@ -314,50 +315,77 @@ bool parseGri(const std::string& path, RawMesh& out, std::string& err)
         }
         return false;
     };
+    // whole-token integer in [lo, hi], without exceptions
+    auto num = [&](size_t k, long lo, long hi, long& v) -> bool {
+        if (k >= tok.size()) return false;
+        const char* b = tok[k].c_str();
+        char* e = nullptr;
+        errno = 0;
+        const long x = std::strtol(b, &e, 10);
+        if (e == b || *e != '\0' || errno == ERANGE || x < lo || x > hi) return false;
+        v = x;
+        return true;
+    };
+    constexpr long MAXN = 0x7fffffffL;
     out = RawMesh();
-    if (!next() || tok.size() < 3) { err = "bad .gri header"; return false; }
-    const long nNodes = std::stol(tok[0]), nElems = std::stol(tok[1]), nDims = std::stol(tok[2]);
+    long nNodes = 0, nElems = 0, nDims = 0;
+    if (!next() || !num(0, 0, MAXN, nNodes) || !num(1, 0, MAXN, nElems) || !num(2, 0, MAXN, nDims)) {
+        err = "bad .gri header";
+        return false;
+    }
     if (nDims != 2) { err = std::to_string(nDims) + " dimensional meshes not supported"; return false; }
     for (long i = 0; i < nNodes; i++) {
         if (!next() || tok.size() < 2) { err = "Unexpected end of file in nodes"; return false; }
         out.nodes.push_back(std::strtod(tok[0].c_str(), nullptr));
         out.nodes.push_back(std::strtod(tok[1].c_str(), nullptr));
     }
-    if (!next()) { err = "missing boundary group count"; return false; }
-    const long nGroups = std::stol(tok[0]);
+    long nGroups = 0;
+    if (!next() || !num(0, 0, MAXN, nGroups)) { err = "missing boundary group count"; return false; }
     for (long g = 0; g < nGroups; g++) {
-        if (!next() || tok.size() < 2) { err = "bad boundary group header"; return false; }
-        const long nf = std::stol(tok[0]);
+        long nf = 0, npf = 0;
+        if (!next() || !num(0, 0, MAXN, nf) || !num(1, 2, 16, npf)) { err = "bad boundary group header"; return false; }
         if (tok.size() == 3) out.btags.push_back(tok[2]);
         out.bgroupStart.push_back(out.bnodes.size() / 2);
         for (long k = 0; k < nf; k++) {
-            if (!next() || tok.size() < 2) { err = "Unexpected end of file in boundary faces"; return false; }
-            out.bnodes.push_back(uint32_t(std::stol(tok.front()) - 1));
-            out.bnodes.push_back(uint32_t(std::stol(tok.back()) - 1));
+            long a = 0, b = 0;
+            if (!next() || tok.size() < 2 || !num(0, 1, nNodes, a) || !num(tok.size() - 1, 1, nNodes, b)) {
+                err = "bad boundary face";
+                return false;
+            }
+            out.bnodes.push_back(uint32_t(a - 1));
+            out.bnodes.push_back(uint32_t(b - 1));
         }
     }
     out.eoff.assign(1, 0);
     long found = 0;
     int faces = 0, firstFaces = 0;
     while (found < nElems) {
-        if (!next() || tok.size() < 3) { err = "bad element group header"; return false; }
-        const long sub = std::stol(tok[0]);
-        const long q = std::stol(tok[1]);
+        long sub = 0, q = 0;
+        if (!next() || tok.size() < 3 || !num(0, 0, MAXN, sub) || !num(1, 0, MAXN, q)) {
+            err = "bad element group header";
+            return false;
+        }
         if (q >= 4 || q == 0) { err = "Unsuported q"; return false; }
         if (tok[2].find("Tri") != std::string::npos) faces = 3;
         else if (tok[2].find("Quad") != std::string::npos) faces = 4;
         else { err = "Unsuported cell type"; return false; }
         if (firstFaces == 0) firstFaces = faces;
         if (faces != firstFaces) { err = "mixed triangle/quad .gri meshes are not supported"; return false; }
+        // nodes per element of a Lagrange triangle / quadrilateral of order q
+        const size_t need = faces == 3 ? size_t((q + 1) * (q + 2) / 2) : size_t((q + 1) * (q + 1));
         for (long i = 0; i < sub; i++) {
             if (!next()) { err = "Unexpected end of file in elements"; return false; }
-            std::vector<uint32_t> e;
-            for (auto& t : tok) e.push_back(uint32_t(std::stol(t)));
+            if (tok.size() != need) { err = "element has the wrong number of nodes"; return false; }
+            std::vector<uint32_t> e(need);
+            for (size_t k = 0; k < need; k++) {
+                long v = 0;
+                if (!num(k, 1, nNodes, v)) { err = "bad element node"; return false; }
+                e[k] = uint32_t(v);
+            }
             std::vector<uint32_t> c;
             if (q == 1) c = e;
             else if (q == 2) c = faces == 3 ? std::vector<uint32_t>{e[0], e[2], e[5]} : std::vector<uint32_t>{e[0], e[2], e[8], e[6]};
             else c = faces == 3 ? std::vector<uint32_t>{e[0], e[3], e[9]} : std::vector<uint32_t>{e[0], e[3], e[15], e[12]};
-            if (int(c.size()) != faces) { err = "element has the wrong number of nodes"; return false; }
             out.enodes.insert(out.enodes.end(), c.begin(), c.end());
             out.eoff.push_back(uint32_t(out.enodes.size()));
         }

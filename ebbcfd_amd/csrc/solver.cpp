@@ -711,6 +711,24 @@ int buildDevice(ebb_solver* s)
 
 }  // namespace
 
+// Every C entry point that parses, allocates or touches the device runs its body
+// through guarded(): an exception (std::bad_alloc, a length_error from a size read
+// out of a file, ...) becomes a status code instead of unwinding into the caller.
+template <typename F>
+auto guarded(F&& f) -> decltype(f())
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return EBB_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ebbcfd: %s\n", e.what());
+        return EBB_ERR_PARSE;
+    } catch (...) {
+        return EBB_ERR_PARSE;
+    }
+}
+
 // ==================================================================== C ABI
 extern "C" {
 
@@ -727,45 +745,50 @@ const char* ebb_error_string(int code)
     case EBB_ERR_EDGE_NAN: return "NaN flux on edge";
     case EBB_ERR_CELL_NAN: return "NaN state in cell";
     case EBB_ERR_STATE: return "invalid solver state";
+    case EBB_ERR_ALLOC: return "out of memory";
     }
     return "unknown";
 }
 
 int ebb_mesh_load(const char* path, ebb_mesh** out)
 {
-    if (!path || !out) return EBB_ERR_ARG;
-    std::string p(path);
-    const bool gri = p.find(".gri") != std::string::npos;   // io.d:673-684 dispatch order
-    if (!gri && p.find(".msh") == std::string::npos && p.find(".gmsh") == std::string::npos) return EBB_ERR_PARSE;
-    auto m = std::make_unique<ebb_mesh>();
-    std::string err;
-    {
-        std::ifstream f(p);
-        if (!f) return EBB_ERR_IO;
-    }
-    const bool ok = gri ? parseGri(p, m->raw, err) : parseGmsh(p, m->raw, err);
-    if (!ok) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_PARSE; }
-    if (!buildMesh(m->raw, m->b, err)) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_MESH; }
-    *out = m.release();
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!path || !out) return EBB_ERR_ARG;
+        std::string p(path);
+        const bool gri = p.find(".gri") != std::string::npos;   // io.d:673-684 dispatch order
+        if (!gri && p.find(".msh") == std::string::npos && p.find(".gmsh") == std::string::npos) return EBB_ERR_PARSE;
+        auto m = std::make_unique<ebb_mesh>();
+        std::string err;
+        {
+            std::ifstream f(p);
+            if (!f) return EBB_ERR_IO;
+        }
+        const bool ok = gri ? parseGri(p, m->raw, err) : parseGmsh(p, m->raw, err);
+        if (!ok) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_PARSE; }
+        if (!buildMesh(m->raw, m->b, err)) { std::fprintf(stderr, "ebb_mesh_load: %s\n", err.c_str()); return EBB_ERR_MESH; }
+        *out = m.release();
+        return EBB_OK;
+    });
 }
 
 int ebb_mesh_create(uint32_t n_nodes, const double* xy, uint32_t n_elem, const uint32_t* eoff, const uint32_t* enodes,
                     uint32_t n_bnodes, const uint32_t* bnodes, uint32_t n_tags, const char* const* tags,
                     const uint64_t* bgroup_start, ebb_mesh** out)
 {
-    if (!out || !xy || !eoff || !enodes || (n_bnodes && !bnodes) || (n_tags && (!tags || !bgroup_start))) return EBB_ERR_ARG;
-    auto m = std::make_unique<ebb_mesh>();
-    m->raw.nodes.assign(xy, xy + 2 * size_t(n_nodes));
-    m->raw.eoff.assign(eoff, eoff + n_elem + 1);
-    m->raw.enodes.assign(enodes, enodes + eoff[n_elem]);
-    m->raw.bnodes.assign(bnodes, bnodes + 2 * size_t(n_bnodes));
-    for (uint32_t t = 0; t < n_tags; t++) m->raw.btags.emplace_back(tags[t]);
-    m->raw.bgroupStart.assign(bgroup_start, bgroup_start + n_tags);
-    std::string err;
-    if (!buildMesh(m->raw, m->b, err)) { std::fprintf(stderr, "ebb_mesh_create: %s\n", err.c_str()); return EBB_ERR_MESH; }
-    *out = m.release();
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!out || !xy || !eoff || !enodes || (n_bnodes && !bnodes) || (n_tags && (!tags || !bgroup_start))) return EBB_ERR_ARG;
+        auto m = std::make_unique<ebb_mesh>();
+        m->raw.nodes.assign(xy, xy + 2 * size_t(n_nodes));
+        m->raw.eoff.assign(eoff, eoff + n_elem + 1);
+        m->raw.enodes.assign(enodes, enodes + eoff[n_elem]);
+        m->raw.bnodes.assign(bnodes, bnodes + 2 * size_t(n_bnodes));
+        for (uint32_t t = 0; t < n_tags; t++) m->raw.btags.emplace_back(tags[t]);
+        m->raw.bgroupStart.assign(bgroup_start, bgroup_start + n_tags);
+        std::string err;
+        if (!buildMesh(m->raw, m->b, err)) { std::fprintf(stderr, "ebb_mesh_create: %s\n", err.c_str()); return EBB_ERR_MESH; }
+        *out = m.release();
+        return EBB_OK;
+    });
 }
 
 int ebb_mesh_info(const ebb_mesh* m, int64_t info[8])
@@ -783,71 +806,79 @@ int ebb_mesh_info(const ebb_mesh* m, int64_t info[8])
 int ebb_mesh_cell_geometry(const ebb_mesh* m, double* area, double* d, double* perim, double* centroid,
                            double* grad_mat, uint32_t* edges, int8_t* flux_mult, uint32_t* neighbors, uint8_t* n_edges)
 {
-    if (!m) return EBB_ERR_ARG;
-    const BuiltMesh& b = m->b;
-    for (uint32_t i = 0; i < b.nCells; i++) {
-        if (area) area[i] = b.area[i];
-        if (d) d[i] = b.d[i];
-        if (perim) perim[i] = b.perim[i];
-        if (centroid) { centroid[2 * i] = b.cx[i]; centroid[2 * i + 1] = b.cy[i]; }
-    }
-    if (grad_mat) std::memcpy(grad_mat, b.gradMat.data(), b.gradMat.size() * sizeof(double));
-    if (edges) std::memcpy(edges, b.cellEdges.data(), b.cellEdges.size() * sizeof(uint32_t));
-    if (flux_mult) std::memcpy(flux_mult, b.fluxMult.data(), b.fluxMult.size());
-    if (neighbors) std::memcpy(neighbors, b.nbr.data(), b.nbr.size() * sizeof(uint32_t));
-    if (n_edges) std::memcpy(n_edges, b.nEdgesCell.data(), b.nEdgesCell.size());
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!m) return EBB_ERR_ARG;
+        const BuiltMesh& b = m->b;
+        for (uint32_t i = 0; i < b.nCells; i++) {
+            if (area) area[i] = b.area[i];
+            if (d) d[i] = b.d[i];
+            if (perim) perim[i] = b.perim[i];
+            if (centroid) { centroid[2 * i] = b.cx[i]; centroid[2 * i + 1] = b.cy[i]; }
+        }
+        if (grad_mat) std::memcpy(grad_mat, b.gradMat.data(), b.gradMat.size() * sizeof(double));
+        if (edges) std::memcpy(edges, b.cellEdges.data(), b.cellEdges.size() * sizeof(uint32_t));
+        if (flux_mult) std::memcpy(flux_mult, b.fluxMult.data(), b.fluxMult.size());
+        if (neighbors) std::memcpy(neighbors, b.nbr.data(), b.nbr.size() * sizeof(uint32_t));
+        if (n_edges) std::memcpy(n_edges, b.nEdgesCell.data(), b.nEdgesCell.size());
+        return EBB_OK;
+    });
 }
 
 int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, double* len, double* normal,
                            double* mid, int32_t* bgroup)
 {
-    if (!m) return EBB_ERR_ARG;
-    const BuiltMesh& b = m->b;
-    for (uint32_t e = 0; e < b.nEdges; e++) {
-        if (nodes) { nodes[2 * e] = b.eNode0[e]; nodes[2 * e + 1] = b.eNode1[e]; }
-        if (cells) { cells[2 * e] = b.eCell0[e]; cells[2 * e + 1] = b.eCell1[e]; }
-        if (len) len[e] = b.len[e];
-        if (normal) { normal[2 * e] = b.nx[e]; normal[2 * e + 1] = b.ny[e]; }
-        if (mid) { mid[2 * e] = b.mx[e]; mid[2 * e + 1] = b.my[e]; }
-        if (bgroup) bgroup[e] = b.eBGroup[e];
-    }
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!m) return EBB_ERR_ARG;
+        const BuiltMesh& b = m->b;
+        for (uint32_t e = 0; e < b.nEdges; e++) {
+            if (nodes) { nodes[2 * e] = b.eNode0[e]; nodes[2 * e + 1] = b.eNode1[e]; }
+            if (cells) { cells[2 * e] = b.eCell0[e]; cells[2 * e + 1] = b.eCell1[e]; }
+            if (len) len[e] = b.len[e];
+            if (normal) { normal[2 * e] = b.nx[e]; normal[2 * e + 1] = b.ny[e]; }
+            if (mid) { mid[2 * e] = b.mx[e]; mid[2 * e + 1] = b.my[e]; }
+            if (bgroup) bgroup[e] = b.eBGroup[e];
+        }
+        return EBB_OK;
+    });
 }
 
 int ebb_mesh_tile_stats(const ebb_mesh* m, int tile_cells, int order, double out[8])
 {
-    if (!m || !out || tile_cells <= 0 || tile_cells > 1024 || order < 0 || order > 2) return EBB_ERR_ARG;
-    Layout L;
-    makeLayout(m->b, L);
-    TileHost h;
-    std::string err;
-    std::vector<uint32_t> ord, start;
-    tileOrder(L, uint32_t(tile_cells), order, ord, start);
-    if (!makeTiles(L, ord, start, h, err)) {
-        std::fprintf(stderr, "ebb_mesh_tile_stats: %s\n", err.c_str());
-        return EBB_ERR_MESH;
-    }
-    const double n = L.N ? double(L.N) : 1.0, nt = h.nTiles ? double(h.nTiles) : 1.0;
-    out[0] = double(h.nTiles);
-    out[1] = double(h.sumC) / nt;
-    out[2] = double(h.sumG) / n; out[3] = double(h.sumQ) / n; out[4] = double(h.sumL) / n;
-    out[5] = double(h.sumF) / n;
-    out[6] = h.maxL;
-    out[7] = 2.0 * h.maxBlob + h.maxDyn[0];
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!m || !out || tile_cells <= 0 || tile_cells > 1024 || order < 0 || order > 2) return EBB_ERR_ARG;
+        Layout L;
+        makeLayout(m->b, L);
+        TileHost h;
+        std::string err;
+        std::vector<uint32_t> ord, start;
+        tileOrder(L, uint32_t(tile_cells), order, ord, start);
+        if (!makeTiles(L, ord, start, h, err)) {
+            std::fprintf(stderr, "ebb_mesh_tile_stats: %s\n", err.c_str());
+            return EBB_ERR_MESH;
+        }
+        const double n = L.N ? double(L.N) : 1.0, nt = h.nTiles ? double(h.nTiles) : 1.0;
+        out[0] = double(h.nTiles);
+        out[1] = double(h.sumC) / nt;
+        out[2] = double(h.sumG) / n; out[3] = double(h.sumQ) / n; out[4] = double(h.sumL) / n;
+        out[5] = double(h.sumF) / n;
+        out[6] = h.maxL;
+        out[7] = 2.0 * h.maxBlob + h.maxDyn[0];
+        return EBB_OK;
+    });
 }
 
 void ebb_mesh_destroy(ebb_mesh* m) { delete m; }
 
 int ebb_config_parse(const char* json, ebb_config** out)
 {
-    if (!json || !out) return EBB_ERR_ARG;
-    auto c = std::make_unique<ebb_config>();
-    std::string err;
-    if (!parseConfig(json, c->c, err)) { std::fprintf(stderr, "ebb_config_parse: %s\n", err.c_str()); return EBB_ERR_PARSE; }
-    *out = c.release();
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!json || !out) return EBB_ERR_ARG;
+        auto c = std::make_unique<ebb_config>();
+        std::string err;
+        if (!parseConfig(json, c->c, err)) { std::fprintf(stderr, "ebb_config_parse: %s\n", err.c_str()); return EBB_ERR_PARSE; }
+        *out = c.release();
+        return EBB_OK;
+    });
 }
 
 static double* cfgField(Config& c, const std::string& k, bool& isInt, long long*& ip, bool*& bp)
@@ -880,30 +911,34 @@ static double* cfgField(Config& c, const std::string& k, bool& isInt, long long*
 
 int ebb_config_get(const ebb_config* c, const char* key, double* value)
 {
-    if (!c || !key || !value) return EBB_ERR_ARG;
-    bool isInt; long long* ip; bool* bp;
-    double* dp = cfgField(const_cast<Config&>(c->c), key, isInt, ip, bp);
-    if (dp) *value = *dp;
-    else if (ip) *value = double(*ip);
-    else if (bp) *value = *bp ? 1.0 : 0.0;
-    else if (std::string(key) == "precision") *value = c->c.precision;
-    else if (std::string(key) == "integrator") *value = c->c.integratorKind();
-    else if (std::string(key) == "flux") *value = c->c.fluxKind();
-    else return EBB_ERR_ARG;
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!c || !key || !value) return EBB_ERR_ARG;
+        bool isInt; long long* ip; bool* bp;
+        double* dp = cfgField(const_cast<Config&>(c->c), key, isInt, ip, bp);
+        if (dp) *value = *dp;
+        else if (ip) *value = double(*ip);
+        else if (bp) *value = *bp ? 1.0 : 0.0;
+        else if (std::string(key) == "precision") *value = c->c.precision;
+        else if (std::string(key) == "integrator") *value = c->c.integratorKind();
+        else if (std::string(key) == "flux") *value = c->c.fluxKind();
+        else return EBB_ERR_ARG;
+        return EBB_OK;
+    });
 }
 
 int ebb_config_set(ebb_config* c, const char* key, double value)
 {
-    if (!c || !key) return EBB_ERR_ARG;
-    bool isInt; long long* ip; bool* bp;
-    double* dp = cfgField(c->c, key, isInt, ip, bp);
-    if (dp) *dp = value;
-    else if (ip) *ip = (long long)value;
-    else if (bp) *bp = value != 0.0;
-    else if (std::string(key) == "precision") c->c.precision = int(value);
-    else return EBB_ERR_ARG;
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!c || !key) return EBB_ERR_ARG;
+        bool isInt; long long* ip; bool* bp;
+        double* dp = cfgField(c->c, key, isInt, ip, bp);
+        if (dp) *dp = value;
+        else if (ip) *ip = (long long)value;
+        else if (bp) *bp = value != 0.0;
+        else if (std::string(key) == "precision") c->c.precision = int(value);
+        else return EBB_ERR_ARG;
+        return EBB_OK;
+    });
 }
 
 int ebb_config_mesh_file(const ebb_config* c, char* buf, size_t len)
@@ -919,30 +954,31 @@ void ebb_solver_destroy(ebb_solver* s);
 
 int ebb_solver_create(const ebb_mesh* m, const ebb_config* c, int device, ebb_solver** out)
 {
-    if (!m || !c || !out) return EBB_ERR_ARG;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
-        std::fprintf(stderr, "ebb_solver_create: no HIP device %d\n", device);
-        return EBB_ERR_DEVICE;
-    }
-    auto s = new ebb_solver();
-    s->raw = m->raw;
-    s->b = m->b;
-    s->cfg = c->c;
-    s->device = device;
-    if (s->cfg.precision != 0) {
-        delete s;
-        std::fprintf(stderr, "ebb_solver_create: mixed precision is not available in this library build\n");
-        return EBB_ERR_ARG;
-    }
-    int rc = buildDevice(s);
-    if (rc != EBB_OK) {
-        std::fprintf(stderr, "ebb_solver_create: %s %s\n", s->errMsg.c_str(), lastDeviceError.c_str());
-        ebb_solver_destroy(s);
-        return rc;
-    }
-    *out = s;
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!m || !c || !out) return EBB_ERR_ARG;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+            std::fprintf(stderr, "ebb_solver_create: no HIP device %d\n", device);
+            return EBB_ERR_DEVICE;
+        }
+        // released to the caller on success; destroyed (device buffers included) on any failure
+        std::unique_ptr<ebb_solver, void (*)(ebb_solver*)> s(new ebb_solver(), ebb_solver_destroy);
+        s->raw = m->raw;
+        s->b = m->b;
+        s->cfg = c->c;
+        s->device = device;
+        if (s->cfg.precision != 0) {
+            std::fprintf(stderr, "ebb_solver_create: mixed precision is not available in this library build\n");
+            return EBB_ERR_ARG;
+        }
+        int rc = buildDevice(s.get());
+        if (rc != EBB_OK) {
+            std::fprintf(stderr, "ebb_solver_create: %s %s\n", s->errMsg.c_str(), lastDeviceError.c_str());
+            return rc;
+        }
+        *out = s.release();
+        return EBB_OK;
+    });
 }
 
 // Copy an N-row array (rows of `per` doubles) between host (reference numbering)
@@ -979,179 +1015,197 @@ static int downloadCells(ebb_solver* s, const void* devPtr, double* host, size_t
 
 int ebb_state_upload(ebb_solver* s, const double* q)
 {
-    if (!s || !q) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    return uploadCells(s, q, s->qcur(), 4);
+    return guarded([&]() -> int {
+        if (!s || !q) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        return uploadCells(s, q, s->qcur(), 4);
+    });
 }
 
 int ebb_state_download(ebb_solver* s, double* q)
 {
-    if (!s || !q) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    return downloadCells(s, s->qcur(), q, 4);
+    return guarded([&]() -> int {
+        if (!s || !q) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        return downloadCells(s, s->qcur(), q, 4);
+    });
 }
 
 int ebb_state_download_all(ebb_solver* s, double* q)
 {
-    if (!s || !q) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    int rc = downloadCells(s, s->qcur(), q, 4);
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(q + size_t(s->N) * 4, s->qcur() + s->N, size_t(s->G) * sizeof(double4), hipMemcpyDeviceToHost,
-                          s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!s || !q) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        int rc = downloadCells(s, s->qcur(), q, 4);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(q + size_t(s->N) * 4, s->qcur() + s->N, size_t(s->G) * sizeof(double4), hipMemcpyDeviceToHost,
+                              s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return EBB_OK;
+    });
 }
 
 int ebb_residual(ebb_solver* s, int limit, int dt_update, double* R, double* new_dt, double* rmax)
 {
-    if (!s) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    if (!s->d_R.p) HIPCHK((hipError_t)trackAlloc(s, s->d_R, s->N));
-    if (!s->d_estate.p) HIPCHK((hipError_t)trackAlloc(s, s->d_estate, size_t(s->Fi + s->Fb) * 2));
-    { int rc_ = readScalars(s); if (rc_) return rc_; }
-    s->h_sc->rmaxBits = 0;
-    if (rmax && *rmax > 0) std::memcpy(&s->h_sc->rmaxBits, rmax, 8);
-    beginErrorWindow(s);
-    s->h_sc->errEdge = ~0ull;
-    s->h_sc->errCell = ~0ull;
-    s->h_sc->errBC = ~0ull;
-    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
-    stage(s, SK_RESIDUAL, s->qcur(), s->qcur(), nullptr, limit, dt_update, 0, 0, 0, s->d_R.p, s->d_estate.p, 0);
-    HIPCHK(hipGetLastError());
-    { int rc_ = readScalars(s); if (rc_) return rc_; }
-    if (R) { int rc_ = downloadCells(s, s->d_R.p, R, 4); if (rc_) return rc_; }
-    double nd, rm;
-    std::memcpy(&nd, &s->h_sc->newDtBits, 8);
-    std::memcpy(&rm, &s->h_sc->rmaxBits, 8);
-    if (new_dt) *new_dt = nd;
-    if (rmax) *rmax = rm;
-    return checkErrors(s);
+    return guarded([&]() -> int {
+        if (!s) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        if (!s->d_R.p) HIPCHK((hipError_t)trackAlloc(s, s->d_R, s->N));
+        if (!s->d_estate.p) HIPCHK((hipError_t)trackAlloc(s, s->d_estate, size_t(s->Fi + s->Fb) * 2));
+        { int rc_ = readScalars(s); if (rc_) return rc_; }
+        s->h_sc->rmaxBits = 0;
+        if (rmax && *rmax > 0) std::memcpy(&s->h_sc->rmaxBits, rmax, 8);
+        beginErrorWindow(s);
+        s->h_sc->errEdge = ~0ull;
+        s->h_sc->errCell = ~0ull;
+        s->h_sc->errBC = ~0ull;
+        HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+        stage(s, SK_RESIDUAL, s->qcur(), s->qcur(), nullptr, limit, dt_update, 0, 0, 0, s->d_R.p, s->d_estate.p, 0);
+        HIPCHK(hipGetLastError());
+        { int rc_ = readScalars(s); if (rc_) return rc_; }
+        if (R) { int rc_ = downloadCells(s, s->d_R.p, R, 4); if (rc_) return rc_; }
+        double nd, rm;
+        std::memcpy(&nd, &s->h_sc->newDtBits, 8);
+        std::memcpy(&rm, &s->h_sc->rmaxBits, 8);
+        if (new_dt) *new_dt = nd;
+        if (rmax) *rmax = rm;
+        return checkErrors(s);
+    });
 }
 
 int ebb_step(ebb_solver* s, double* dt, double* rmax)
 {
-    if (!s || !dt) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    { int rc_ = readScalars(s); if (rc_) return rc_; }
-    DevScalars saved = *s->h_sc;
-    s->h_sc->dt = *dt;
-    beginErrorWindow(s);
-    s->h_sc->errEdge = ~0ull;
-    s->h_sc->errCell = ~0ull;
-    s->h_sc->errBC = ~0ull;
-    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
-    stepStages(s, 0, uint32_t(s->stepsTotal));
-    HIPCHK(hipGetLastError());
-    { int rc_ = readScalars(s); if (rc_) return rc_; }
-    s->cur ^= 1;
-    double nd, rm;
-    std::memcpy(&nd, &s->h_sc->newDtBits, 8);
-    std::memcpy(&rm, &s->h_sc->rmaxBits, 8);
-    *dt = nd;
-    if (rmax) *rmax = rm;
-    int rc = checkErrors(s);
-    // the run loop's time/dt are not advanced by a bare integrator step
-    s->h_sc->dt = saved.dt;
-    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return rc;
+    return guarded([&]() -> int {
+        if (!s || !dt) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        { int rc_ = readScalars(s); if (rc_) return rc_; }
+        DevScalars saved = *s->h_sc;
+        s->h_sc->dt = *dt;
+        beginErrorWindow(s);
+        s->h_sc->errEdge = ~0ull;
+        s->h_sc->errCell = ~0ull;
+        s->h_sc->errBC = ~0ull;
+        HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+        stepStages(s, 0, uint32_t(s->stepsTotal));
+        HIPCHK(hipGetLastError());
+        { int rc_ = readScalars(s); if (rc_) return rc_; }
+        s->cur ^= 1;
+        double nd, rm;
+        std::memcpy(&nd, &s->h_sc->newDtBits, 8);
+        std::memcpy(&rm, &s->h_sc->rmaxBits, 8);
+        *dt = nd;
+        if (rmax) *rmax = rm;
+        int rc = checkErrors(s);
+        // the run loop's time/dt are not advanced by a bare integrator step
+        s->h_sc->dt = saved.dt;
+        HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return rc;
+    });
 }
 
 int ebb_run(ebb_solver* s, int64_t max_steps, double* t, double* dt, int64_t* steps_done)
 {
-    if (!s) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    { int rc_ = readScalars(s); if (rc_) return rc_; }
-    int64_t done = 0;
-    int rc = EBB_OK;
-    if (s->h_sc->errEdge != ~0ull || s->h_sc->errCell != ~0ull || s->h_sc->errBC != ~0ull) return fail(s, EBB_ERR_STATE, "solver stopped on an error");
-    if (!(s->h_sc->t < s->cfg.tEnd)) goto out;
-    {
-        // (re)activate: the device stops itself at tEnd, after maxSteps or on an error
-        const int64_t cfgMax = s->cfg.maxSteps;
-        int64_t limit = max_steps;
-        if (cfgMax >= 0) {
-            const int64_t left = std::max<int64_t>(0, cfgMax - s->stepsTotal);
-            limit = (limit < 0) ? left : std::min(limit, left);
-        }
-        if (limit == 0) goto out;
-        s->h_sc->active = 1;
-        HIPCHK(hipMemcpyAsync(&s->d_sc->active, &s->h_sc->active, sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
-        while (limit < 0 || done < limit) {
-            const int64_t batch = limit < 0 ? s->histCap : std::min<int64_t>(s->histCap, limit - done);
-            const int64_t base = s->h_sc->steps;
-            beginErrorWindow(s);   // no error is pending: the previous batch would have stopped
-            const int64_t stopAt = base + batch;
-            int c0 = s->cur;
-            for (int64_t k = 0; k < batch; k++) {
-                const double4* qold = s->qcur();
-                stepStages(s, 1, uint32_t(base + k));
-                const double4* qnew = s->qnext();
-                DiagArgs d = diagArgs(s, qnew, qold, base, stopAt);
-                launchRmsChunks(d, s->stream);
-                launchFinalize(d, s->stream);
-                s->cur ^= 1;
+    return guarded([&]() -> int {
+        if (!s) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        { int rc_ = readScalars(s); if (rc_) return rc_; }
+        int64_t done = 0;
+        int rc = EBB_OK;
+        if (s->h_sc->errEdge != ~0ull || s->h_sc->errCell != ~0ull || s->h_sc->errBC != ~0ull) return fail(s, EBB_ERR_STATE, "solver stopped on an error");
+        if (!(s->h_sc->t < s->cfg.tEnd)) goto out;
+        {
+            // (re)activate: the device stops itself at tEnd, after maxSteps or on an error
+            const int64_t cfgMax = s->cfg.maxSteps;
+            int64_t limit = max_steps;
+            if (cfgMax >= 0) {
+                const int64_t left = std::max<int64_t>(0, cfgMax - s->stepsTotal);
+                limit = (limit < 0) ? left : std::min(limit, left);
             }
-            // the batch's last step (or the one that ended the loop) still has its force pending
-            if (deferForce(s)) launchPendingForce(forceArgs(s), s->d_sc, s->stream);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(s->h_hist, s->d_hist.p, sizeof(double) * size_t(batch) * HIST_FIELDS,
-                                  hipMemcpyDeviceToHost, s->stream));
-            { int rc_ = readScalars(s); if (rc_) return rc_; }
-            const int64_t ran = s->h_sc->steps - base;
-            s->cur = (c0 ^ int(ran & 1));
-            s->history.insert(s->history.end(), s->h_hist, s->h_hist + ran * HIST_FIELDS);
-            s->stepsTotal += ran;
-            done += ran;
-            rc = checkErrors(s);
-            if (rc != EBB_OK) break;
-            if (ran < batch) break;              // t >= tEnd
-            if (!(s->h_sc->t < s->cfg.tEnd)) break;
+            if (limit == 0) goto out;
             s->h_sc->active = 1;
             HIPCHK(hipMemcpyAsync(&s->d_sc->active, &s->h_sc->active, sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+            while (limit < 0 || done < limit) {
+                const int64_t batch = limit < 0 ? s->histCap : std::min<int64_t>(s->histCap, limit - done);
+                const int64_t base = s->h_sc->steps;
+                beginErrorWindow(s);   // no error is pending: the previous batch would have stopped
+                const int64_t stopAt = base + batch;
+                int c0 = s->cur;
+                for (int64_t k = 0; k < batch; k++) {
+                    const double4* qold = s->qcur();
+                    stepStages(s, 1, uint32_t(base + k));
+                    const double4* qnew = s->qnext();
+                    DiagArgs d = diagArgs(s, qnew, qold, base, stopAt);
+                    launchRmsChunks(d, s->stream);
+                    launchFinalize(d, s->stream);
+                    s->cur ^= 1;
+                }
+                // the batch's last step (or the one that ended the loop) still has its force pending
+                if (deferForce(s)) launchPendingForce(forceArgs(s), s->d_sc, s->stream);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(s->h_hist, s->d_hist.p, sizeof(double) * size_t(batch) * HIST_FIELDS,
+                                      hipMemcpyDeviceToHost, s->stream));
+                { int rc_ = readScalars(s); if (rc_) return rc_; }
+                const int64_t ran = s->h_sc->steps - base;
+                s->cur = (c0 ^ int(ran & 1));
+                s->history.insert(s->history.end(), s->h_hist, s->h_hist + ran * HIST_FIELDS);
+                s->stepsTotal += ran;
+                done += ran;
+                rc = checkErrors(s);
+                if (rc != EBB_OK) break;
+                if (ran < batch) break;              // t >= tEnd
+                if (!(s->h_sc->t < s->cfg.tEnd)) break;
+                s->h_sc->active = 1;
+                HIPCHK(hipMemcpyAsync(&s->d_sc->active, &s->h_sc->active, sizeof(int32_t), hipMemcpyHostToDevice, s->stream));
+            }
         }
-    }
-out:
-    if (t) *t = s->h_sc->t;
-    if (dt) *dt = s->h_sc->dt;
-    if (steps_done) *steps_done = done;
-    return rc;
+    out:
+        if (t) *t = s->h_sc->t;
+        if (dt) *dt = s->h_sc->dt;
+        if (steps_done) *steps_done = done;
+        return rc;
+    });
 }
 
 int ebb_run_state(ebb_solver* s, double* t, double* dt, double* cfl, int64_t* steps)
 {
-    if (!s) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    { int rc_ = readScalars(s); if (rc_) return rc_; }
-    if (t) *t = s->h_sc->t;
-    if (dt) *dt = s->h_sc->dt;
-    if (cfl) *cfl = s->h_sc->CFL;
-    if (steps) *steps = s->stepsTotal;
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!s) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        { int rc_ = readScalars(s); if (rc_) return rc_; }
+        if (t) *t = s->h_sc->t;
+        if (dt) *dt = s->h_sc->dt;
+        if (cfl) *cfl = s->h_sc->CFL;
+        if (steps) *steps = s->stepsTotal;
+        return EBB_OK;
+    });
 }
 
 int ebb_run_reset(ebb_solver* s, double t, double dt)
 {
-    if (!s) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    launchInitScalars(s->d_sc, dt, t, s->cfg.CFL, s->stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s->stream));
-    s->history.clear();
-    s->stepsTotal = 0;
-    s->errKind = 0;
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!s) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        launchInitScalars(s->d_sc, dt, t, s->cfg.CFL, s->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->history.clear();
+        s->stepsTotal = 0;
+        s->errKind = 0;
+        return EBB_OK;
+    });
 }
 
 int64_t ebb_history_length(ebb_solver* s) { return s ? int64_t(s->history.size() / HIST_FIELDS) : 0; }
 
 int ebb_history(ebb_solver* s, int64_t first, int64_t count, double* out)
 {
-    if (!s || !out || first < 0 || count < 0) return EBB_ERR_ARG;
-    if ((first + count) * HIST_FIELDS > int64_t(s->history.size())) return EBB_ERR_ARG;
-    std::memcpy(out, s->history.data() + first * HIST_FIELDS, sizeof(double) * size_t(count) * HIST_FIELDS);
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!s || !out || first < 0 || count < 0) return EBB_ERR_ARG;
+        if ((first + count) * HIST_FIELDS > int64_t(s->history.size())) return EBB_ERR_ARG;
+        std::memcpy(out, s->history.data() + first * HIST_FIELDS, sizeof(double) * size_t(count) * HIST_FIELDS);
+        return EBB_OK;
+    });
 }
 
 // Recompute the edge states of the last residual evaluation into d_estate.
@@ -1179,63 +1233,72 @@ static int refreshEdgeStates(ebb_solver* s)
 
 int ebb_edge_states(ebb_solver* s, double* eq, double* flux, double* smax)
 {
-    if (!s) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    int rc = refreshEdgeStates(s);
-    if (rc) return rc;
-    const uint32_t F = s->Fi + s->Fb;
-    std::vector<double4> es(size_t(F) * 2);
-    std::vector<double> lf(size_t(F) * 4), ls(F);
-    HIPCHK(hipMemcpy(es.data(), s->d_estate.p, es.size() * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(lf.data(), s->d_lenF.p, lf.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ls.data(), s->d_lenS.p, ls.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint32_t f = 0; f < F; f++) {
-        const uint32_t e = s->edgeOfFace[f];
-        if (eq) {
-            const double4 a = es[2 * f], b = es[2 * f + 1];
-            double* o = eq + size_t(e) * 8;
-            o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+    return guarded([&]() -> int {
+        if (!s) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        int rc = refreshEdgeStates(s);
+        if (rc) return rc;
+        const uint32_t F = s->Fi + s->Fb;
+        std::vector<double4> es(size_t(F) * 2);
+        std::vector<double> lf(size_t(F) * 4), ls(F);
+        HIPCHK(hipMemcpy(es.data(), s->d_estate.p, es.size() * sizeof(double4), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(lf.data(), s->d_lenF.p, lf.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ls.data(), s->d_lenS.p, ls.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (uint32_t f = 0; f < F; f++) {
+            const uint32_t e = s->edgeOfFace[f];
+            if (eq) {
+                const double4 a = es[2 * f], b = es[2 * f + 1];
+                double* o = eq + size_t(e) * 8;
+                o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+            }
+            const double len = s->b.len[e];
+            // flux = (len*flux)/len is exact only up to rounding; report len*flux/len
+            if (flux) for (int k = 0; k < 4; k++) flux[size_t(e) * 4 + k] = lf[size_t(f) * 4 + k] / len;
+            if (smax) smax[e] = ls[f] / len;
         }
-        const double len = s->b.len[e];
-        // flux = (len*flux)/len is exact only up to rounding; report len*flux/len
-        if (flux) for (int k = 0; k < 4; k++) flux[size_t(e) * 4 + k] = lf[size_t(f) * 4 + k] / len;
-        if (smax) smax[e] = ls[f] / len;
-    }
-    return EBB_OK;
+        return EBB_OK;
+    });
 }
 
 int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
 {
-    if (!s) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    // the fused stage kernel keeps gradients in LDS: recompute them for the last stage input
-    if (s->useTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
-    if (grad) { int rc_ = downloadCells(s, s->d_grad.p, grad, 8); if (rc_) return rc_; }
-    if (lim && !(s->phys.limited && s->lastLimit)) {
-        // an unlimited evaluation resets every limiter to 1 (solve.d:324-340) and stores none
-        std::fill(lim, lim + size_t(s->N) * 8, 1.0);
-    } else if (lim) {
-        const bool lp = s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0;
-        std::vector<double> l(size_t(s->N) * (lp ? 8 : 4));
-        { int rc_ = downloadCells(s, s->d_lim.p, l.data(), lp ? 8 : 4); if (rc_) return rc_; }
-        for (uint32_t i = 0; i < s->N; i++)
-            for (int k = 0; k < 4; k++) {
-                lim[size_t(i) * 8 + 2 * k] = lp ? l[size_t(i) * 8 + 2 * k] : l[size_t(i) * 4 + k];
-                lim[size_t(i) * 8 + 2 * k + 1] = lp ? l[size_t(i) * 8 + 2 * k + 1] : l[size_t(i) * 4 + k];
-            }
-    }
-    return EBB_OK;
+    return guarded([&]() -> int {
+        if (!s) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        // the fused stage kernel keeps gradients in LDS: recompute them for the last stage input
+        if (s->useTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
+        if (grad) { int rc_ = downloadCells(s, s->d_grad.p, grad, 8); if (rc_) return rc_; }
+        if (lim && !(s->phys.limited && s->lastLimit)) {
+            // an unlimited evaluation resets every limiter to 1 (solve.d:324-340) and stores none
+            std::fill(lim, lim + size_t(s->N) * 8, 1.0);
+        } else if (lim) {
+            const bool lp = s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0;
+            std::vector<double> l(size_t(s->N) * (lp ? 8 : 4));
+            { int rc_ = downloadCells(s, s->d_lim.p, l.data(), lp ? 8 : 4); if (rc_) return rc_; }
+            for (uint32_t i = 0; i < s->N; i++)
+                for (int k = 0; k < 4; k++) {
+                    lim[size_t(i) * 8 + 2 * k] = lp ? l[size_t(i) * 8 + 2 * k] : l[size_t(i) * 4 + k];
+                    lim[size_t(i) * 8 + 2 * k + 1] = lp ? l[size_t(i) * 8 + 2 * k + 1] : l[size_t(i) * 4 + k];
+                }
+        }
+        return EBB_OK;
+    });
 }
 
 // source/io.d:68-161
 static int saveSolutionImpl(ebb_solver* s, const char* path, const double* tdt);
 
-int ebb_save_solution(ebb_solver* s, const char* path) { return saveSolutionImpl(s, path, nullptr); }
+int ebb_save_solution(ebb_solver* s, const char* path)
+{
+    return guarded([&]() -> int { return saveSolutionImpl(s, path, nullptr); });
+}
 
 int ebb_save_solution_as(ebb_solver* s, const char* path, double t, double dt)
 {
-    const double tdt[2] = {t, dt};
-    return saveSolutionImpl(s, path, tdt);
+    return guarded([&]() -> int {
+        const double tdt[2] = {t, dt};
+        return saveSolutionImpl(s, path, tdt);
+    });
 }
 
 static int saveSolutionImpl(ebb_solver* s, const char* path, const double* tdt)
@@ -1295,55 +1358,59 @@ static int saveSolutionImpl(ebb_solver* s, const char* path, const double* tdt)
 // source/io.d:218-366 (single rank: data points must equal the interior cell count)
 int ebb_load_solution(ebb_solver* s, const char* path)
 {
-    if (!s || !path) return EBB_ERR_ARG;
-    std::ifstream f(path, std::ios::binary);
-    if (!f) return EBB_ERR_IO;
-    std::vector<uint8_t> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-    if (buf.size() < 28 || getBE32(buf.data()) != 0xEA98E1F5u) return EBB_ERR_PARSE;
-    const uint32_t ver = getBE32(buf.data() + 4), np = getBE32(buf.data() + 8);
-    if (np != s->N) return EBB_ERR_PARSE;
-    const double t = getBEd(buf.data() + 12), dt = getBEd(buf.data() + 20);
-    size_t off = 28;
-    std::vector<double4> q(s->N);
-    if (ver == 1) {
-        if (buf.size() < off + size_t(np) * 32) return EBB_ERR_PARSE;
-        for (uint32_t i = 0; i < np; i++, off += 32)
-            q[i] = make_double4(getBEd(&buf[off]), getBEd(&buf[off + 8]), getBEd(&buf[off + 16]), getBEd(&buf[off + 24]));
-    } else if (ver == 2) {
-        off += 8;  // M, order
-        for (uint32_t i = 0; i < np; i++) {
-            if (buf.size() < off + 5 + 32) return EBB_ERR_PARSE;
-            const uint8_t ne = buf[off + 4];
-            off += 5;
-            q[i] = make_double4(getBEd(&buf[off]), getBEd(&buf[off + 8]), getBEd(&buf[off + 16]), getBEd(&buf[off + 24]));
-            off += 32 + size_t(ne) * 32;
-            if (off > buf.size()) return EBB_ERR_PARSE;
+    return guarded([&]() -> int {
+        if (!s || !path) return EBB_ERR_ARG;
+        std::ifstream f(path, std::ios::binary);
+        if (!f) return EBB_ERR_IO;
+        std::vector<uint8_t> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        if (buf.size() < 28 || getBE32(buf.data()) != 0xEA98E1F5u) return EBB_ERR_PARSE;
+        const uint32_t ver = getBE32(buf.data() + 4), np = getBE32(buf.data() + 8);
+        if (np != s->N) return EBB_ERR_PARSE;
+        const double t = getBEd(buf.data() + 12), dt = getBEd(buf.data() + 20);
+        size_t off = 28;
+        std::vector<double4> q(s->N);
+        if (ver == 1) {
+            if (buf.size() < off + size_t(np) * 32) return EBB_ERR_PARSE;
+            for (uint32_t i = 0; i < np; i++, off += 32)
+                q[i] = make_double4(getBEd(&buf[off]), getBEd(&buf[off + 8]), getBEd(&buf[off + 16]), getBEd(&buf[off + 24]));
+        } else if (ver == 2) {
+            off += 8;  // M, order
+            for (uint32_t i = 0; i < np; i++) {
+                if (buf.size() < off + 5 + 32) return EBB_ERR_PARSE;
+                const uint8_t ne = buf[off + 4];
+                off += 5;
+                q[i] = make_double4(getBEd(&buf[off]), getBEd(&buf[off + 8]), getBEd(&buf[off + 16]), getBEd(&buf[off + 24]));
+                off += 32 + size_t(ne) * 32;
+                if (off > buf.size()) return EBB_ERR_PARSE;
+            }
+        } else {
+            return EBB_ERR_PARSE;
         }
-    } else {
-        return EBB_ERR_PARSE;
-    }
-    HIPCHK(hipSetDevice(s->device));
-    { int rc_ = uploadCells(s, reinterpret_cast<const double*>(q.data()), s->qcur(), 4); if (rc_) return rc_; }
-    return ebb_run_reset(s, t, dt);
+        HIPCHK(hipSetDevice(s->device));
+        { int rc_ = uploadCells(s, reinterpret_cast<const double*>(q.data()), s->qcur(), 4); if (rc_) return rc_; }
+        return ebb_run_reset(s, t, dt);
+    });
 }
 
 int ebb_write_forces(ebb_solver* s, const char* path, int64_t first, int64_t count, int append)
 {
-    if (!s || !path || first < 0 || count < 0) return EBB_ERR_ARG;
-    if ((first + count) * HIST_FIELDS > int64_t(s->history.size())) return EBB_ERR_ARG;
-    std::vector<uint8_t> buf;
-    buf.reserve(size_t(count) * 24);
-    for (int64_t k = first; k < first + count; k++) {
-        const double* h = s->history.data() + k * HIST_FIELDS;
-        putBEd(buf, h[0]);
-        putBEd(buf, h[7]);
-        putBEd(buf, h[8]);
-    }
-    FILE* f = std::fopen(path, append ? "ab" : "wb");
-    if (!f) return EBB_ERR_IO;
-    const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
-    std::fclose(f);
-    return w == buf.size() ? EBB_OK : EBB_ERR_IO;
+    return guarded([&]() -> int {
+        if (!s || !path || first < 0 || count < 0) return EBB_ERR_ARG;
+        if ((first + count) * HIST_FIELDS > int64_t(s->history.size())) return EBB_ERR_ARG;
+        std::vector<uint8_t> buf;
+        buf.reserve(size_t(count) * 24);
+        for (int64_t k = first; k < first + count; k++) {
+            const double* h = s->history.data() + k * HIST_FIELDS;
+            putBEd(buf, h[0]);
+            putBEd(buf, h[7]);
+            putBEd(buf, h[8]);
+        }
+        FILE* f = std::fopen(path, append ? "ab" : "wb");
+        if (!f) return EBB_ERR_IO;
+        const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+        std::fclose(f);
+        return w == buf.size() ? EBB_OK : EBB_ERR_IO;
+    });
 }
 
 int ebb_last_error(ebb_solver* s, int* kind, int64_t* index, int64_t* step, char* msg, size_t len)
@@ -1358,82 +1425,84 @@ int ebb_last_error(ebb_solver* s, int* kind, int64_t* index, int64_t* step, char
 
 int ebb_time_steps(ebb_solver* s, int steps, double out[6])
 {
-    if (!s || !out || steps <= 0) return EBB_ERR_ARG;
-    HIPCHK(hipSetDevice(s->device));
-    { int rc_ = readScalars(s); if (rc_) return rc_; }
-    DevScalars saved = *s->h_sc;
-    const int nst = stagesPerStep(s->cfg);
-    std::vector<hipEvent_t> ev(size_t(steps) * nst * 3 + 1);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    std::vector<int> part;   // which figure each event interval adds to
-    // same launch sequence as stepStages, with an event after each kernel
-    HIPCHK(hipEventRecord(ev[0], s->stream));
-    size_t k = 1;
-    const Config& c = s->cfg;
-    const int msl = c.multistageLimiting ? 1 : 0, lts = c.localTimestep ? 1 : 0;
-    for (int st = 0; st < steps; st++) {
-        double4 *q = s->qcur(), *nx = s->qnext(), *T = s->d_T.p, *T2 = s->d_T2.p;
-        struct St { int kind; double4 *in, *out; int limit, dtu, first, last; };
-        std::vector<St> seq;
-        switch (c.integratorKind()) {
-        case IntEuler: seq = {{SK_EULER, q, nx, 1, 1, 1, 1}}; break;
-        case IntRK2: seq = {{SK_RK2_1, q, T, 1, 1, 1, 0}, {SK_RK2_2, T, nx, msl, 0, 0, 1}}; break;
-        case IntRK2TVD: seq = {{SK_TVD_1, q, T, 1, lts, 1, 0}, {SK_TVD_2, T, nx, 0, lts ? 0 : 1, 0, 1}}; break;
-        default: seq = {{SK_RK4_1, q, T, 1, 1, 1, 0}, {SK_RK4_2, T, T2, msl, 0, 0, 0}, {SK_RK4_3, T2, T, msl, 0, 0, 0}, {SK_RK4_4, T, nx, msl, 0, 0, 1}};
-        }
-        for (auto& x : seq) {
-            StageArgs a{};
-            a.qin = x.in; a.qn = q; a.qout = x.out; a.acc = s->d_acc.p; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
-            a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
-            a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
-            a.rmsBlocks = (x.last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
-            a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
-            s->stageStep.push_back(s->stepsTotal);
-            if (s->useTiles) {
-                launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
-                HIPCHK(hipEventRecord(ev[k++], s->stream));
-                part.push_back(3);
-            } else if (s->useFaceTiles) {
-                launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
-                HIPCHK(hipEventRecord(ev[k++], s->stream));
-                launchFaceTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream);
-                HIPCHK(hipEventRecord(ev[k++], s->stream));
-                part.push_back(0); part.push_back(3);
-            } else {
-                launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
-                HIPCHK(hipEventRecord(ev[k++], s->stream));
-                launchFaces(s->dm, s->phys, a, s->stream);
-                HIPCHK(hipEventRecord(ev[k++], s->stream));
-                launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
-                HIPCHK(hipEventRecord(ev[k++], s->stream));
-                part.push_back(0); part.push_back(1); part.push_back(2);
+    return guarded([&]() -> int {
+        if (!s || !out || steps <= 0) return EBB_ERR_ARG;
+        HIPCHK(hipSetDevice(s->device));
+        { int rc_ = readScalars(s); if (rc_) return rc_; }
+        DevScalars saved = *s->h_sc;
+        const int nst = stagesPerStep(s->cfg);
+        std::vector<hipEvent_t> ev(size_t(steps) * nst * 3 + 1);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        std::vector<int> part;   // which figure each event interval adds to
+        // same launch sequence as stepStages, with an event after each kernel
+        HIPCHK(hipEventRecord(ev[0], s->stream));
+        size_t k = 1;
+        const Config& c = s->cfg;
+        const int msl = c.multistageLimiting ? 1 : 0, lts = c.localTimestep ? 1 : 0;
+        for (int st = 0; st < steps; st++) {
+            double4 *q = s->qcur(), *nx = s->qnext(), *T = s->d_T.p, *T2 = s->d_T2.p;
+            struct St { int kind; double4 *in, *out; int limit, dtu, first, last; };
+            std::vector<St> seq;
+            switch (c.integratorKind()) {
+            case IntEuler: seq = {{SK_EULER, q, nx, 1, 1, 1, 1}}; break;
+            case IntRK2: seq = {{SK_RK2_1, q, T, 1, 1, 1, 0}, {SK_RK2_2, T, nx, msl, 0, 0, 1}}; break;
+            case IntRK2TVD: seq = {{SK_TVD_1, q, T, 1, lts, 1, 0}, {SK_TVD_2, T, nx, 0, lts ? 0 : 1, 0, 1}}; break;
+            default: seq = {{SK_RK4_1, q, T, 1, 1, 1, 0}, {SK_RK4_2, T, T2, msl, 0, 0, 0}, {SK_RK4_3, T2, T, msl, 0, 0, 0}, {SK_RK4_4, T, nx, msl, 0, 0, 1}};
             }
-            s->lastInput = x.in;
-            s->lastLimit = x.limit;
-            s->haveStage = true;
+            for (auto& x : seq) {
+                StageArgs a{};
+                a.qin = x.in; a.qn = q; a.qout = x.out; a.acc = s->d_acc.p; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
+                a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
+                a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
+                a.rmsBlocks = (x.last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
+                a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
+                s->stageStep.push_back(s->stepsTotal);
+                if (s->useTiles) {
+                    launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    part.push_back(3);
+                } else if (s->useFaceTiles) {
+                    launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    launchFaceTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    part.push_back(0); part.push_back(3);
+                } else {
+                    launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    launchFaces(s->dm, s->phys, a, s->stream);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    launchUpdate(s->dm, s->phys, a, int(s->S), s->stream);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    part.push_back(0); part.push_back(1); part.push_back(2);
+                }
+                s->lastInput = x.in;
+                s->lastLimit = x.limit;
+                s->haveStage = true;
+            }
+            s->cur ^= 1;
         }
-        s->cur ^= 1;
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventSynchronize(ev[k - 1]));
-    double tp[4] = {0, 0, 0, 0};
-    for (size_t j = 1; j < k; j++) {
-        float a;
-        HIPCHK(hipEventElapsedTime(&a, ev[j - 1], ev[j]));
-        tp[part[j - 1]] += a;
-    }
-    float total;
-    HIPCHK(hipEventElapsedTime(&total, ev[0], ev[k - 1]));
-    const double n = double(steps) * nst;
-    out[0] = tp[0] / n; out[1] = tp[1] / n; out[2] = tp[2] / n; out[3] = tp[3] / n;
-    out[4] = double(total) / steps;
-    out[5] = nst;
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    // timing steps advance the state but not the run loop's clock
-    *s->h_sc = saved;
-    HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return EBB_OK;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(ev[k - 1]));
+        double tp[4] = {0, 0, 0, 0};
+        for (size_t j = 1; j < k; j++) {
+            float a;
+            HIPCHK(hipEventElapsedTime(&a, ev[j - 1], ev[j]));
+            tp[part[j - 1]] += a;
+        }
+        float total;
+        HIPCHK(hipEventElapsedTime(&total, ev[0], ev[k - 1]));
+        const double n = double(steps) * nst;
+        out[0] = tp[0] / n; out[1] = tp[1] / n; out[2] = tp[2] / n; out[3] = tp[3] / n;
+        out[4] = double(total) / steps;
+        out[5] = nst;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        // timing steps advance the state but not the run loop's clock
+        *s->h_sc = saved;
+        HIPCHK(hipMemcpyAsync(s->d_sc, s->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return EBB_OK;
+    });
 }
 
 int64_t ebb_solver_device_bytes(const ebb_solver* s) { return s ? s->deviceBytes : 0; }

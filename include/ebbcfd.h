@@ -34,7 +34,8 @@ enum {
     EBB_ERR_DEVICE = 6,     /* HIP runtime error or no device */
     EBB_ERR_EDGE_NAN = 7,   /* NaN flux on an edge (EdgeException, source/solve.d:806) */
     EBB_ERR_CELL_NAN = 8,   /* NaN state in a cell (CellException, app/solver.d:148) */
-    EBB_ERR_STATE = 9       /* call not valid in the solver's current state */
+    EBB_ERR_STATE = 9,      /* call not valid in the solver's current state */
+    EBB_ERR_ALLOC = 10      /* host or device memory exhausted, or a request larger than the memory allowed it */
 };
 
 /* Text for a status code. */

@@ -76,6 +76,8 @@ def test_mms_refinement_matches_published(family):
     res = [steady_l2(mname % k, cname % k, over) for k in range(len(l2_pub))]
     l2 = np.array([r[0][0] for r in res])
     print(family, "L2", l2, "published", l2_pub, "rmax", [r[1] for r in res], "steps", [r[2] for r in res])
+    # every level reached the reference's stop criterion (app/ms.d:328) before its L2 is compared
+    assert all(r[1] <= 1e-10 for r in res), [r[1] for r in res]
     dx = DX[mname]
     order = np.log(l2[a] / l2[b]) / np.log(dx[a] / dx[b])
     assert abs(order - order_pub) < 0.15, (order, order_pub)

@@ -264,3 +264,104 @@ def test_pairwise_sum_structure():
         assert oracle.pairwise_sum(x) == recursive(list(x)), n
         ints = np.round(rng.uniform(-1e6, 1e6, n))
         assert oracle.pairwise_sum(ints) == float(np.sum(ints))
+
+
+def test_rusanov_wave_speed():
+    """sMax of rusanovFlux (flux.d:88-131): for qL = qR it is |u.n| + a, the largest
+    eigenvalue, whatever the averaging (the Harten fix only acts below 0.01 a); it is
+    symmetric in the two states and in the sign of n; and for distinct states it lies
+    between the wave speeds of the two states' own Harten-free maxima, since the
+    averaged velocity, pressure and density are means of the two sides'."""
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        q = random_state(rng)
+        th = rng.uniform(0, 2 * np.pi)
+        n = np.array([np.cos(th), np.sin(th)])
+        _, sm = oracle.rusanov_flux(q, q, n)
+        rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+        p = (G - 1) * (q[3] - 0.5 * rho * (u * u + v * v))
+        ref = abs(u * n[0] + v * n[1]) + math.sqrt(G * p / rho)
+        assert abs(sm - ref) <= 4e-15 * ref, (sm, ref)
+    for _ in range(50):
+        qL, qR = random_state(rng), random_state(rng)
+        n = np.array([0.6, -0.8])
+        _, s1 = oracle.rusanov_flux(qL, qR, n)
+        _, s2 = oracle.rusanov_flux(qR, qL, n)
+        _, s3 = oracle.rusanov_flux(qL, qR, -n)
+        assert s1 == s2 and abs(s1 - s3) <= 1e-15 * s1
+
+        def speeds(q):
+            rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+            p = (G - 1) * (q[3] - 0.5 * rho * (u * u + v * v))
+            return abs(u * n[0] + v * n[1]), p, rho
+
+        (uL, pL, rL), (uR, pR, rR) = speeds(qL), speeds(qR)
+        hi = max(uL, uR) + math.sqrt(G * max(pL, pR) / min(rL, rR))
+        lo = math.sqrt(G * min(pL, pR) / max(rL, rR)) - max(uL, uR)
+        assert lo - 1e-12 <= s1 <= hi + 1e-12, (lo, s1, hi)
+
+
+def _mms_fields(x, y):
+    q = oracle.solution(x, y)
+    rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+    p = (G - 1) * (q[3] - 0.5 * rho * (u * u + v * v))
+    return rho, u, v, p
+
+
+def _d(f, x, y, ax, h):
+    if ax == 0:
+        return (f(x + h, y) - f(x - h, y)) / (2 * h)
+    return (f(x, y + h) - f(x, y - h)) / (2 * h)
+
+
+def test_mms_viscous_source_energy_row():
+    """Energy row of the NS source (manufacturedsolution.d:219-236), in two parts that
+    finite differences of the exact fields fix independently of the restatement:
+      * heat part, proportional to k = gamma R mu / ((gamma - 1) Pr): varying Pr at fixed
+        mu changes only it.  It equals k times d2T/dx2 + d2T/dy2 of T = p / (rho R) plus
+        the reference's extra 2 rho_x p_x / (R rho^2) + 2 rho_y p_y / (R rho^2) (its
+        d2Tdx2 expands -(1/r^3)(r p_x - 2 p r_x) r_x, which cancels the +(1/r^2) r_x p_x
+        term where the true derivative needs -2 (1/r^2) r_x p_x);
+      * work part: d/dx(u tau_xx + v tau_xy) + d/dy(u tau_xy + v tau_yy) with the
+        textbook stress.
+    A dropped term, a sign flip or the textbook d2T/dx2 in place of the reference's
+    fails one of the two."""
+    mu, Rg = 0.01, 1.0
+    k = lambda Pr: G * Rg * mu / ((G - 1) * Pr)
+    # points where the reference's extra term is a sizable part of the heat term
+    for (x, y) in [(0.5, 0.95), (0.3, 0.55), (0.15, 0.25)]:
+        S_inv = oracle.source_term(x, y, viscosity=False)
+        S1 = oracle.source_term(x, y, viscosity=True, R=Rg, mu=mu, Pr=0.71)
+        S2 = oracle.source_term(x, y, viscosity=True, R=Rg, mu=mu, Pr=1.3)
+        # S = S_inv - (work + k * heat)
+        heat_src = -(S1[3] - S2[3]) / (k(0.71) - k(1.3))
+        work_src = -(S1[3] - S_inv[3]) - k(0.71) * heat_src
+
+        T = lambda a, b: _mms_fields(a, b)[3] / (_mms_fields(a, b)[0] * Rg)
+        h = 1e-4
+        Txx = (T(x + h, y) - 2 * T(x, y) + T(x - h, y)) / (h * h)
+        Tyy = (T(x, y + h) - 2 * T(x, y) + T(x, y - h)) / (h * h)
+        rho = _mms_fields(x, y)[0]
+        rx = _d(lambda a, b: _mms_fields(a, b)[0], x, y, 0, 1e-6)
+        ry = _d(lambda a, b: _mms_fields(a, b)[0], x, y, 1, 1e-6)
+        px = _d(lambda a, b: _mms_fields(a, b)[3], x, y, 0, 1e-6)
+        py = _d(lambda a, b: _mms_fields(a, b)[3], x, y, 1, 1e-6)
+        extra = 2 * rx * px / (Rg * rho * rho) + 2 * ry * py / (Rg * rho * rho)
+        np.testing.assert_allclose(heat_src, Txx + Tyy + extra, rtol=2e-5)
+        assert abs(extra) > 0.05 * abs(Txx + Tyy + extra)   # dropping it would fail the check above
+
+        def work_flux(a, b, ax):
+            e = 1e-5
+            _, u, v, _ = _mms_fields(a, b)
+            ux = _d(lambda s, t: _mms_fields(s, t)[1], a, b, 0, e)
+            uy = _d(lambda s, t: _mms_fields(s, t)[1], a, b, 1, e)
+            vx = _d(lambda s, t: _mms_fields(s, t)[2], a, b, 0, e)
+            vy = _d(lambda s, t: _mms_fields(s, t)[2], a, b, 1, e)
+            txx = mu * (4.0 / 3.0 * ux - 2.0 / 3.0 * vy)
+            tyy = mu * (4.0 / 3.0 * vy - 2.0 / 3.0 * ux)
+            txy = mu * (uy + vx)
+            return u * txx + v * txy if ax == 0 else u * txy + v * tyy
+
+        hw = 1e-3
+        div = _d(lambda a, b: work_flux(a, b, 0), x, y, 0, hw) + _d(lambda a, b: work_flux(a, b, 1), x, y, 1, hw)
+        np.testing.assert_allclose(work_src, div, rtol=1e-4, atol=1e-9)

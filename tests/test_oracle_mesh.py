@@ -230,3 +230,36 @@ def test_gmsh_reader_fixture_counts():
         assert m.n_ghost == m.n_boundary_edges
         # Euler: E = N + n_nodes - 1 for a simply connected planar mesh
         assert m.n_edges == m.n_interior + raw.nodes.shape[0] - 1
+
+
+def test_limiter_is_tight():
+    """The scalar limiter is the largest phi that keeps every neighbour-centroid
+    reconstruction in bounds (solve.d:360-403): wherever phi < 1, the reconstruction at
+    some neighbour centroid lies on its bound.  An over-restrictive limiter (phi scaled
+    down, phi = 0) passes the bounds check above but fails this one."""
+    raw, cfg = channel_case(limited=True)
+    m = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    rng = np.random.default_rng(1)
+    q = m.q
+    q[: m.n_interior] *= 1 + 0.1 * rng.uniform(-1, 1, (m.n_interior, 4))
+    m.q = q
+    m.residual(limit=True)
+    g, l, _ = m.cell_data()
+    c = m.cells()
+    q = m.q
+    N = m.n_interior
+    limited = 0
+    for i in range(N):
+        nb = c["nbrs"][i, : int(c["nEdges"][i])]
+        lo, hi = np.minimum(q[i], q[nb].min(axis=0)), np.maximum(q[i], q[nb].max(axis=0))
+        for k in range(4):
+            if l[i, k, 0] >= 1.0:
+                continue
+            limited += 1
+            gap = []
+            for j in nb:
+                dx = c["centroid"][j] - c["centroid"][i]
+                qe = q[i, k] + l[i, k, 0] * (g[i, k, 0] * dx[0] + g[i, k, 1] * dx[1])
+                gap.append(min(abs(qe - lo[k]), abs(qe - hi[k])))
+            assert min(gap) <= 1e-12 * max(1.0, abs(q[i, k])), (i, k, l[i, k, 0], min(gap))
+    assert limited > 50
