@@ -61,16 +61,198 @@ def cylinder_config(M=0.2, CFL=0.05):
     }
 
 
-def cpu_baseline(raw, cfg, steps, n_cells):
+def cpu_baseline(raw, cfg, steps, n_cells, threads):
+    """The oracle (C restatement of the reference loop) on `threads` host threads: its
+    per-cell and per-edge loops are OpenMP-parallel and give the single-thread bits."""
     import oracle
     oracle.build()
+    oracle.set_threads(threads)
     m = oracle.Mesh(raw)
     c = dict(cfg)
     m.setup(c)
     t0 = time.perf_counter()
     m.run(steps)
     el = time.perf_counter() - t0
+    oracle.set_threads(1)
     return n_cells * 2 * steps / el / 1e6, el
+
+
+def mixed_leg(mesh, cfg, s64, args, N, value64, ms64):
+    """The same workload in mixed precision (FP32 flux, reconstruction and limiter; FP64
+    geometry, accumulation and norms), timed like the FP64 line, and its error against the
+    FP64 solver after the same steps from the same initial state (the FP64 path equals the
+    oracle bit for bit)."""
+    from ebbcfd_amd import Solver
+    c = dict(cfg)
+    c["extensions"] = dict(cfg["extensions"], precision="mixed")
+    s = Solver(mesh, c)
+    s.run(args.warmup)
+    t0 = time.perf_counter()
+    hist, _, _ = s.run(args.steps)
+    el = time.perf_counter() - t0
+    assert len(hist) == args.steps and np.all(np.isfinite(hist[:, 2:9]))
+    ms = el / args.steps * 1e3
+    v = N * 2 * args.steps / el / 1e6
+    tk = s.time_steps(args.time_steps)
+    h64 = s64.history()
+    h32 = s.history()
+    n = min(len(h64), len(h32))
+    q64, q32 = s64.q, s.q
+    st = float((np.abs(q64 - q32) / np.abs(q64)).max())
+    drag = abs(h64[n - 1, 7] - h32[n - 1, 7]) / abs(h64[n - 1, 7])
+    lift = abs(h64[n - 1, 8] - h32[n - 1, 8]) / max(abs(h64[n - 1, 8]), abs(h64[n - 1, 7]))
+    log("mixed: %.3f ms/step, %.1f Mcell-stage/s (x%.3f of FP64); tile %.3f ms, gradient %.3f ms; "
+        "after %d steps: state %.3g, drag %.3g, lift %.3g" % (ms, v, v / value64, tk["fused_ms"], tk["gradient_ms"], n,
+                                                               st, drag, lift))
+    del s
+    return {"value": round(v, 3), "unit": "Mcell-stage/s", "ms_per_step": round(ms, 4), "dtype": "mixed",
+            "speedup_vs_fp64": round(v / value64, 4),
+            "kernels_ms": {"k_face_tile": round(tk["fused_ms"], 4), "k_gradient": round(tk["gradient_ms"], 4)},
+            "error_vs_fp64": {"steps": int(n), "max_rel_state": st, "drag_rel": drag, "lift_rel": lift},
+            "precision": "FP32 flux/reconstruction/gradient/limiter; FP64 geometry, residual sum, update, norms"}
+
+
+def bump_request(rng, cells, M_range, seed, precision="mixed", **ext):
+    """One channel-bump Euler solve: bump height U(0.02, 0.1) of a sin^2 bump on the lower
+    wall, `cells` jittered triangles, M and AoA from the given ranges, Roe flux, scalar
+    (Barth-Jespersen-type) limiter, RK4, inviscid walls, full-state inflow/outflow."""
+    from ebbcfd_amd import meshgen
+    nx, ny = meshgen.channel_bump_cells(cells)
+    raw = meshgen.channel_bump(nx, ny, float(rng.uniform(0.02, 0.1)), seed=seed)
+    q = [float(rng.uniform(*M_range)), float(rng.uniform(-5.0, 5.0)), 1.0, 1.0]
+    bcs = [{"tag": t, "type": "fullState" if t in ("Inflow", "Outflow") else "inviscidWall", "q": q}
+           for t in ("Inflow", "Outflow", "Lower", "Upper")]
+    cfg = {"mesh": "bump.msh", "dt": 1.0e-5, "tEnd": 1.0e9, "flux": "roeFlux", "plotIter": -1, "saveIter": -1,
+           "CFL": 0.5, "integrator": "RK4", "forceBoundary": "Lower", "limited": True, "order": 2,
+           "initialConditions": q, "viscosity": False, "cflAdjust": True,
+           "physicalConfig": {"Pr": 0.7, "R": 8.3145, "gamma": 1.4, "Re": 1000.0, "L": 1.0},
+           "boudaryConditions": bcs, "extensions": dict(precision=precision, **ext)}
+    return raw, cfg
+
+
+def batched_leg(args):
+    """Batched Euler: `--batched-meshes` channel-bump meshes, cell counts log-uniform in
+    [5e3, 5e5], M U(0.3, 2.0), AoA U(-5, 5) deg, mixed precision, Roe + scalar limiter, RK4;
+    every mesh advanced `--batched-steps` steps by one launch per kernel per RK stage
+    (HIP-graph replays; tile size and graph length from the autotuner), against the same
+    solves run one at a time through the solo path."""
+    from ebbcfd_amd import Batch, Solver, build_mesh
+    rng = np.random.Generator(np.random.PCG64(args.batched_seed))
+    n, K = args.batched_meshes, args.batched_steps
+    cells = np.exp(rng.uniform(np.log(5e3), np.log(5e5), n))
+    t0 = time.perf_counter()
+    reqs = [bump_request(rng, int(c), (0.3, 2.0), args.batched_seed * 1000 + k) for k, c in enumerate(cells)]
+    meshes = [build_mesh(raw) for raw, _ in reqs]
+    total = sum(m.n_interior for m in meshes)
+    log("C4: %d meshes, %d cells in all, built in %.1f s" % (n, total, time.perf_counter() - t0))
+    b = Batch(graph_steps=8)
+    t0 = time.perf_counter()
+    ids = [b.add(m, cfg) for m, (_, cfg) in zip(meshes, reqs)]
+    log("C4: admitted (ufvmSetup + upload) in %.1f s, %.2f GB on the device" % (time.perf_counter() - t0,
+                                                                              b.info()["device_bytes"] / 1e9))
+    b.run(8)
+    tune = b.tune(steps=8) if args.batched_tune else None
+    if tune:
+        log("C4 autotune: tile cells %d, graph steps %d: %.3f ms/step (default %.3f)"
+            % (tune["tile_cells"], tune["graph_steps"], tune["ms_per_step"], tune["default_ms_per_step"]))
+    b.run(16)
+    t0 = time.perf_counter()
+    running = b.run(K)
+    el = time.perf_counter() - t0
+    stopped = n - running
+    batched_sps = n / el
+    # one at a time through the solo path, on a subsample of the same solves
+    sample = list(range(0, n, max(1, n // args.batched_solo_sample)))
+    solo_el = 0.0
+    for k in sample:
+        s = Solver(meshes[k], reqs[k][1])
+        s.run(8)
+        t0 = time.perf_counter()
+        s.run(K)
+        solo_el += time.perf_counter() - t0
+        del s
+    solo_sps = len(sample) / solo_el
+    log("C4: batched %.1f solves/s (%d steps each, %.2f ms/step for all %d), one at a time %.1f solves/s "
+        "(%d-solve sample): x%.2f" % (batched_sps, K, el / K * 1e3, n, solo_sps, len(sample), batched_sps / solo_sps))
+    del b
+    return {"workload": "channel_bump_batch_%d_loguniform_5e3_5e5_rk4_mixed" % n, "meshes": n, "cells": int(total),
+            "steps_per_solve": K, "batched_solves_per_s": round(batched_sps, 2), "ms_per_step_all": round(el / K * 1e3, 4),
+            "Mcell_stage_per_s": round(total * 4 * K / el / 1e6, 1), "solo_solves_per_s": round(solo_sps, 2),
+            "solo_sample": len(sample), "speedup": round(batched_sps / solo_sps, 3), "stopped_early": int(stopped),
+            "tune": tune, "seed": args.batched_seed}
+
+
+def serve_bench(args):
+    """Serving: Poisson arrivals of steady channel-bump Euler solves (sizes log-uniform in
+    [1e4, 2e6] cells, M U(0.3, 1.5)), each run to a 1e-8 drop of the rho RMS residual or the
+    step cap; continuous batching with admission by device footprint; p50/p99 time to
+    converged and solves/s per arrival rate, beside the same stream served one at a time."""
+    from ebbcfd_amd import Server, build_mesh
+    rng = np.random.Generator(np.random.PCG64(args.serve_seed))
+    n = args.serve_requests
+    cells = np.exp(rng.uniform(np.log(1e4), np.log(2e6), n))
+    t0 = time.perf_counter()
+    reqs = [bump_request(rng, int(c), (0.3, 1.5), args.serve_seed * 1000 + k, precision=args.serve_precision,
+                         rmsDrop=1e-8, maxSteps=args.serve_max_steps) for k, c in enumerate(cells)]
+    for _, cfg in reqs:
+        cfg["localTimestep"] = True
+    meshes = [build_mesh(raw) for raw, _ in reqs]
+    log("serve: %d requests, %d cells in all, built in %.1f s" % (n, sum(m.n_interior for m in meshes),
+                                                               time.perf_counter() - t0))
+    rows = []
+    for rate in [float(r) for r in args.serve_rates.split(",")]:
+        for mode, max_solves in (("batched", 256), ("one_at_a_time", 1)):
+            if mode == "one_at_a_time" and rate != float(args.serve_rates.split(",")[0]):
+                continue
+            sv = Server(max_solves=max_solves, steps_per_round=args.serve_round, graph_steps=8)
+            arr = np.cumsum(rng.exponential(1.0 / rate, n))
+            t0 = time.perf_counter()
+            for k in range(n):
+                d = arr[k] - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                sv.submit(meshes[k], reqs[k][1])
+            ok = sv.wait(n, 3000.0)
+            st = sv.stats()
+            polls = [sv.poll(k) for k in range(n)]
+            sv.close()
+            conv = sum(1 for p in polls if p["converged"])
+            row = {"rate_per_s": rate, "mode": mode, "done": int(st["done"]), "failed": int(st["failed"]),
+                   "p50_s": round(st["p50_s"], 3), "p99_s": round(st["p99_s"], 3),
+                   "solves_per_s": round(st["solves_per_s"], 3), "converged": conv, "complete": bool(ok)}
+            log("serve %s rate %g/s: %s" % (mode, rate, row))
+            rows.append(row)
+    out = {"metric": "batched solves/s + p99 latency (serving)", "unit": "solves/s",
+           "config": {"workload": "channel_bump_poisson_loguniform_1e4_2e6", "requests": n, "max_steps": args.serve_max_steps,
+                      "rms_drop": 1e-8, "precision": args.serve_precision, "steps_per_round": args.serve_round,
+                      "seed": args.serve_seed},
+           "rows": rows}
+    print(json.dumps(out), flush=True)
+
+
+def c1_leg(steps):
+    """Latency of the manufactured-solution Euler workload: 8,192 jittered triangles,
+    Roe, scalar limiter, RK4, CFL 0.5 -- cache-resident, reported for latency."""
+    from ebbcfd_amd import Solver, build_mesh, meshgen
+    raw = meshgen.mms_square(64, 0.1, 1)
+    q = [0.1, 0.0, 1.1, 1.0]
+    cfg = {"mesh": "mms.msh", "dt": 1.0e-5, "tEnd": 1.0e9, "flux": "roeFlux", "plotIter": -1, "saveIter": -1,
+           "CFL": 0.5, "integrator": "RK4", "forceBoundary": "Wall", "limited": True, "order": 2,
+           "initialConditions": q, "viscosity": False,
+           "physicalConfig": {"Pr": 0.71, "R": 1.0, "gamma": 1.4, "Re": 1000.0, "L": 1.0},
+           "boudaryConditions": [{"tag": "Freestream", "type": "dirichlet", "q": q}], "extensions": {"mms": True}}
+    s = Solver(build_mesh(raw), cfg)
+    s.run(50)
+    t0 = time.perf_counter()
+    h, _, _ = s.run(steps)
+    el = time.perf_counter() - t0
+    assert len(h) == steps
+    tk = s.time_steps(50)
+    log("C1: %d RK4 steps of 8192 cells: %.2f us/step (stage kernels %.2f us/step)"
+        % (steps, el / steps * 1e6, tk["step_ms"] * 1e3))
+    return {"workload": "mms_euler_64x64x2_jittered_rk4", "cells": 8192, "steps": steps,
+            "us_per_step": round(el / steps * 1e6, 2), "stage_kernels_us_per_step": round(tk["step_ms"] * 1e3, 2),
+            "Mcell_stage_per_s": round(8192 * 4 * steps / el / 1e6, 2)}
 
 
 def copy_calibration():
@@ -160,6 +342,21 @@ def main():
                     help="how tiles are cut: Morton order of 16-cell blocks, greedily grown regions, Morton order")
     ap.add_argument("--cell-order", default="reference", choices=("reference", "tiles"),
                     help="device numbering of the cells: the mesh's, or tile by tile")
+    ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision form of the workload (0: skip)")
+    ap.add_argument("--c1-steps", type=int, default=1000,
+                    help="steps of the 8,192-cell manufactured-solution latency leg (0: skip)")
+    ap.add_argument("--batched-meshes", type=int, default=256, help="meshes of the batched leg (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=64, help="steps per solve timed in the batched leg")
+    ap.add_argument("--batched-seed", type=int, default=1)
+    ap.add_argument("--batched-tune", type=int, default=1, help="run the batch autotuner first")
+    ap.add_argument("--batched-solo-sample", type=int, default=32, help="solves timed one at a time")
+    ap.add_argument("--serve", action="store_true", help="run the serving sweep instead")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rates", default="2,20,200", help="arrival rates (1/s); the first also runs one at a time")
+    ap.add_argument("--serve-max-steps", type=int, default=20000)
+    ap.add_argument("--serve-round", type=int, default=200, help="steps per serving round")
+    ap.add_argument("--serve-precision", default="mixed", choices=("mixed", "fp64"))
+    ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters) and exit")
     ap.add_argument("--ingest", action="store_true",
@@ -171,6 +368,9 @@ def main():
         return
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.serve:
+        serve_bench(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver keeps the whole mesh on a single MI355X "
@@ -226,14 +426,28 @@ def main():
     traffic = MEASURED_TRAFFIC.get((kernel, workload, args.stage_path, args.tile_cells, args.tile_order,
                                     args.cell_order))
 
+    mixed = None
+    if args.mixed:
+        mixed = mixed_leg(mesh, cfg, s, args, N, value, ms_per_step)
+    c1 = None
+    if args.c1_steps > 0:
+        c1 = c1_leg(args.c1_steps)
+    batched = None
+    if args.batched_meshes > 0:
+        del s
+        batched = batched_leg(args)
+
     cpu = None
     if args.cpu_steps > 0:
-        log("CPU baseline: oracle, %d steps on the same mesh, 1 core" % args.cpu_steps)
-        v, el_cpu = cpu_baseline(raw, cfg, args.cpu_steps, N)
-        cpu = {"value": round(v, 4), "unit": "Mcell-stage/s", "cores": 1, "kind": "port",
-               "sample": "oracle (C restatement, gcc -O2 -ffp-contract=off, 1 thread): %d TVD-RK2 steps of the same "
-                         "4.2M-cell cylinder case, %.1f s" % (args.cpu_steps, el_cpu)}
-        log("CPU baseline %.3f Mcell-stage/s (%.1f s)" % (v, el_cpu))
+        T = host_threads()
+        log("CPU baseline: oracle, %d steps on the same mesh, %d threads, then 1" % (args.cpu_steps * 4, T))
+        vT, elT = cpu_baseline(raw, cfg, args.cpu_steps * 4, N, T)
+        v1, el1 = cpu_baseline(raw, cfg, args.cpu_steps, N, 1)
+        cpu = {"value": round(vT, 4), "unit": "Mcell-stage/s", "cores": T, "kind": "port",
+               "sample": "oracle (C restatement, gcc -O2 -ffp-contract=off, OpenMP over cells and edges, %d threads): "
+                         "%d TVD-RK2 steps of the same 4.2M-cell cylinder case, %.1f s" % (T, args.cpu_steps * 4, elT),
+               "single_core": {"value": round(v1, 4), "cores": 1, "steps": args.cpu_steps, "seconds": round(el1, 1)}}
+        log("CPU baseline %.3f Mcell-stage/s on %d threads (%.1f s), %.3f on 1 (%.1f s)" % (vT, T, elT, v1, el1))
 
     out = {
         "metric": "Mcell-RK-stage updates/s @4M cells (%HBM roof)",
@@ -261,6 +475,9 @@ def main():
                            "achieved": round(stage_gbs, 1), "frac": round(stage_gbs / HBM_PEAK_GBS, 4),
                            "unit": "GB/s"},
         "cpu_baseline": cpu,
+        "mixed": mixed,
+        "c1": c1,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -4,4 +4,5 @@ The compute path is libebbcfd.so (HIP kernels for gfx950 behind the C ABI in
 include/ebbcfd.h); this package is its Python host mirror plus mesh generators.
 """
 from .meshgen import RawMesh, channel_bump, cylinder_ogrid, mms_square, write_gmsh  # noqa: F401
-from .solver import HIST_FIELDS, Config, Solver, UMesh2, build_mesh, import_mesh, load_config  # noqa: F401
+from .solver import (HIST_FIELDS, Batch, Config, Server, Solver, UMesh2, build_mesh,  # noqa: F401
+                     estimate_device_bytes, import_mesh, load_config)

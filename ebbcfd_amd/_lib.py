@@ -19,7 +19,11 @@ SYMBOLS = (
     "ebb_history", "ebb_history_length", "ebb_edge_states", "ebb_cell_gradients", "ebb_save_solution",
     "ebb_save_solution_as",
     "ebb_load_solution", "ebb_write_forces", "ebb_last_error", "ebb_time_steps", "ebb_solver_device_bytes",
-    "ebb_solver_destroy",
+    "ebb_solver_destroy", "ebb_estimate_device_bytes", "ebb_batch_create", "ebb_batch_add", "ebb_batch_remove",
+    "ebb_batch_run", "ebb_batch_status", "ebb_batch_state_download", "ebb_batch_history_length", "ebb_batch_history",
+    "ebb_batch_info", "ebb_batch_tune", "ebb_batch_device_bytes", "ebb_batch_destroy", "ebb_server_create",
+    "ebb_server_submit", "ebb_server_poll", "ebb_server_result", "ebb_server_wait", "ebb_server_stats",
+    "ebb_server_destroy",
 )
 
 _lib = None
@@ -83,5 +87,29 @@ def lib():
     L.ebb_solver_device_bytes.restype = i64
     L.ebb_solver_destroy.argtypes = [vp]
     L.ebb_solver_destroy.restype = None
+    L.ebb_estimate_device_bytes.argtypes = [vp, vp, P(i64)]
+    L.ebb_batch_create.argtypes = [C.c_int, i64, C.c_int, C.c_int, P(vp)]
+    L.ebb_batch_add.argtypes = [vp, vp, vp, P(i64)]
+    L.ebb_batch_remove.argtypes = [vp, i64]
+    L.ebb_batch_run.argtypes = [vp, i64, P(i64)]
+    L.ebb_batch_status.argtypes = [vp, i64, P(d)]
+    L.ebb_batch_state_download.argtypes = [vp, i64, P(d)]
+    L.ebb_batch_history_length.argtypes = [vp, i64]
+    L.ebb_batch_history_length.restype = i64
+    L.ebb_batch_history.argtypes = [vp, i64, i64, i64, P(d)]
+    L.ebb_batch_info.argtypes = [vp, P(i64)]
+    L.ebb_batch_tune.argtypes = [vp, C.c_int, P(d)]
+    L.ebb_batch_device_bytes.argtypes = [vp]
+    L.ebb_batch_device_bytes.restype = i64
+    L.ebb_batch_destroy.argtypes = [vp]
+    L.ebb_batch_destroy.restype = None
+    L.ebb_server_create.argtypes = [C.c_int, i64, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]
+    L.ebb_server_submit.argtypes = [vp, vp, vp, P(i64)]
+    L.ebb_server_poll.argtypes = [vp, i64, P(d)]
+    L.ebb_server_result.argtypes = [vp, i64, P(d)]
+    L.ebb_server_wait.argtypes = [vp, i64, d]
+    L.ebb_server_stats.argtypes = [vp, P(d)]
+    L.ebb_server_destroy.argtypes = [vp]
+    L.ebb_server_destroy.restype = None
     _lib = L
     return L

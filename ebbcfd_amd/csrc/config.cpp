@@ -224,6 +224,10 @@ bool parseConfig(const std::string& json, Config& c, std::string& err)
         const JVal* e;
         if ((e = v->get("mms"))) c.mms = isTrue(e);
         if ((e = v->get("maxSteps")) && e->kind == JVal::Int) c.maxSteps = e->i;
+        if ((e = v->get("rmsDrop")) && e->isNum()) {
+            if (!(e->num() >= 0 && e->num() < 1)) { err = "rmsDrop must be in [0, 1)"; return false; }
+            c.rmsDrop = e->num();
+        }
         if ((e = v->get("cellOrder")) && e->kind == JVal::Str) {
             if (e->s == "reference") c.renumber = false;
             else if (e->s == "tiles") c.renumber = true;

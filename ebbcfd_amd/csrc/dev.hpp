@@ -38,8 +38,8 @@ struct DevScalars {
     double residLast;               // residMax of the previous step (cflAdjust)
     int64_t incIters;               // cflAdjust counter
     int64_t steps;                  // completed steps
-    int32_t active;                 // 0 once t >= tEnd or an error was recorded
-    int32_t pad;
+    int32_t active;                 // 0 once t >= tEnd, maxSteps, convergence or an error was recorded
+    int32_t converged;              // 1 once the rho RMS dropped by rmsDrop below its first-step value
     double rms[4];
     double force[2];
     double fsum[2];                 // boundary force sums of the current step, before the AoA rotation
@@ -48,6 +48,7 @@ struct DevScalars {
     int64_t forceSlot;              // its history row (index into the history buffer), -1: none
     unsigned long long errBC;       // (stage << 32) | boundary face whose total-conditions inflow has no
                                     // non-negative Mach root (solve.d:648-671), min wins; ~0 = none
+    double rms0;                    // rho RMS residual of the first step (convergence stop)
 };
 
 // The ordered boundary-force sum of a time-loop step (mesh.d:645-671), when it is
@@ -116,6 +117,13 @@ struct StageArgs {
     double* lenF;          // (F+1)*4: len*flux per face
     double* lenS;          // (F+1): len*sMax per face
     double4* estate;       // optional (F)*2 edge states for .esln, null otherwise
+    // mixed precision (PhysParams::mixed): FP32 shadow of the stage input (ghost slots written
+    // by the gradient kernel) and of the stage output, FP32 gradient (N*8) and limiter (N*4)
+    // rows; null on the FP64 path
+    float4* qin32;
+    float4* qout32;
+    float* grad32;
+    float* lim32;
     double* celldt;        // N local time step (localTimestep)
     double* partial;       // 2 per update workgroup: min dt candidate, max |R|
     double* rmsBlocks;     // last stage of a time-loop step: per 16-cell reference block, the 8 leaf
@@ -242,14 +250,54 @@ struct DiagArgs {
     const double* partial;
     uint32_t nPart, pstride, nStages;
     int deferForce;          // 1: the force is left pending for the next gradient kernel
+    int histRing;            // 1: history row = steps % histCap (batched runs) instead of steps - histBase
+    double rmsDrop;          // > 0: stop once rho RMS <= rmsDrop * its first-step value
     DevScalars* sc;
 };
 constexpr int HIST_FIELDS = 10;
+
+// Batched runs (many meshes per launch): the per-mesh arguments of every kernel of a
+// time-loop step, kept in device memory, one record per mesh.  A work item (mesh, local
+// index) names a cell block of the gradient kernel (FORCE_ITEM: that mesh's pending
+// boundary force), a face tile, or a workgroup of the RMS kernel; each workgroup runs the
+// solo kernels' code on its item, so every mesh's results equal its solo run's.
+struct MemberArgs {
+    DevMesh m;
+    PhysParams p;
+    FaceTileArgs t;
+    StageArgs st[2][4];      // [parity][stage in step]: parity 0 reads the state buffer qA
+    DiagArgs d[2];           // [parity] the step's RMS kernel and epilogue
+    ForceArgs f;             // the pending boundary force at the end of a run
+    uint32_t parityFlip;     // 1: the mesh's current state is in qB (its parity is the batch's ^ 1)
+    uint32_t nRmsFull;       // full RMS chunks (k_rms_blocks' nfull)
+    uint32_t pad[2];
+};
+constexpr uint32_t FORCE_ITEM = 0xFFFFFFFFu;
 constexpr uint32_t RMS_CHUNK_BLOCKS = 1024;     // 16-element blocks per workgroup
 void launchRmsChunks(const DiagArgs& d, hipStream_t stream);
 // sums a pending boundary force (end of a time-loop batch); no-op when none is pending
 void launchPendingForce(const ForceArgs& f, DevScalars* sc, hipStream_t stream);
 void launchFinalize(const DiagArgs& d, hipStream_t stream);
 void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStream_t stream);
+
+// Batched launches over many meshes (one launch per kernel for all of them).  The kernel
+// shape (S, flux, viscous, order, limiter mode, precision) is the group's; items are
+// (mesh record, local index).  parity selects each mesh's state buffers (see MemberArgs).
+struct GroupShape {
+    int S, flux, viscous, order, limited, lp, mixed;
+};
+void launchGradientBatch(const GroupShape& g, int limitStage, const MemberArgs* mem, const uint2* items,
+                         uint32_t nItems, uint32_t parity, uint32_t stage, hipStream_t stream);
+void launchFaceTilesBatch(const GroupShape& g, int limitStage, const MemberArgs* mem, const uint2* items,
+                          uint32_t nItems, uint32_t ldsBytes, uint32_t parity, uint32_t stage, hipStream_t stream);
+void launchRmsBatch(const MemberArgs* mem, const uint2* items, uint32_t nItems, uint32_t parity, hipStream_t stream);
+void launchFinalizeBatch(const MemberArgs* mem, const uint32_t* members, uint32_t n, uint32_t parity,
+                         hipStream_t stream);
+void launchPendingForceBatch(const MemberArgs* mem, const uint32_t* members, uint32_t n, hipStream_t stream);
+// copies each listed mesh's DevScalars into out[k] (one host copy for all)
+void launchGatherScalars(const MemberArgs* mem, const uint32_t* members, uint32_t n, DevScalars* out,
+                         hipStream_t stream);
+// FP32 shadow q32[0..n) of q[0..n) (mixed precision)
+void launchShadow(const double4* q, float4* q32, uint32_t n, hipStream_t stream);
 
 }  // namespace ebb

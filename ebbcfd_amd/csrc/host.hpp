@@ -77,6 +77,8 @@ struct Config {
     // extensions (namespaced key "extensions"; absent from reference configs)
     bool mms = false;                 // manufactured solution: dirichlet = exact, source per stage
     long long maxSteps = -1;          // stop after this many steps (-1: only tEnd)
+    double rmsDrop = 0;               // stop once the rho RMS residual drops by this factor below its
+                                      // first-step value (0: never)
     int precision = 0;                // 0 = fp64, 1 = mixed fp32 flux/reconstruction/limiter
     int stagePath = 1;                // 0: gradient, face, update kernels; 1: gradient + face/update tiles;
                                       // 2: one fused kernel per stage

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "dev.hpp"
 
@@ -41,22 +42,31 @@ DEV uint32_t xcdBlock(uint32_t b, uint32_t nb)
     return x * per + (x < rem ? x : rem) + k;
 }
 
-DEV double pressure(const double q[4])
+// The face, reconstruction and limiter arithmetic below is written once for the
+// real type R it computes in: double on the FP64 path, float on the mixed path
+// (extensions.precision = "mixed").  Every literal is converted to R, so the double
+// instantiation performs exactly the operations of the FP64 path and the float one
+// never promotes to double.
+template <typename R>
+DEV R pressure(const R q[4])
 {
-    const double a = q[1] / q[0], b = q[2] / q[0];
-    return (G_ - 1) * (q[3] - 0.5 * q[0] * (a * a + b * b));
+    const R a = q[1] / q[0], b = q[2] / q[0];
+    return R(G_ - 1) * (q[3] - R(0.5) * q[0] * (a * a + b * b));
 }
 
-DEV double dmax(double a, double b) { return a < b ? b : a; }
+template <typename R>
+DEV R dmax(R a, R b) { return a < b ? b : a; }
 
-DEV double harten(double lam, double eps)
+template <typename R>
+DEV R harten(R lam, R eps)
 {
-    if (lam < eps) lam = (eps * eps + lam * lam) / (2.0 * eps);
+    if (lam < eps) lam = (eps * eps + lam * lam) / (R(2.0) * eps);
     return lam;
 }
 
 // flux.d:19-24
-DEV void convective(double p, double u, double v, double rho, double e, double n0, double n1, double out[4])
+template <typename R>
+DEV void convective(R p, R u, R v, R rho, R e, R n0, R n1, R out[4])
 {
     out[0] = (rho * u) * n0 + (rho * v) * n1;
     out[1] = (p + rho * (u * u)) * n0 + (rho * u * v) * n1;
@@ -65,117 +75,124 @@ DEV void convective(double p, double u, double v, double rho, double e, double n
 }
 
 // flux.d:224-303
-DEV void roeFlux(const double qL[4], const double qR[4], double n0, double n1, double out[4], double& sMax)
+template <typename R>
+DEV void roeFlux(const R qL[4], const R qR[4], R n0, R n1, R out[4], R& sMax)
 {
-    const double rhoL = qL[0], uL = qL[1] / rhoL, vL = qL[2] / rhoL;
-    const double rvL0 = rhoL * uL, rvL1 = rhoL * vL;
-    const double pL = ((G_ - 1) * (qL[3] - 0.5 * rhoL * (uL * uL + vL * vL)));
-    const double hL = (qL[3] + pL) / rhoL;
-    const double rhoR = qR[0], uR = qR[1] / rhoR, vR = qR[2] / rhoR;
-    const double rvR0 = rhoR * uR, rvR1 = rhoR * vR;
-    const double pR = ((G_ - 1) * (qR[3] - 0.5 * rhoR * (uR * uR + vR * vR)));
-    const double hR = (qR[3] + pR) / rhoR;
-    const double di = sqrt(rhoR / rhoL);
-    const double d1 = 1.0 / (1.0 + di);
-    const double vel0 = (di * uR + uL) * d1, vel1 = (di * vR + vL) * d1;
-    const double u = vel0 * n0 + vel1 * n1;
-    const double h = (di * hR + hL) * d1;
-    const double a = sqrt((G_ - 1.0) * (h - 0.5 * (vel0 * vel0 + vel1 * vel1)));
-    const double dRhoV0 = rvR0 - rvL0, dRhoV1 = rvR1 - rvL1;
-    const double dRho = rhoR - rhoL, dRhoE = qR[3] - qL[3];
-    const double eps = 0.01 * a;
-    const double lam1 = harten(fabs(u + a), eps);
-    const double lam2 = harten(fabs(u - a), eps);
-    const double lam3 = harten(fabs(u), eps);
-    const double lam4 = harten(fabs(u), eps);
-    sMax = dmax(dmax(dmax(fabs(lam1), fabs(lam2)), fabs(lam3)), fabs(lam4));
-    const double s1 = 0.5 * (lam1 + lam2), s2 = 0.5 * (lam1 - lam2);
-    const double qq = sqrt(vel0 * vel0 + vel1 * vel1);
-    const double G1 = (G_ - 1) * (((qq * qq) / 2.0) * dRho - (vel0 * dRhoV0 + vel1 * dRhoV1) + dRhoE);
-    const double G2 = -u * dRho + (dRhoV0 * n0 + dRhoV1 * n1);
-    const double C1 = (G1 / (a * a)) * (s1 - lam3) + G2 / a * s2;
-    const double C2 = (G1 / a) * s2 + (s1 - lam3) * G2;
-    double fL[4], fR[4];
-    convective(pL, uL, vL, rhoL, qL[3], n0, n1, fL);
-    convective(pR, uR, vR, rhoR, qR[3], n0, n1, fR);
-    const double d[4] = {lam3 * dRho + C1, lam3 * dRhoV0 + C1 * vel0 + C2 * n0,
-                         lam3 * dRhoV1 + C1 * vel1 + C2 * n1, lam3 * dRhoE + C1 * h + C2 * u};
+    const R gm1 = R(G_ - 1), half = R(0.5);
+    const R rhoL = qL[0], uL = qL[1] / rhoL, vL = qL[2] / rhoL;
+    const R rvL0 = rhoL * uL, rvL1 = rhoL * vL;
+    const R pL = (gm1 * (qL[3] - half * rhoL * (uL * uL + vL * vL)));
+    const R hL = (qL[3] + pL) / rhoL;
+    const R rhoR = qR[0], uR = qR[1] / rhoR, vR = qR[2] / rhoR;
+    const R rvR0 = rhoR * uR, rvR1 = rhoR * vR;
+    const R pR = (gm1 * (qR[3] - half * rhoR * (uR * uR + vR * vR)));
+    const R hR = (qR[3] + pR) / rhoR;
+    const R di = sqrt(rhoR / rhoL);
+    const R d1 = R(1.0) / (R(1.0) + di);
+    const R vel0 = (di * uR + uL) * d1, vel1 = (di * vR + vL) * d1;
+    const R u = vel0 * n0 + vel1 * n1;
+    const R h = (di * hR + hL) * d1;
+    const R a = sqrt(R(G_ - 1.0) * (h - half * (vel0 * vel0 + vel1 * vel1)));
+    const R dRhoV0 = rvR0 - rvL0, dRhoV1 = rvR1 - rvL1;
+    const R dRho = rhoR - rhoL, dRhoE = qR[3] - qL[3];
+    const R eps = R(0.01) * a;
+    const R lam1 = harten<R>(fabs(u + a), eps);
+    const R lam2 = harten<R>(fabs(u - a), eps);
+    const R lam3 = harten<R>(fabs(u), eps);
+    const R lam4 = harten<R>(fabs(u), eps);
+    sMax = dmax<R>(dmax<R>(dmax<R>(fabs(lam1), fabs(lam2)), fabs(lam3)), fabs(lam4));
+    const R s1 = half * (lam1 + lam2), s2 = half * (lam1 - lam2);
+    const R qq = sqrt(vel0 * vel0 + vel1 * vel1);
+    const R G1 = gm1 * (((qq * qq) / R(2.0)) * dRho - (vel0 * dRhoV0 + vel1 * dRhoV1) + dRhoE);
+    const R G2 = -u * dRho + (dRhoV0 * n0 + dRhoV1 * n1);
+    const R C1 = (G1 / (a * a)) * (s1 - lam3) + G2 / a * s2;
+    const R C2 = (G1 / a) * s2 + (s1 - lam3) * G2;
+    R fL[4], fR[4];
+    convective<R>(pL, uL, vL, rhoL, qL[3], n0, n1, fL);
+    convective<R>(pR, uR, vR, rhoR, qR[3], n0, n1, fR);
+    const R d[4] = {lam3 * dRho + C1, lam3 * dRhoV0 + C1 * vel0 + C2 * n0,
+                    lam3 * dRhoV1 + C1 * vel1 + C2 * n1, lam3 * dRhoE + C1 * h + C2 * u};
 #pragma unroll
-    for (int k = 0; k < 4; k++) out[k] = 0.5 * (fL[k] + fR[k]) - 0.5 * d[k];
+    for (int k = 0; k < 4; k++) out[k] = half * (fL[k] + fR[k]) - half * d[k];
 }
 
 // flux.d:66-141 (Roe-type averages, C1 = C2 = 0: central flux + sMax jump)
-DEV void rusanovFlux(const double qL[4], const double qR[4], double n0, double n1, double out[4], double& sMax)
+template <typename R>
+DEV void rusanovFlux(const R qL[4], const R qR[4], R n0, R n1, R out[4], R& sMax)
 {
-    const double rhoL = qL[0], uL = qL[1] / rhoL, vL = qL[2] / rhoL;
-    const double rvL0 = rhoL * uL, rvL1 = rhoL * vL;
-    const double pL = ((G_ - 1) * (qL[3] - 0.5 * rhoL * (uL * uL + vL * vL)));
-    const double hL = (qL[3] + pL) / rhoL;
-    const double rhoR = qR[0], uR = qR[1] / rhoR, vR = qR[2] / rhoR;
-    const double rvR0 = rhoR * uR, rvR1 = rhoR * vR;
-    const double pR = ((G_ - 1) * (qR[3] - 0.5 * rhoR * (uR * uR + vR * vR)));
-    const double hR = (qR[3] + pR) / rhoR;
-    const double rho = sqrt(rhoL * rhoR);
-    const double sL = sqrt(rhoL), sR = sqrt(rhoR);
-    const double vel0 = (sL * uL + sR * uR) / (sL + sR);
-    const double vel1 = (sL * vL + sR * vR) / (sL + sR);
-    const double u = vel0 * n0 + vel1 * n1;
-    const double p = (sL * pL + sR * pR) / (sL + sR);
-    const double h = (sL * hL + sR * hR) / (sL + sR);
-    const double a = sqrt(G_ * (p / rho));
-    const double dRhoV0 = rvR0 - rvL0, dRhoV1 = rvR1 - rvL1;
-    const double dRho = rhoR - rhoL, dRhoE = qR[3] - qL[3];
-    const double eps = 0.01 * a;
-    const double lam1 = harten(fabs(u + a), eps);
-    const double lam2 = harten(fabs(u - a), eps);
-    const double lam3 = harten(fabs(u), eps);
-    const double lam4 = harten(fabs(u), eps);
-    const double sm = dmax(dmax(dmax(fabs(lam1), fabs(lam2)), fabs(lam3)), fabs(lam4));
+    const R gm1 = R(G_ - 1), half = R(0.5);
+    const R rhoL = qL[0], uL = qL[1] / rhoL, vL = qL[2] / rhoL;
+    const R rvL0 = rhoL * uL, rvL1 = rhoL * vL;
+    const R pL = (gm1 * (qL[3] - half * rhoL * (uL * uL + vL * vL)));
+    const R hL = (qL[3] + pL) / rhoL;
+    const R rhoR = qR[0], uR = qR[1] / rhoR, vR = qR[2] / rhoR;
+    const R rvR0 = rhoR * uR, rvR1 = rhoR * vR;
+    const R pR = (gm1 * (qR[3] - half * rhoR * (uR * uR + vR * vR)));
+    const R hR = (qR[3] + pR) / rhoR;
+    const R rho = sqrt(rhoL * rhoR);
+    const R sL = sqrt(rhoL), sR = sqrt(rhoR);
+    const R vel0 = (sL * uL + sR * uR) / (sL + sR);
+    const R vel1 = (sL * vL + sR * vR) / (sL + sR);
+    const R u = vel0 * n0 + vel1 * n1;
+    const R p = (sL * pL + sR * pR) / (sL + sR);
+    const R h = (sL * hL + sR * hR) / (sL + sR);
+    const R a = sqrt(R(G_) * (p / rho));
+    const R dRhoV0 = rvR0 - rvL0, dRhoV1 = rvR1 - rvL1;
+    const R dRho = rhoR - rhoL, dRhoE = qR[3] - qL[3];
+    const R eps = R(0.01) * a;
+    const R lam1 = harten<R>(fabs(u + a), eps);
+    const R lam2 = harten<R>(fabs(u - a), eps);
+    const R lam3 = harten<R>(fabs(u), eps);
+    const R lam4 = harten<R>(fabs(u), eps);
+    const R sm = dmax<R>(dmax<R>(dmax<R>(fabs(lam1), fabs(lam2)), fabs(lam3)), fabs(lam4));
     sMax = sm;
-    const double s1 = 0.5 * (sm + sm), s2 = 0.5 * (sm - sm);
-    const double qq = sqrt(vel0 * vel0 + vel1 * vel1);
-    const double G1 = (G_ - 1) * ((qq * qq / 2) * dRho - (vel0 * dRhoV0 + vel1 * dRhoV1) + dRhoE);
-    const double G2 = -u * dRho + (dRhoV0 * n0 + dRhoV1 * n1);
-    const double C1 = (G1 / (a * a)) * (s1 - sm) + G2 / a * s2;
-    const double C2 = (G1 / a) * s2 + (s1 - sm) * G2;
-    double fL[4], fR[4];
-    convective(pL, uL, vL, rhoL, qL[3], n0, n1, fL);
-    convective(pR, uR, vR, rhoR, qR[3], n0, n1, fR);
-    const double d[4] = {sm * dRho + C1, sm * (dRhoV0) + C1 * vel0 + C2 * n0,
-                         sm * (dRhoV1) + C1 * vel1 + C2 * n1, sm * dRhoE + C1 * h + C2 * u};
+    const R s1 = half * (sm + sm), s2 = half * (sm - sm);
+    const R qq = sqrt(vel0 * vel0 + vel1 * vel1);
+    const R G1 = gm1 * ((qq * qq / R(2)) * dRho - (vel0 * dRhoV0 + vel1 * dRhoV1) + dRhoE);
+    const R G2 = -u * dRho + (dRhoV0 * n0 + dRhoV1 * n1);
+    const R C1 = (G1 / (a * a)) * (s1 - sm) + G2 / a * s2;
+    const R C2 = (G1 / a) * s2 + (s1 - sm) * G2;
+    R fL[4], fR[4];
+    convective<R>(pL, uL, vL, rhoL, qL[3], n0, n1, fL);
+    convective<R>(pR, uR, vR, rhoR, qR[3], n0, n1, fR);
+    const R d[4] = {sm * dRho + C1, sm * (dRhoV0) + C1 * vel0 + C2 * n0,
+                    sm * (dRhoV1) + C1 * vel1 + C2 * n1, sm * dRhoE + C1 * h + C2 * u};
 #pragma unroll
-    for (int k = 0; k < 4; k++) out[k] = 0.5 * (fL[k] + fR[k]) - 0.5 * d[k];
+    for (int k = 0; k < 4; k++) out[k] = half * (fL[k] + fR[k]) - half * d[k];
 }
 
-template <int FLUX>
-DEV void numFlux(const double qL[4], const double qR[4], double n0, double n1, double out[4], double& sMax)
+template <int FLUX, typename R>
+DEV void numFlux(const R qL[4], const R qR[4], R n0, R n1, R out[4], R& sMax)
 {
-    if (FLUX == 1) roeFlux(qL, qR, n0, n1, out, sMax);
-    else rusanovFlux(qL, qR, n0, n1, out, sMax);
+    if (FLUX == 1) roeFlux<R>(qL, qR, n0, n1, out, sMax);
+    else rusanovFlux<R>(qL, qR, n0, n1, out, sMax);
 }
 
 // flux.d:28-64.  The four 4x4 matrices are applied entry by entry in column
 // order; their structural zeros only ever add a signed zero, so they are skipped.
-DEV void viscousFlux(double Pr, double mu, const double q[4], const double dq[8], double n0, double n1, double out[4])
+// Pr and mu are physical parameters (FP64), converted to R.
+template <typename R>
+DEV void viscousFlux(double Prd, double mud, const R q[4], const R dq[8], R n0, R n1, R out[4])
 {
-    const double rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3] / q[0];
-    const double gp = G_ / Pr;
-    const double s = mu / rho;
-    const double c43 = 4.0 / 3.0, c23 = 2.0 / 3.0, cm43 = -4.0 / 3.0, cm23 = -2.0 / 3.0, c231 = (2.0 / 3.0 - 1.0);
-    const double t0 = (-4.0 / 3.0 + gp) * (u * u) + (-1.0 + gp) * (v * v) - gp * E;
-    const double t1 = (-1.0 + gp) * (u * u) + (-4.0 / 3.0 + gp) * (v * v) - gp * E;
-    const double x0 = dq[0], x1 = dq[2], x2 = dq[4], x3 = dq[6];
-    const double y0 = dq[1], y1 = dq[3], y2 = dq[5], y3 = dq[7];
-    const double uv = c231 * u * v;
-    const double G01 = ((s * (cm43 * u)) * x0 + (s * c43) * x1) + ((s * (c23 * v)) * y0 + (s * cm23) * y2);
-    const double G02 = ((s * -v) * x0 + (s * 1.0) * x2) + ((s * -u) * y0 + (s * 1.0) * y1);
-    const double G03 = ((s * t0) * x0 + (s * ((c43 - gp) * u)) * x1 + (s * ((1.0 - gp) * v)) * x2 + (s * gp) * x3)
-                     + ((s * uv) * y0 + (s * v) * y1 + (s * (cm23 * u)) * y2);
-    const double G11 = ((s * -v) * x0 + (s * 1.0) * x2) + ((s * -u) * y0 + (s * 1.0) * y1);
-    const double G12 = ((s * (c23 * u)) * x0 + (s * cm23) * x2) + ((s * (cm43 * v)) * y0 + (s * c43) * y2);
-    const double G13 = ((s * uv) * x0 + (s * (cm23 * v)) * x1 + (s * u) * x2)
-                     + ((s * t1) * y0 + (s * ((1.0 - gp) * u)) * y1 + (s * ((c43 - gp) * v)) * y2 + (s * gp) * y3);
-    out[0] = 0.0;
+    const R rho = q[0], u = q[1] / q[0], v = q[2] / q[0], E = q[3] / q[0];
+    const R gp = R(G_) / R(Prd);
+    const R s = R(mud) / rho;
+    const R one = R(1.0);
+    const R c43 = R(4.0 / 3.0), c23 = R(2.0 / 3.0), cm43 = R(-4.0 / 3.0), cm23 = R(-2.0 / 3.0), c231 = R(2.0 / 3.0 - 1.0);
+    const R t0 = (cm43 + gp) * (u * u) + (-one + gp) * (v * v) - gp * E;
+    const R t1 = (-one + gp) * (u * u) + (cm43 + gp) * (v * v) - gp * E;
+    const R x0 = dq[0], x1 = dq[2], x2 = dq[4], x3 = dq[6];
+    const R y0 = dq[1], y1 = dq[3], y2 = dq[5], y3 = dq[7];
+    const R uv = c231 * u * v;
+    const R G01 = ((s * (cm43 * u)) * x0 + (s * c43) * x1) + ((s * (c23 * v)) * y0 + (s * cm23) * y2);
+    const R G02 = ((s * -v) * x0 + (s * one) * x2) + ((s * -u) * y0 + (s * one) * y1);
+    const R G03 = ((s * t0) * x0 + (s * ((c43 - gp) * u)) * x1 + (s * ((one - gp) * v)) * x2 + (s * gp) * x3)
+                + ((s * uv) * y0 + (s * v) * y1 + (s * (cm23 * u)) * y2);
+    const R G11 = ((s * -v) * x0 + (s * one) * x2) + ((s * -u) * y0 + (s * one) * y1);
+    const R G12 = ((s * (c23 * u)) * x0 + (s * cm23) * x2) + ((s * (cm43 * v)) * y0 + (s * c43) * y2);
+    const R G13 = ((s * uv) * x0 + (s * (cm23 * v)) * x1 + (s * u) * x2)
+                + ((s * t1) * y0 + (s * ((one - gp) * u)) * y1 + (s * ((c43 - gp) * v)) * y2 + (s * gp) * y3);
+    out[0] = R(0.0);
     out[1] = G01 * n0 + G11 * n1;
     out[2] = G02 * n0 + G12 * n1;
     out[3] = G03 * n0 + G13 * n1;
@@ -190,23 +207,34 @@ DEV void st4(double4* p, uint32_t i, const double o[4]) { p[i] = make_double4(o[
 // gradients and limiters are read once, by the next kernel: stored non-temporally so
 // they do not displace the neighbour data that kernel's gathers reuse from L2
 typedef double ntd2 __attribute__((ext_vector_type(2)));
+typedef float ntf4 __attribute__((ext_vector_type(4)));
+
+// the FP32 shadow of a state row (mixed path): the stage's gathers read it in place of the FP64 row
+DEV void ld4(const float4* p, uint32_t i, float o[4])
+{
+    const float4 t = p[i];
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+DEV void st4(float4* p, uint32_t i, const float o[4]) { p[i] = make_float4(o[0], o[1], o[2], o[3]); }
 
 // ghost state from the owner state (solve.d:520-689); the owner is the only writer
 // false where the reference would stop: a total-conditions inflow with no non-negative Mach
 // root (solve.d:661-670)
-DEV bool ghostState(const BFace& b, const double qo[4], double qg[4], const double qgOld[4])
+template <typename R>
+DEV bool ghostState(const BFace& b, const R qo[4], R qg[4], const R qgOld[4])
 {
+    const R g = R(G_), gm1 = R(G_ - 1), half = R(0.5), two = R(2.0);
     switch (b.type) {
     case BT_DIRICHLET:
-        qg[0] = b.gq[0]; qg[1] = b.gq[1]; qg[2] = b.gq[2]; qg[3] = b.gq[3];
+        qg[0] = R(b.gq[0]); qg[1] = R(b.gq[1]); qg[2] = R(b.gq[2]); qg[3] = R(b.gq[3]);
         break;
     case BT_INVWALL:
     case BT_SYM: {
-        const double v0 = qo[1] / qo[0], v1 = qo[2] / qo[0];
-        double w0, w1;
+        const R v0 = qo[1] / qo[0], v1 = qo[2] / qo[0];
+        R w0, w1;
         if (!b.vertical) {
-            w0 = 0.0 + b.R[0] * v0 + b.R[1] * v1;
-            w1 = 0.0 + b.R[2] * v0 + b.R[3] * v1;
+            w0 = R(0.0) + R(b.R[0]) * v0 + R(b.R[1]) * v1;
+            w1 = R(0.0) + R(b.R[2]) * v0 + R(b.R[3]) * v1;
         } else {
             w0 = -v0;
             w1 = v1;
@@ -218,20 +246,20 @@ DEV bool ghostState(const BFace& b, const double qo[4], double qg[4], const doub
         qg[0] = qo[0]; qg[1] = -qo[1]; qg[2] = -qo[2]; qg[3] = qo[3];
         break;
     case BT_CONSTP: {
-        const double rhop = qo[0], up = qo[1] / qo[0], vp = qo[2] / qo[0];
-        const double pp = pressure(qo);
-        const double ap = sqrt(G_ * (pp / rhop));
-        const double Jp = up + (2.0 * ap) / (G_ - 1);
-        const double Sp = pp / pow(rhop, G_);
-        const double pb = b.data[0];
-        const double rhob = pow(pb / Sp, 1.0 / G_);
-        const double ab = sqrt(G_ * (pb / rhob));
-        const double ub = Jp - (2.0 * ab) / (G_ - 1);
-        const double vb = vp;
-        const double mag = sqrt(ub * ub + vb * vb);
-        const double rEb = pb / (G_ - 1) + 0.5 * rhob * (mag * mag);
-        const double n0 = b.nrm[0], n1 = b.nrm[1];
-        const double vpn = up * n0 + vp * n1, vbn = ub * n0 + vb * n1;
+        const R rhop = qo[0], up = qo[1] / qo[0], vp = qo[2] / qo[0];
+        const R pp = pressure<R>(qo);
+        const R ap = sqrt(g * (pp / rhop));
+        const R Jp = up + (two * ap) / gm1;
+        const R Sp = pp / pow(rhop, g);
+        const R pb = R(b.data[0]);
+        const R rhob = pow(pb / Sp, R(1.0) / g);
+        const R ab = sqrt(g * (pb / rhob));
+        const R ub = Jp - (two * ab) / gm1;
+        const R vb = vp;
+        const R mag = sqrt(ub * ub + vb * vb);
+        const R rEb = pb / gm1 + half * rhob * (mag * mag);
+        const R n0 = R(b.nrm[0]), n1 = R(b.nrm[1]);
+        const R vpn = up * n0 + vp * n1, vbn = ub * n0 + vb * n1;
         qg[0] = rhob;
         qg[1] = rhob * (up - vpn * n0 + vbn * n0);
         qg[2] = rhob * (vp - vpn * n1 + vbn * n1);
@@ -240,31 +268,31 @@ DEV bool ghostState(const BFace& b, const double qo[4], double qg[4], const doub
     }
     case BT_TPIN: {
         // solve.d:625-683, including its four writes to component 0
-        const double Tt = b.data[0], Pt = b.data[1], ni0 = b.data[2], ni1 = b.data[3];
-        const double rhop = qo[0], up = qo[1] / qo[0];
-        const double pp = pressure(qo);
-        const double ap = sqrt(G_ * (pp / rhop));
-        const double Jp = up + (2.0 * ap) / (G_ - 1);
-        const double d = ni0 * b.nrm[0] + ni1 * b.nrm[1];
-        const double Rg = b.Rgas;
-        const double A = G_ * Rg * Tt * (d * d) - 0.5 * (G_ - 1) * (Jp * Jp);
-        const double B = (4.0 * G_ * Rg * Tt * d) / (G_ - 1);
-        const double Cc = (4.0 * G_ * Rg * Tt) / ((G_ - 1) * (G_ - 1)) - Jp * Jp;
-        const double M1 = (-B + sqrt(B * B - 4 * A * Cc)) / (2.0 * A);
-        const double M2 = (-B - sqrt(B * B - 4 * A * Cc)) / (2.0 * A);
-        double Mb = NAN;
+        const R Tt = R(b.data[0]), Pt = R(b.data[1]), ni0 = R(b.data[2]), ni1 = R(b.data[3]);
+        const R rhop = qo[0], up = qo[1] / qo[0];
+        const R pp = pressure<R>(qo);
+        const R ap = sqrt(g * (pp / rhop));
+        const R Jp = up + (two * ap) / gm1;
+        const R d = ni0 * R(b.nrm[0]) + ni1 * R(b.nrm[1]);
+        const R Rg = R(b.Rgas);
+        const R A = g * Rg * Tt * (d * d) - half * gm1 * (Jp * Jp);
+        const R B = (R(4.0) * g * Rg * Tt * d) / gm1;
+        const R Cc = (R(4.0) * g * Rg * Tt) / (gm1 * gm1) - Jp * Jp;
+        const R M1 = (-B + sqrt(B * B - R(4) * A * Cc)) / (two * A);
+        const R M2 = (-B - sqrt(B * B - R(4) * A * Cc)) / (two * A);
+        R Mb = R(NAN);
         bool rootFound = true;
         if (M1 >= 0 && M2 >= 0) Mb = fmin(M1, M2);
         else if (M1 >= 0 && M2 < 0) Mb = M1;
         else if (M1 < 0 && M2 >= 0) Mb = M2;
         else rootFound = false;   // the reference stops here (enforce)
-        const double Tb = Tt / (1 + 0.5 * (G_ - 1) * (Mb * Mb));
-        const double pb = Pt * pow(Tb / Tt, G_ / (G_ - 1));
-        const double rhob = pb / (Rg * Tb);
-        const double ab = sqrt(G_ * (pb / rhob));
-        const double vb0 = Mb * ab * ni0, vb1 = Mb * ab * ni1;
-        const double mag = sqrt(vb0 * vb0 + vb1 * vb1);
-        qg[0] = pb / (G_ - 1) + 0.5 * rhob * (mag * mag);
+        const R Tb = Tt / (R(1) + half * gm1 * (Mb * Mb));
+        const R pb = Pt * pow(Tb / Tt, g / gm1);
+        const R rhob = pb / (Rg * Tb);
+        const R ab = sqrt(g * (pb / rhob));
+        const R vb0 = Mb * ab * ni0, vb1 = Mb * ab * ni1;
+        const R mag = sqrt(vb0 * vb0 + vb1 * vb1);
+        qg[0] = pb / gm1 + half * rhob * (mag * mag);
         qg[1] = qgOld[1]; qg[2] = qgOld[2]; qg[3] = qgOld[3];
         return rootFound;
     }
@@ -350,22 +378,31 @@ DEV void lsqWeights(double2 ci, const double2 cn[S], double w[2][S])
 }
 
 // gm: stored weights ((dir, nbr)-major, stride gstride between entries), or null
-// to compute them from the centroids
-template <int S, int MODE>
-DEV void gradLimit(const double qi[4], const double qn[S][4], const double* gm, uint32_t gstride, double2 ci,
-                   const double2 cn[S], const bool real[S], int limit, double lpThresh, double g[4][2], double lx[4],
-                   double ly[4])
+// to compute them from the centroids.  Weights and centroid offsets are geometry,
+// computed in FP64 and converted to R; the gradient, bounds and limiter are R.
+template <int S, int MODE, typename R>
+DEV void gradLimit(const R qi[4], const R qn[S][4], const double* gm, uint32_t gstride, double2 ci,
+                   const double2 cn[S], const bool real[S], int limit, double lpThresh, R g[4][2], R lx[4],
+                   R ly[4])
 {
-    double w[2][S];
-    if (gm) {
+    static_assert(MODE != 3 || sizeof(R) == 8, "the LP vector limiter runs in FP64 only");
+    R w[2][S];
+    {
+        double wd[2][S];
+        if (gm) {
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+                for (int j = 0; j < S; j++) wd[r][j] = gm[(r * S + j) * gstride];
+        } else {
+            lsqWeights<S>(ci, cn, wd);
+        }
 #pragma unroll
         for (int r = 0; r < 2; r++)
 #pragma unroll
-            for (int j = 0; j < S; j++) w[r][j] = gm[(r * S + j) * gstride];
-    } else {
-        lsqWeights<S>(ci, cn, w);
+            for (int j = 0; j < S; j++) w[r][j] = R(wd[r][j]);
     }
-    double mn[4], mx[4], du[4][S];
+    R mn[4], mx[4], du[4][S];
 #pragma unroll
     for (int k = 0; k < 4; k++) { mn[k] = qi[k]; mx[k] = qi[k]; }
 #pragma unroll
@@ -380,57 +417,59 @@ DEV void gradLimit(const double qi[4], const double qn[S][4], const double* gm, 
     for (int k = 0; k < 4; k++)
 #pragma unroll
         for (int r = 0; r < 2; r++) {
-            double s = 0.0;
+            R s = R(0.0);
 #pragma unroll
             for (int j = 0; j < S; j++) s += w[r][j] * du[k][j];
             g[k][r] = s;
         }
 #pragma unroll
-    for (int k = 0; k < 4; k++) { lx[k] = 1.0; ly[k] = 1.0; }
+    for (int k = 0; k < 4; k++) { lx[k] = R(1.0); ly[k] = R(1.0); }
     if (MODE >= 2 && limit) {
-        double dxs[S], dys[S];
+        R dxs[S], dys[S];
 #pragma unroll
         for (int j = 0; j < S; j++) {
-            const double dx = cn[j].x - ci.x, dy = cn[j].y - ci.y;
+            const R dx = R(cn[j].x - ci.x), dy = R(cn[j].y - ci.y);
             dxs[j] = dx; dys[j] = dy;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                double s = 1.0;
-                const double t = g[k][0] * dx + g[k][1] * dy;
-                const double qE = qi[k] + t;
+                R s = R(1.0);
+                const R t = g[k][0] * dx + g[k][1] * dy;
+                const R qE = qi[k] + t;
                 if (qE < mn[k]) s = (mn[k] - qi[k]) / t;
                 else if (qE > mx[k]) s = (mx[k] - qi[k]) / t;
                 lx[k] = fmin(lx[k], s);
                 ly[k] = lx[k];
             }
         }
-        if (MODE == 3 && lpThresh > 0) {
-            int nn = 0;
+        if constexpr (MODE == 3) {
+            if (lpThresh > 0) {
+                int nn = 0;
 #pragma unroll
-            for (int j = 0; j < S; j++) nn += real[j] ? 1 : 0;
-            for (int k = 0; k < 4; k++) {
-                if (lx[k] < lpThresh && (nn == 3 || nn == 4)) {
-                    double A[24], bb[12];
-                    for (int r = 0; r < 24; r++) A[r] = 0.0;
-                    for (int r = 0; r < 12; r++) bb[r] = 0.0;
-                    A[0] = 1; A[3] = 1; A[4] = -1; A[7] = -1;
-                    bb[2] = -1; bb[3] = -1;
-                    int c2 = 0;
-                    for (int j = 0; j < S; j++) {
-                        if (!real[j]) continue;
-                        A[2 * (c2 + 4)] = dxs[j] * g[k][0];
-                        A[2 * (c2 + 4) + 1] = dys[j] * g[k][1];
-                        bb[c2 + 4] = mn[k] - qi[k];
-                        A[2 * (c2 + 5)] = -dxs[j] * g[k][0];
-                        A[2 * (c2 + 5) + 1] = -dys[j] * g[k][1];
-                        bb[c2 + 5] = qi[k] - mx[k];
-                        c2 += 2;
+                for (int j = 0; j < S; j++) nn += real[j] ? 1 : 0;
+                for (int k = 0; k < 4; k++) {
+                    if (lx[k] < lpThresh && (nn == 3 || nn == 4)) {
+                        double A[24], bb[12];
+                        for (int r = 0; r < 24; r++) A[r] = 0.0;
+                        for (int r = 0; r < 12; r++) bb[r] = 0.0;
+                        A[0] = 1; A[3] = 1; A[4] = -1; A[7] = -1;
+                        bb[2] = -1; bb[3] = -1;
+                        int c2 = 0;
+                        for (int j = 0; j < S; j++) {
+                            if (!real[j]) continue;
+                            A[2 * (c2 + 4)] = dxs[j] * g[k][0];
+                            A[2 * (c2 + 4) + 1] = dys[j] * g[k][1];
+                            bb[c2 + 4] = mn[k] - qi[k];
+                            A[2 * (c2 + 5)] = -dxs[j] * g[k][0];
+                            A[2 * (c2 + 5) + 1] = -dys[j] * g[k][1];
+                            bb[c2 + 5] = qi[k] - mx[k];
+                            c2 += 2;
+                        }
+                        const double cc[2] = {-fabs(g[k][0]), -fabs(g[k][1])};
+                        double xk[2] = {0.0, 0.0};
+                        lpSolve(4 + 2 * nn, A, bb, cc, xk);
+                        lx[k] = xk[0];
+                        ly[k] = xk[1];
                     }
-                    const double cc[2] = {-fabs(g[k][0]), -fabs(g[k][1])};
-                    double xk[2] = {0.0, 0.0};
-                    lpSolve(4 + 2 * nn, A, bb, cc, xk);
-                    lx[k] = xk[0];
-                    ly[k] = xk[1];
                 }
             }
         }
@@ -516,31 +555,50 @@ DEV void pendingForce(const ForceArgs& f, DevScalars* sc, double2* fp)
 // MODE 0: ghost states only (order 1); 1: unlimited gradient (no limiter stored);
 // 2: scalar limiter; 3: scalar limiter followed by the LP vector limiter (lim
 // stored as x,y pairs).
-template <int S, int MODE>
-__global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, StageArgs a)
+// MIX 0: FP64 throughout.  MIX 1 (mixed precision): states are read from the FP32
+// shadows, ghosts, gradients and limiters computed in FP32 (least-squares weights and
+// centroid offsets in FP64, converted); gradients and limiters are stored as FP32
+// rows (32 + 16 bytes per cell); each ghost is written to the shadow and, widened, to
+// the FP64 row.
+// LDS of one gradient workgroup: staging of the gradient rows (and of the force faces)
+// and of the limiter rows
+template <int MODE>
+struct GradLds {
+    static constexpr int LWS = MODE == 3 ? 8 : (MODE == 2 ? 4 : 0);
+    double sg[BLK][9];
+    double sl[BLK][LWS + 1];
+};
+
+// One workgroup of the gradient kernel on cell block `bid` of its mesh (the solo and the
+// batched kernels run this same code).
+template <int S, int MODE, int MIX>
+DEV void gradientBody(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t bid, GradLds<MODE>& L)
 {
-    // with a force boundary, workgroup 0 (dispatched first) sums a pending boundary force
-    // of the previous step, whether or not the loop is still active; the others are the
-    // cell blocks
-    constexpr int LWS = MODE == 3 ? 8 : (MODE == 2 ? 4 : 0);
-    __shared__ double sg[BLK][9];   // staging of the gradient rows (and of the force faces)
-    __shared__ double sl[BLK][LWS + 1];
-    static_assert(BLK * 9 >= 2 * FORCE_LDS, "force staging exceeds the gradient staging rows");
-    const uint32_t fw = a.force.n ? 1u : 0u;
-    if (fw && blockIdx.x == 0) {
-        pendingForce<BLK, FORCE_LDS>(a.force, a.sc, reinterpret_cast<double2*>(&sg[0][0]));
-        return;
-    }
+    using R = typename std::conditional<MIX == 1, float, double>::type;
+    constexpr int LWS = GradLds<MODE>::LWS;
+    auto& sg = L.sg;
+    auto& sl = L.sl;
     if (a.checkActive && !a.sc->active) return;
-    const uint32_t bid = xcdBlock(blockIdx.x - fw, gridDim.x - fw);
     if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
     const uint32_t i = bid * BLK + threadIdx.x;
     if (MODE == 0 && i >= m.N) return;
     const bool have = i < m.N;   // no early return below: the staged stores need every thread
-    double g[4][2], lx[4], ly[4];
+    R g[4][2], lx[4], ly[4];
+    auto ldState = [&](uint32_t c, R o[4]) {
+        if constexpr (MIX == 1) ld4(a.qin32, c, o);
+        else ld4(a.qin, c, o);
+    };
+    auto stGhost = [&](uint32_t c, const R o[4]) {
+        if constexpr (MIX == 1) {
+            st4(a.qin32, c, o);
+            a.qin[c] = make_double4(o[0], o[1], o[2], o[3]);
+        } else {
+            st4(a.qin, c, o);
+        }
+    };
     if (have) {
-        double qi[4];
-        ld4(a.qin, i, qi);
+        R qi[4];
+        ldState(i, qi);
         uint32_t nb[S];
 #pragma unroll
         for (int j = 0; j < S; j++) nb[j] = m.nbr[i * S + j];
@@ -549,10 +607,12 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
 #pragma unroll
             for (int j = 0; j < S; j++)
                 if (nb[j] >= m.N) {
-                    double qg[4];
-                    if (!ghostFromOwner(m.bface[nb[j] - m.N], qi, a.qin, nb[j], qg) && a.recordErrors)
+                    const BFace& bf = m.bface[nb[j] - m.N];
+                    R qg[4], old[4] = {0, 0, 0, 0};
+                    if (bf.type == BT_TPIN) ldState(nb[j], old);
+                    if (!ghostState<R>(bf, qi, qg, old) && a.recordErrors)
                         atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
-                    st4(a.qin, nb[j], qg);
+                    stGhost(nb[j], qg);
                 }
             return;
         } else {
@@ -561,55 +621,94 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
             // alone (the owner of its boundary face), so its row still holds the stage
             // buffer's previous ghost state, which TempPresInflow keeps in the components
             // it does not write.
-            double qn[S][4];
+            R qn[S][4];
             double2 cn[S];
 #pragma unroll
-            for (int j = 0; j < S; j++) { ld4(a.qin, nb[j], qn[j]); cn[j] = m.cen[nb[j]]; }
+            for (int j = 0; j < S; j++) { ldState(nb[j], qn[j]); cn[j] = m.cen[nb[j]]; }
             const double2 ci = m.cen[i];
 #pragma unroll
             for (int j = 0; j < S; j++)
                 if (nb[j] >= m.N) {
                     const BFace& bf = m.bface[nb[j] - m.N];
-                    double old[4] = {0, 0, 0, 0};
+                    R old[4] = {0, 0, 0, 0};
                     if (bf.type == BT_TPIN) {
 #pragma unroll
                         for (int k = 0; k < 4; k++) old[k] = qn[j][k];
                     }
-                    if (!ghostState(bf, qi, qn[j], old) && a.recordErrors)
+                    if (!ghostState<R>(bf, qi, qn[j], old) && a.recordErrors)
                         atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
-                    st4(a.qin, nb[j], qn[j]);
+                    stGhost(nb[j], qn[j]);
                 }
             bool real[S];
 #pragma unroll
             for (int j = 0; j < S; j++) real[j] = nb[j] != i;
-            gradLimit<S, MODE>(qi, qn, nullptr, 0, ci, cn, real, a.limit, p.lpThresh, g, lx, ly);
+            gradLimit<S, MODE, R>(qi, qn, nullptr, 0, ci, cn, real, a.limit, p.lpThresh, g, lx, ly);
         }
     }
     if (MODE == 0) return;
     // stage the rows in LDS and store the workgroup's contiguous run of cells with
     // lane-consecutive 16-byte stores
     const uint32_t tid = threadIdx.x;
-    if (have) {
+    const uint32_t base = bid * BLK, n = (m.N - base) < BLK ? (m.N - base) : BLK;
+    if constexpr (MIX == 1) {
+        float* fg = reinterpret_cast<float*>(&sg[0][0]);   // 8 floats per row
+        float* fl = reinterpret_cast<float*>(&sl[0][0]);   // 4 floats per row
+        if (have) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) { sg[tid][2 * c] = g[c][0]; sg[tid][2 * c + 1] = g[c][1]; }
-        if (MODE == 3) {
+            for (int c = 0; c < 4; c++) { fg[8 * tid + 2 * c] = g[c][0]; fg[8 * tid + 2 * c + 1] = g[c][1]; }
+            if (MODE == 2) {
 #pragma unroll
-            for (int c = 0; c < 4; c++) { sl[tid][2 * c] = lx[c]; sl[tid][2 * c + 1] = ly[c]; }
-        } else if (MODE == 2) {
+                for (int c = 0; c < 4; c++) fl[4 * tid + c] = lx[c];
+            }
+        }
+        __syncthreads();
+        float4* gd = reinterpret_cast<float4*>(a.grad32 + size_t(base) * 8);
+        for (uint32_t c = tid; c < 2 * n; c += BLK)
+            __builtin_nontemporal_store(*reinterpret_cast<const ntf4*>(fg + 4 * c), reinterpret_cast<ntf4*>(gd) + c);
+        if (MODE == 2) {
+            float4* ld = reinterpret_cast<float4*>(a.lim32 + size_t(base) * 4);
+            for (uint32_t c = tid; c < n; c += BLK)
+                __builtin_nontemporal_store(*reinterpret_cast<const ntf4*>(fl + 4 * c), reinterpret_cast<ntf4*>(ld) + c);
+        }
+        return;
+    } else {
+        if (have) {
 #pragma unroll
-            for (int c = 0; c < 4; c++) sl[tid][c] = lx[c];
+            for (int c = 0; c < 4; c++) { sg[tid][2 * c] = g[c][0]; sg[tid][2 * c + 1] = g[c][1]; }
+            if (MODE == 3) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) { sl[tid][2 * c] = lx[c]; sl[tid][2 * c + 1] = ly[c]; }
+            } else if (MODE == 2) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) sl[tid][c] = lx[c];
+            }
+        }
+        __syncthreads();
+        double2* gd = reinterpret_cast<double2*>(a.grad + size_t(base) * 8);
+        for (uint32_t c = tid; c < 4 * n; c += BLK) __builtin_nontemporal_store((ntd2){sg[c / 4][2 * (c % 4)], sg[c / 4][2 * (c % 4) + 1]}, reinterpret_cast<ntd2*>(gd) + c);
+        if constexpr (LWS > 0) {
+            double2* ld = reinterpret_cast<double2*>(a.lim + size_t(base) * LWS);
+            constexpr uint32_t PR = LWS / 2;
+            for (uint32_t c = tid; c < PR * n; c += BLK) __builtin_nontemporal_store((ntd2){sl[c / PR][2 * (c % PR)], sl[c / PR][2 * (c % PR) + 1]}, reinterpret_cast<ntd2*>(ld) + c);
         }
     }
-    __syncthreads();
-    const uint32_t base = bid * BLK, n = (m.N - base) < BLK ? (m.N - base) : BLK;
-    double2* gd = reinterpret_cast<double2*>(a.grad + size_t(base) * 8);
-    for (uint32_t c = tid; c < 4 * n; c += BLK) __builtin_nontemporal_store((ntd2){sg[c / 4][2 * (c % 4)], sg[c / 4][2 * (c % 4) + 1]}, reinterpret_cast<ntd2*>(gd) + c);
-    if constexpr (LWS > 0) {
-        double2* ld = reinterpret_cast<double2*>(a.lim + size_t(base) * LWS);
-        constexpr uint32_t PR = LWS / 2;
-        for (uint32_t c = tid; c < PR * n; c += BLK) __builtin_nontemporal_store((ntd2){sl[c / PR][2 * (c % PR)], sl[c / PR][2 * (c % PR) + 1]}, reinterpret_cast<ntd2*>(ld) + c);
-    }
     // MODE 1 (unlimited stage): every limiter is 1; the face kernels take LP = 2 and read none
+}
+
+template <int S, int MODE, int MIX>
+__global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, StageArgs a)
+{
+    static_assert(BLK * 9 >= 2 * FORCE_LDS, "force staging exceeds the gradient staging rows");
+    __shared__ GradLds<MODE> L;
+    // with a force boundary, workgroup 0 (dispatched first) sums a pending boundary force
+    // of the previous step, whether or not the loop is still active; the others are the
+    // cell blocks
+    const uint32_t fw = a.force.n ? 1u : 0u;
+    if (fw && blockIdx.x == 0) {
+        pendingForce<BLK, FORCE_LDS>(a.force, a.sc, reinterpret_cast<double2*>(&L.sg[0][0]));
+        return;
+    }
+    gradientBody<S, MODE, MIX>(m, p, a, xcdBlock(blockIdx.x - fw, gridDim.x - fw), L);
 }
 
 // ---------------------------------------------------------------- faces
@@ -631,6 +730,19 @@ DEV void loadLim(const double* lim, uint32_t c, double lx[4], double ly[4])
         lx[0] = ly[0] = a.x; lx[1] = ly[1] = a.y; lx[2] = ly[2] = a.z; lx[3] = ly[3] = a.w;
     }
 }
+// FP32 rows of the mixed path (no LP vector limiter there)
+template <int LP>
+DEV void loadLim(const float* lim, uint32_t c, float lx[4], float ly[4])
+{
+    static_assert(LP != 1, "the LP vector limiter runs in FP64 only");
+    if (LP == 2) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) { lx[k] = 1.0f; ly[k] = 1.0f; }
+    } else {
+        const float4 a = *reinterpret_cast<const float4*>(lim + size_t(c) * 4);
+        lx[0] = ly[0] = a.x; lx[1] = ly[1] = a.y; lx[2] = ly[2] = a.z; lx[3] = ly[3] = a.w;
+    }
+}
 
 DEV void loadGrad(const double* grad, uint32_t c, double g[8])
 {
@@ -638,16 +750,21 @@ DEV void loadGrad(const double* grad, uint32_t c, double g[8])
     const double4 b = *reinterpret_cast<const double4*>(grad + size_t(c) * 8 + 4);
     g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w; g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
 }
+DEV void loadGrad(const float* grad, uint32_t c, float g[8])
+{
+    const float4 a = *reinterpret_cast<const float4*>(grad + size_t(c) * 8);
+    const float4 b = *reinterpret_cast<const float4*>(grad + size_t(c) * 8 + 4);
+    g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w; g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+}
 
 // solve.d:980-1010 (interior) reconstruction with the p<0 fallback
-template <int LP>
-DEV void reconstruct(const double qm[4], const double g[8], const double lx[4], const double ly[4],
-                     double dx, double dy, double out[4])
+template <int LP, typename R>
+DEV void reconstruct(const R qm[4], const R g[8], const R lx[4], const R ly[4], R dx, R dy, R out[4])
 {
 #pragma unroll
     for (int j = 0; j < 4; j++) out[j] = qm[j] + lx[j] * g[2 * j] * dx + ly[j] * g[2 * j + 1] * dy;
-    if (pressure(out) < 0) {
-        double l0 = 1.0, l1 = 1.0;
+    if (pressure<R>(out) < 0) {
+        R l0 = R(1.0), l1 = R(1.0);
 #pragma unroll
         for (int j = 0; j < 4; j++) { l0 = fmin(l0, lx[j]); l1 = fmin(l1, ly[j]); }
 #pragma unroll
@@ -655,51 +772,53 @@ DEV void reconstruct(const double qm[4], const double g[8], const double lx[4], 
     }
 }
 
-// solve.d:970-1033: one interior face from both cells' raw data
-template <int FLUX, int VISC, int ORDER, int LP>
-DEV void interiorFace(const double qa[4], const double ga[8], const double lxa[4], const double lya[4], double2 ca,
-                      const double qb[4], const double gb[8], const double lxb[4], const double lyb[4], double2 cb,
-                      double2 n, double2 mid, const PhysParams& p, double qL[4], double qR[4], double F[4],
-                      double& sMax)
+// solve.d:970-1033: one interior face from both cells' raw data.  Centroids, midpoint
+// and normal are FP64 geometry; the offsets and the normal enter the R arithmetic.
+template <int FLUX, int VISC, int ORDER, int LP, typename R>
+DEV void interiorFace(const R qa[4], const R ga[8], const R lxa[4], const R lya[4], double2 ca,
+                      const R qb[4], const R gb[8], const R lxb[4], const R lyb[4], double2 cb,
+                      double2 n, double2 mid, const PhysParams& p, R qL[4], R qR[4], R F[4], R& sMax)
 {
     if (ORDER > 1) {
-        reconstruct<LP>(qa, ga, lxa, lya, mid.x - ca.x, mid.y - ca.y, qL);
-        reconstruct<LP>(qb, gb, lxb, lyb, mid.x - cb.x, mid.y - cb.y, qR);
+        reconstruct<LP, R>(qa, ga, lxa, lya, R(mid.x - ca.x), R(mid.y - ca.y), qL);
+        reconstruct<LP, R>(qb, gb, lxb, lyb, R(mid.x - cb.x), R(mid.y - cb.y), qR);
     } else {
 #pragma unroll
         for (int k = 0; k < 4; k++) { qL[k] = qa[k]; qR[k] = qb[k]; }
     }
     // the face gradient is formed before the convective flux, so the two cell gradients
     // are not live across it
-    double dq[8];
+    R dq[8];
     if (VISC) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) dq[k] = 0.5 * (ga[k] + gb[k]);
+        for (int k = 0; k < 8; k++) dq[k] = R(0.5) * (ga[k] + gb[k]);
     }
-    numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+    numFlux<FLUX, R>(qL, qR, R(n.x), R(n.y), F, sMax);
     if (VISC) {
-        double qa2[4], Fv[4];
+        R qa2[4], Fv[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) qa2[k] = 0.5 * (qL[k] + qR[k]);
-        viscousFlux(p.Pr, p.mu, qa2, dq, n.x, n.y, Fv);
+        for (int k = 0; k < 4; k++) qa2[k] = R(0.5) * (qL[k] + qR[k]);
+        viscousFlux<R>(p.Pr, p.mu, qa2, dq, R(n.x), R(n.y), Fv);
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
     }
 }
 
 // solve.d:283-319 + 769-968: one boundary face from its owner's data and ghost state
-template <int FLUX, int VISC, int ORDER, int LP>
-DEV void boundaryFace(const double qo[4], const double gO[8], const double lx[4], const double ly[4], double2 co,
-                      const double qg[4], const BFace& bf, double2 n, double2 mid, const PhysParams& p,
-                      double qL[4], double qR[4], double F[4], double& sMax)
+template <int FLUX, int VISC, int ORDER, int LP, typename R>
+DEV void boundaryFace(const R qo[4], const R gO[8], const R lx[4], const R ly[4], double2 co,
+                      const R qg[4], const BFace& bf, double2 nd, double2 mid, const PhysParams& p,
+                      R qL[4], R qR[4], R F[4], R& sMax)
 {
+    const R g = R(G_), gm1 = R(G_ - 1), half = R(0.5);
+    const R nx = R(nd.x), ny = R(nd.y);
     // updateBoundaryEdge: the fallback uses lim[0] for both directions
     if (ORDER > 1) {
-        const double dx = mid.x - co.x, dy = mid.y - co.y;
+        const R dx = R(mid.x - co.x), dy = R(mid.y - co.y);
 #pragma unroll
         for (int j = 0; j < 4; j++) qL[j] = qo[j] + lx[j] * gO[2 * j] * dx + ly[j] * gO[2 * j + 1] * dy;
-        if (pressure(qL) < 0) {
-            double l0 = 1.0;
+        if (pressure<R>(qL) < 0) {
+            R l0 = R(1.0);
 #pragma unroll
             for (int j = 0; j < 4; j++) l0 = fmin(l0, lx[j]);
 #pragma unroll
@@ -712,119 +831,146 @@ DEV void boundaryFace(const double qo[4], const double gO[8], const double lx[4]
     const int type = bf.type;
     if (type == BT_FULL || type == BT_DIRICHLET) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) qR[k] = bf.far[k];
-        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+        for (int k = 0; k < 4; k++) qR[k] = R(bf.far[k]);
+        numFlux<FLUX, R>(qL, qR, nx, ny, F, sMax);
         if (VISC) {
-            double qa[4], dq[8], Fv[4];
+            R qa[4], dq[8], Fv[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
+            for (int k = 0; k < 4; k++) qa[k] = half * (qL[k] + qR[k]);
             if (type == BT_DIRICHLET) {
 #pragma unroll
-                for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + bf.gg[k]);
+                for (int k = 0; k < 8; k++) dq[k] = half * (gO[k] + R(bf.gg[k]));
             } else {
 #pragma unroll
-                for (int k = 0; k < 8; k++) dq[k] = 0.5 * (gO[k] + 0.0);
+                for (int k = 0; k < 8; k++) dq[k] = half * (gO[k] + R(0.0));
             }
-            viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
+            viscousFlux<R>(p.Pr, p.mu, qa, dq, nx, ny, Fv);
 #pragma unroll
             for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
         }
     } else if (type == BT_INVWALL) {
-        const double inv = 1 / qL[0];
-        const double v0 = inv * qL[1], v1 = inv * qL[2];
-        const double vn = v0 * n.x + v1 * n.y;
-        const double t0 = v0 - vn * n.x, t1 = v1 - vn * n.y;
-        const double vel = sqrt(t0 * t0 + t1 * t1);
-        double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
-        const double aw = sqrt(G_ * (pw / qL[0]));
-        if (pw < 0) pw = 1.0e-12;
-        F[0] = 0; F[1] = pw * n.x; F[2] = pw * n.y; F[3] = 0;
+        const R inv = R(1) / qL[0];
+        const R v0 = inv * qL[1], v1 = inv * qL[2];
+        const R vn = v0 * nx + v1 * ny;
+        const R t0 = v0 - vn * nx, t1 = v1 - vn * ny;
+        const R vel = sqrt(t0 * t0 + t1 * t1);
+        R pw = gm1 * (qL[3] - half * qL[0] * (vel * vel));
+        const R aw = sqrt(g * (pw / qL[0]));
+        if (pw < 0) pw = R(1.0e-12);
+        F[0] = R(0); F[1] = pw * nx; F[2] = pw * ny; F[3] = R(0);
         sMax = fabs(aw);
         for (int k = 0; k < 4; k++) qR[k] = qg[k];
     } else if (type == BT_VISCWALL) {
-        const double vel = 0.0;
-        double pw = (G_ - 1) * (qL[3] - 0.5 * qL[0] * (vel * vel));
-        const double aw = sqrt(G_ * (pw / qL[0]));
-        if (pw < 0) pw = 1.0e-12;
-        qR[0] = qL[0]; qR[1] = qL[1] * -1; qR[2] = qL[2] * -1; qR[3] = qL[3];
-        double qa[4], dq[8], Fv[4];
+        const R vel = R(0.0);
+        R pw = gm1 * (qL[3] - half * qL[0] * (vel * vel));
+        const R aw = sqrt(g * (pw / qL[0]));
+        if (pw < 0) pw = R(1.0e-12);
+        qR[0] = qL[0]; qR[1] = qL[1] * R(-1); qR[2] = qL[2] * R(-1); qR[3] = qL[3];
+        R qa[4], dq[8], Fv[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-        qa[1] = 0.0; qa[2] = 0.0;
+        for (int k = 0; k < 4; k++) qa[k] = half * (qL[k] + qR[k]);
+        qa[1] = R(0.0); qa[2] = R(0.0);
 #pragma unroll
         for (int k = 0; k < 8; k++) dq[k] = gO[k];
         dq[6] = (qa[3] / qa[0]) * dq[0];
         dq[7] = (qa[3] / qa[0]) * dq[1];
-        viscousFlux(p.Pr, p.mu, qa, dq, n.x, n.y, Fv);
-        F[0] = 0 - Fv[0]; F[1] = pw * n.x - Fv[1]; F[2] = pw * n.y - Fv[2]; F[3] = 0 - Fv[3];
+        viscousFlux<R>(p.Pr, p.mu, qa, dq, nx, ny, Fv);
+        F[0] = R(0) - Fv[0]; F[1] = pw * nx - Fv[1]; F[2] = pw * ny - Fv[2]; F[3] = R(0) - Fv[3];
         sMax = fabs(aw);
     } else if (type == BT_CONSTP) {
         for (int k = 0; k < 4; k++) qR[k] = qg[k];
-        const double pg = pressure(qR);
-        const double ug = qR[1] / qR[0], vg = qR[2] / qR[0];
-        const double ag = sqrt(G_ * (pg / qR[0]));
-        convective(pg, ug, vg, qR[0], qR[3], n.x, n.y, F);
+        const R pg = pressure<R>(qR);
+        const R ug = qR[1] / qR[0], vg = qR[2] / qR[0];
+        const R ag = sqrt(g * (pg / qR[0]));
+        convective<R>(pg, ug, vg, qR[0], qR[3], nx, ny, F);
         sMax = fabs(ag);
     } else if (type == BT_SYM) {
-        const double v0 = qL[1] / qL[0], v1 = qL[2] / qL[0];
-        double w0, w1;
+        const R v0 = qL[1] / qL[0], v1 = qL[2] / qL[0];
+        R w0, w1;
         if (!bf.vertical) {
-            w0 = 0.0 + bf.R[0] * v0 + bf.R[1] * v1;
-            w1 = 0.0 + bf.R[2] * v0 + bf.R[3] * v1;
+            w0 = R(0.0) + R(bf.R[0]) * v0 + R(bf.R[1]) * v1;
+            w1 = R(0.0) + R(bf.R[2]) * v0 + R(bf.R[3]) * v1;
         } else {
             w0 = -v0;
             w1 = v1;
         }
         qR[0] = qL[0]; qR[1] = qL[0] * w0; qR[2] = qL[0] * w1; qR[3] = qL[3];
-        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+        numFlux<FLUX, R>(qL, qR, nx, ny, F, sMax);
         if (VISC) {
-            double qa[4], bg[8], Fv[4];
+            const R one = R(1), zero = R(0), two = R(2);
+            R qa[4], bg[8], Fv[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-            const double o00 = n.x * n.x, o01 = n.x * n.y, o10 = n.y * n.x, o11 = n.y * n.y;
-            const double V00 = 1 - o00, V01 = 0 - o01, V10 = 0 - o10, V11 = 1 - o11;
-            const double T00 = 1 - 2 * o00, T01 = 0 - 2 * o01, T10 = 0 - 2 * o10, T11 = 1 - 2 * o11;
+            for (int k = 0; k < 4; k++) qa[k] = half * (qL[k] + qR[k]);
+            const R o00 = nx * nx, o01 = nx * ny, o10 = ny * nx, o11 = ny * ny;
+            const R V00 = one - o00, V01 = zero - o01, V10 = zero - o10, V11 = one - o11;
+            const R T00 = one - two * o00, T01 = zero - two * o01, T10 = zero - two * o10, T11 = one - two * o11;
             // tmp1 = A*gradp with A = identity except the momentum block V (4x4 * 4x2, all columns)
-            double t1[8];
+            R t1[8];
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                t1[0 + c] = 0.0 + 1.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 0.0 * gO[6 + c];
-                t1[2 + c] = 0.0 + 0.0 * gO[0 + c] + V00 * gO[2 + c] + V01 * gO[4 + c] + 0.0 * gO[6 + c];
-                t1[4 + c] = 0.0 + 0.0 * gO[0 + c] + V10 * gO[2 + c] + V11 * gO[4 + c] + 0.0 * gO[6 + c];
-                t1[6 + c] = 0.0 + 0.0 * gO[0 + c] + 0.0 * gO[2 + c] + 0.0 * gO[4 + c] + 1.0 * gO[6 + c];
+                t1[0 + c] = zero + one * gO[0 + c] + zero * gO[2 + c] + zero * gO[4 + c] + zero * gO[6 + c];
+                t1[2 + c] = zero + zero * gO[0 + c] + V00 * gO[2 + c] + V01 * gO[4 + c] + zero * gO[6 + c];
+                t1[4 + c] = zero + zero * gO[0 + c] + V10 * gO[2 + c] + V11 * gO[4 + c] + zero * gO[6 + c];
+                t1[6 + c] = zero + zero * gO[0 + c] + zero * gO[2 + c] + zero * gO[4 + c] + one * gO[6 + c];
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const double s30 = 0.0 + t1[2 * r] * T00 + t1[2 * r + 1] * T10;
-                const double s31 = 0.0 + t1[2 * r] * T01 + t1[2 * r + 1] * T11;
-                const double s10 = 0.0 + gO[2 * r] * o00 + gO[2 * r + 1] * o10;
-                const double s11 = 0.0 + gO[2 * r] * o01 + gO[2 * r + 1] * o11;
+                const R s30 = zero + t1[2 * r] * T00 + t1[2 * r + 1] * T10;
+                const R s31 = zero + t1[2 * r] * T01 + t1[2 * r + 1] * T11;
+                const R s10 = zero + gO[2 * r] * o00 + gO[2 * r + 1] * o10;
+                const R s11 = zero + gO[2 * r] * o01 + gO[2 * r + 1] * o11;
                 bg[2 * r] = s10 + s30;
                 bg[2 * r + 1] = s11 + s31;
             }
-            viscousFlux(p.Pr, p.mu, qa, bg, n.x, n.y, Fv);
+            viscousFlux<R>(p.Pr, p.mu, qa, bg, nx, ny, Fv);
 #pragma unroll
             for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
         }
     } else {  // TempPresInflow
         for (int k = 0; k < 4; k++) qR[k] = qg[k];
-        numFlux<FLUX>(qL, qR, n.x, n.y, F, sMax);
+        numFlux<FLUX, R>(qL, qR, nx, ny, F, sMax);
         if (VISC) {
-            double qa[4], Fv[4];
+            R qa[4], Fv[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) qa[k] = 0.5 * (qL[k] + qR[k]);
-            viscousFlux(p.Pr, p.mu, qa, gO, n.x, n.y, Fv);
+            for (int k = 0; k < 4; k++) qa[k] = half * (qL[k] + qR[k]);
+            viscousFlux<R>(p.Pr, p.mu, qa, gO, nx, ny, Fv);
 #pragma unroll
             for (int k = 0; k < 4; k++) F[k] = F[k] - Fv[k];
         }
     }
 }
 
-template <int FLUX, int VISC, int ORDER, int LP, int STORE>
+// Cell data a face reads: the FP64 rows, or on the mixed path (MIX 1) the FP32
+// shadow of the state and the FP32 gradient and limiter rows.
+template <int MIX>
+struct FaceSrc {
+    using R = typename std::conditional<MIX == 1, float, double>::type;
+    const StageArgs& a;
+    DEV void q(uint32_t c, R o[4]) const
+    {
+        if constexpr (MIX == 1) ld4(a.qin32, c, o);
+        else ld4(a.qin, c, o);
+    }
+    DEV void grad(uint32_t c, R o[8]) const
+    {
+        if constexpr (MIX == 1) loadGrad(a.grad32, c, o);
+        else loadGrad(a.grad, c, o);
+    }
+    template <int LP>
+    DEV void lim(uint32_t c, R lx[4], R ly[4]) const
+    {
+        if constexpr (MIX == 1) loadLim<LP>(a.lim32, c, lx, ly);
+        else loadLim<LP>(a.lim, c, lx, ly);
+    }
+};
+
+template <int FLUX, int VISC, int ORDER, int LP, int STORE, int MIX>
 __global__ __launch_bounds__(BLK) void k_faces(DevMesh m, PhysParams p, StageArgs a, uint32_t nbi)
 {
+    using R = typename std::conditional<MIX == 1, float, double>::type;
     if (a.checkActive && !a.sc->active) return;
-    double qL[4], qR[4], F[4], sMax;
+    const FaceSrc<MIX> src{a};
+    R qL[4], qR[4], F[4], sMax;
     uint32_t f;
     double len;
     uint64_t errKey;
@@ -833,17 +979,18 @@ __global__ __launch_bounds__(BLK) void k_faces(DevMesh m, PhysParams p, StageArg
         if (f >= m.Fi) return;
         const uint2 c = m.fcell[f];
         len = m.flen[f];
-        double qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
-        ld4(a.qin, c.x, qa);
-        ld4(a.qin, c.y, qb);
-        if (ORDER > 1 || VISC) { loadGrad(a.grad, c.x, ga); loadGrad(a.grad, c.y, gb); }
+        R qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
+        src.q(c.x, qa);
+        src.q(c.y, qb);
+        if (ORDER > 1 || VISC) { src.grad(c.x, ga); src.grad(c.y, gb); }
         double2 ca = make_double2(0, 0), cb = make_double2(0, 0), mid = make_double2(0, 0);
         if (ORDER > 1) {
-            loadLim<LP>(a.lim, c.x, lxa, lya);
-            loadLim<LP>(a.lim, c.y, lxb, lyb);
+            src.template lim<LP>(c.x, lxa, lya);
+            src.template lim<LP>(c.y, lxb, lyb);
             ca = m.cen[c.x]; cb = m.cen[c.y]; mid = m.fmid[f];
         }
-        interiorFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, m.fn[f], mid, p, qL, qR, F, sMax);
+        interiorFace<FLUX, VISC, ORDER, LP, R>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, m.fn[f], mid, p, qL, qR, F,
+                                              sMax);
         errKey = uint64_t(m.Fb) + f;   // boundary edges are checked first in the reference
     } else {
         const uint32_t b = (blockIdx.x - nbi) * BLK + threadIdx.x;
@@ -851,25 +998,26 @@ __global__ __launch_bounds__(BLK) void k_faces(DevMesh m, PhysParams p, StageArg
         f = m.Fi + b;
         const uint32_t o = m.fcell[f].x;
         len = m.flen[f];
-        double qo[4], qg[4], gO[8], lx[4], ly[4];
-        ld4(a.qin, o, qo);
-        ld4(a.qin, m.N + b, qg);
-        if (ORDER > 1 || VISC) loadGrad(a.grad, o, gO);
+        R qo[4], qg[4], gO[8], lx[4], ly[4];
+        src.q(o, qo);
+        src.q(m.N + b, qg);
+        if (ORDER > 1 || VISC) src.grad(o, gO);
         double2 co = make_double2(0, 0), mid = make_double2(0, 0);
         if (ORDER > 1) {
-            loadLim<LP>(a.lim, o, lx, ly);
+            src.template lim<LP>(o, lx, ly);
             co = m.cen[o]; mid = m.fmid[f];
         }
-        boundaryFace<FLUX, VISC, ORDER, LP>(qo, gO, lx, ly, co, qg, m.bface[b], m.fn[f], mid, p, qL, qR, F, sMax);
+        boundaryFace<FLUX, VISC, ORDER, LP, R>(qo, gO, lx, ly, co, qg, m.bface[b], m.fn[f], mid, p, qL, qR, F, sMax);
         errKey = b;
     }
     if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
         atomicMin(&a.sc->errEdge, (uint64_t(a.stageId) << 32) | errKey);
-    *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) = make_double4(len * F[0], len * F[1], len * F[2], len * F[3]);
-    a.lenS[f] = len * sMax;
+    *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) =
+        make_double4(len * double(F[0]), len * double(F[1]), len * double(F[2]), len * double(F[3]));
+    a.lenS[f] = len * double(sMax);
     if (STORE) {
-        st4(a.estate, 2 * f, qL);
-        st4(a.estate, 2 * f + 1, qR);
+        a.estate[2 * f] = make_double4(qL[0], qL[1], qL[2], qL[3]);
+        a.estate[2 * f + 1] = make_double4(qR[0], qR[1], qR[2], qR[3]);
     }
 }
 
@@ -892,6 +1040,13 @@ DEV double warpMax(double v)
 // met: (perimeter, d, -1/area) of the cell, or null to read them from the mesh
 // qNew/qOld (optional): the new state and q^n when this stage completes a step
 // qnI, accI (optional): the cell's q^n and RK accumulator rows already loaded
+// the stage output row, and on the mixed path its FP32 shadow for the next stage's gathers
+DEV void storeOut(const StageArgs& a, uint32_t i, const double o[4])
+{
+    st4(a.qout, i, o);
+    if (a.qout32) a.qout32[i] = make_float4(float(o[0]), float(o[1]), float(o[2]), float(o[3]));
+}
+
 template <int KIND>
 DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t i, const double r[4],
                    double sAve, const double* qinI, const double* met, double& dtCand, double& rCand,
@@ -939,34 +1094,34 @@ DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, ui
     } else if (KIND == SK_EULER) {
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     } else if (KIND == SK_RK2_1) {
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + (dt * R[k]);
         st4(a.acc, i, R);
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     } else if (KIND == SK_RK2_2) {
         double k1[4];
         loadAcc(k1);
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 2.0) * (k1[k] + R[k]);
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     } else if (KIND == SK_TVD_1) {
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + h * R[k];
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     } else if (KIND == SK_TVD_2) {
         double fe[4];
         if (qinI) { fe[0] = qinI[0]; fe[1] = qinI[1]; fe[2] = qinI[2]; fe[3] = qinI[3]; }
         else ld4(a.qin, i, fe);
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = 0.5 * (qn[k] + fe[k] + h * R[k]);
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     } else if (KIND == SK_RK4_1) {
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + ((dt / 2.0) * R[k]);
         st4(a.acc, i, R);
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     } else if (KIND == SK_RK4_2 || KIND == SK_RK4_3) {
         double ac[4];
         loadAcc(ac);
@@ -976,13 +1131,13 @@ DEV void cellStage(const DevMesh& m, const PhysParams& p, const StageArgs& a, ui
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + (c * R[k]);
         st4(a.acc, i, ac);
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     } else if (KIND == SK_RK4_4) {
         double ac[4];
         loadAcc(ac);
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = qn[k] + (h / 6.0) * (ac[k] + R[k]);
-        st4(a.qout, i, o);
+        storeOut(a, i, o);
     }
     if (KIND != SK_RESIDUAL && a.last && a.recordErrors &&
         (isnan(o[0]) || isnan(o[1]) || isnan(o[2]) || isnan(o[3])))
@@ -1021,7 +1176,7 @@ DEV void rmsBlockLeaf(bool have, const double qNew[4], const double qOld[4], uin
 
 // per-workgroup min dt / max |R| partial, combined later by k_reduce_stage (device-scope
 // atomics from every workgroup on one address serialise across the XCDs)
-DEV void blockPartial(double dtCand, double rCand, double* partial)
+DEV void blockPartial(double dtCand, double rCand, double* partial, uint32_t slot)
 {
     __shared__ double sMin[BLK / 64], sMax_[BLK / 64];
     dtCand = warpMin(dtCand);
@@ -1033,8 +1188,8 @@ DEV void blockPartial(double dtCand, double rCand, double* partial)
         double a0 = sMin[0], b0 = sMax_[0];
 #pragma unroll
         for (int k = 1; k < BLK / 64; k++) { a0 = fmin(a0, sMin[k]); b0 = fmax(b0, sMax_[k]); }
-        partial[2 * blockIdx.x] = a0;
-        partial[2 * blockIdx.x + 1] = b0;
+        partial[2 * slot] = a0;
+        partial[2 * slot + 1] = b0;
     }
 }
 
@@ -1061,7 +1216,7 @@ __global__ __launch_bounds__(BLK) void k_update(DevMesh m, PhysParams p, StageAr
     }
     if ((KIND == SK_EULER || KIND == SK_RK2_2 || KIND == SK_TVD_2 || KIND == SK_RK4_4) && a.rmsBlocks)
         rmsBlockLeaf(i < m.N, qNew, qOld, i < m.N ? m.refId[i] : 0u, a.rmsBlocks);
-    blockPartial(dtCand, rCand, a.partial);
+    blockPartial(dtCand, rCand, a.partial, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- fused stage
@@ -1167,9 +1322,9 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
                 }
                 double g[4][2], lx[4], ly[4];
                 const double* gmi = gmS + k;
-                if (!lim) gradLimit<S, 1>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 0, p.lpThresh, g, lx, ly);
-                else if (LP) gradLimit<S, 3>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 1, p.lpThresh, g, lx, ly);
-                else gradLimit<S, 2>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 1, p.lpThresh, g, lx, ly);
+                if (!lim) gradLimit<S, 1, double>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 0, p.lpThresh, g, lx, ly);
+                else if (LP) gradLimit<S, 3, double>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 1, p.lpThresh, g, lx, ly);
+                else gradLimit<S, 2, double>(qs + 4 * k, qn, gmi, nG, cs[k], cn, real, 1, p.lpThresh, g, lx, ly);
 #pragma unroll
                 for (int c = 0; c < 4; c++) { gs[8 * k + 2 * c] = g[c][0]; gs[8 * k + 2 * c + 1] = g[c][1]; }
                 if (LP) {
@@ -1279,7 +1434,7 @@ __global__ __launch_bounds__(TBLK, 2) void k_stage_tile(DevMesh m, PhysParams p,
         d1 = n1;
         par ^= 1u;
     }
-    blockPartial(dtCand, rCand, a.partial);
+    blockPartial(dtCand, rCand, a.partial, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- faces + update
@@ -1305,11 +1460,15 @@ DEV void edgeGeometry(bool interior, double2 p0, double2 p1, double2& n, double2
     mid = make_double2(0.5 * (p0.x + p1.x), 0.5 * (p0.y + p1.y));
 }
 
-template <int S, int FLUX, int VISC, int ORDER, int LP>
-__global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
+// One workgroup of the face/update kernel on tile `tile` of its mesh, leaving its (min dt,
+// max |R|) pair in partial slot `pslot` (the solo and the batched kernels run this same code).
+template <int S, int FLUX, int VISC, int ORDER, int LP, int MIX>
+DEV void faceTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t,
+                      uint32_t tile, uint32_t pslot, double* fsd)
 {
-    extern __shared__ double fsd[];
-    const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x), tid = threadIdx.x;
+    using R = typename std::conditional<MIX == 1, float, double>::type;
+    const FaceSrc<MIX> src{a};
+    const uint32_t tid = threadIdx.x;
     // The first two cell rounds' ids are loaded ahead of the faces; the next round's
     // face record is loaded one round ahead, so each round waits for one memory latency
     // (cell data) instead of two (record, then cell data).  The tile's arrays start at
@@ -1332,28 +1491,33 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
         // limiter row: the owner's row is read in its place and not used.
         const double2 p0 = m.node[r.z], p1 = m.node[r.w];
         const uint32_t rb = interior ? r.y : r.x;
-        double qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
-        ld4(a.qin, r.x, qa);
-        ld4(a.qin, r.y, qb);
-        if (ORDER > 1 || VISC) { loadGrad(a.grad, r.x, ga); loadGrad(a.grad, rb, gb); }
+        R qa[4], qb[4], ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
+        src.q(r.x, qa);
+        src.q(r.y, qb);
+        if (ORDER > 1 || VISC) { src.grad(r.x, ga); src.grad(rb, gb); }
         double2 ca = make_double2(0, 0), cb = make_double2(0, 0);
         if (ORDER > 1) {
-            loadLim<LP>(a.lim, r.x, lxa, lya);
-            loadLim<LP>(a.lim, rb, lxb, lyb);
+            src.template lim<LP>(r.x, lxa, lya);
+            src.template lim<LP>(rb, lxb, lyb);
             ca = m.cen[r.x]; cb = m.cen[r.y];
         }
         double2 n, mid;
         double len;
         edgeGeometry(interior, p0, p1, n, mid, len);
-        double qL[4], qR[4], F[4], sMax;
+        R qL[4], qR[4], Fr[4], sMr;
         bool isB = false;
         uint32_t b = 0;
         if (interior) {
-            interiorFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, n, mid, p, qL, qR, F, sMax);
+            interiorFace<FLUX, VISC, ORDER, LP, R>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, n, mid, p, qL, qR, Fr,
+                                                  sMr);
         } else {
             isB = true;
             b = r.y - m.N;   // the ghost of boundary face b is cell N + b
-            boundaryFace<FLUX, VISC, ORDER, LP>(qa, ga, lxa, lya, ca, qb, m.bface[b], n, mid, p, qL, qR, F, sMax);
+            boundaryFace<FLUX, VISC, ORDER, LP, R>(qa, ga, lxa, lya, ca, qb, m.bface[b], n, mid, p, qL, qR, Fr, sMr);
+        }
+        // the residual is accumulated in FP64 from len * flux on either path
+        const double F[4] = {double(Fr[0]), double(Fr[1]), double(Fr[2]), double(Fr[3])}, sMax = double(sMr);
+        if (isB) {
             // boundary faces (each in exactly one tile) feed the force integral
             const uint32_t f = m.Fi + b;
             *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) =
@@ -1439,7 +1603,14 @@ __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, Stag
         }
         if (a.rmsBlocks) rmsBlockLeaf(have, qNew, qOld, have ? refC : 0u, a.rmsBlocks);
     }
-    blockPartial(dtCand, rCand, a.partial);
+    blockPartial(dtCand, rCand, a.partial, pslot);
+}
+
+template <int S, int FLUX, int VISC, int ORDER, int LP, int MIX>
+__global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
+{
+    extern __shared__ double fsd[];
+    faceTileBody<S, FLUX, VISC, ORDER, LP, MIX>(m, p, a, t, xcdBlock(blockIdx.x, gridDim.x), blockIdx.x, fsd);
 }
 
 template <int S, int FLUX, int VISC>
@@ -1447,11 +1618,18 @@ static void faceTileO(const DevMesh& m, const PhysParams& p, const StageArgs& a,
 {
     const bool lp = p.order > 1 && p.limited && p.lpThresh > 0;
     const dim3 g(t.nTiles), b(BLK);
-    if (p.order == 1) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 1, 0>), g, b, t.ldsBytes, st, m, p, a, t);
+    if (p.mixed) {   // no LP vector limiter on the mixed path (rejected at set-up)
+        if (p.order == 1) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 1, 0, 1>), g, b, t.ldsBytes, st, m, p, a, t);
+        else if (!(p.limited && a.limit))
+            hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 2, 1>), g, b, t.ldsBytes, st, m, p, a, t);
+        else hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 0, 1>), g, b, t.ldsBytes, st, m, p, a, t);
+        return;
+    }
+    if (p.order == 1) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 1, 0, 0>), g, b, t.ldsBytes, st, m, p, a, t);
     else if (!(p.limited && a.limit))
-        hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 2>), g, b, t.ldsBytes, st, m, p, a, t);
-    else if (lp) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 1>), g, b, t.ldsBytes, st, m, p, a, t);
-    else hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 0>), g, b, t.ldsBytes, st, m, p, a, t);
+        hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 2, 0>), g, b, t.ldsBytes, st, m, p, a, t);
+    else if (lp) hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 1, 0>), g, b, t.ldsBytes, st, m, p, a, t);
+    else hipLaunchKernelGGL((k_face_tile<S, FLUX, VISC, 2, 0, 0>), g, b, t.ldsBytes, st, m, p, a, t);
 }
 
 template <int S>
@@ -1688,14 +1866,13 @@ __global__ __launch_bounds__(RBLK) void k_rms_chunks(DiagArgs d)
 // pairwise tree does (the two orders of each sum are equal), the 4 waves levels
 // 9 and 10.  Workgroup nfull: the partial last chunk; nfull+1: the boundary force.
 constexpr int RFB = 256;
-__global__ __launch_bounds__(RFB) void k_rms_blocks(DiagArgs d, uint32_t nfull)
+DEV void rmsBlocksBody(const DiagArgs& d, uint32_t nfull, uint32_t wg, double (*v)[RMS_CHUNK_BLOCKS])
 {
     if (!d.sc->active) return;
-    __shared__ double v[8][RMS_CHUNK_BLOCKS];
     const uint64_t nb = uint64_t(d.L) / 16;
-    if (blockIdx.x == nfull + 1) { forceSum<RFB>(d, &v[0][0]); return; }
-    if (blockIdx.x == nfull + 2) { if (d.nStages) stageReduce<RFB>(d, &v[0][0]); return; }
-    const uint64_t c0 = uint64_t(blockIdx.x) * RMS_CHUNK_BLOCKS;
+    if (wg == nfull + 1) { forceSum<RFB>(d, &v[0][0]); return; }
+    if (wg == nfull + 2) { if (d.nStages) stageReduce<RFB>(d, &v[0][0]); return; }
+    const uint64_t c0 = uint64_t(wg) * RMS_CHUNK_BLOCKS;
     auto leaf = [&](uint64_t b, double o[8]) {
         if (b * 16 < d.N) {
             const double4 x = *reinterpret_cast<const double4*>(d.blockSums + b * 8);
@@ -1706,7 +1883,7 @@ __global__ __launch_bounds__(RFB) void k_rms_blocks(DiagArgs d, uint32_t nfull)
             for (int q = 0; q < 8; q++) o[q] = 0.0;
         }
     };
-    if (blockIdx.x == nfull) {   // partial chunk
+    if (wg == nfull) {   // partial chunk
         const uint32_t cnt = uint32_t(nb - c0);
         for (uint32_t lb = threadIdx.x; lb < cnt; lb += RFB) {
             double o[8];
@@ -1734,8 +1911,14 @@ __global__ __launch_bounds__(RFB) void k_rms_blocks(DiagArgs d, uint32_t nfull)
     __syncthreads();
     if (threadIdx.x < 8) {
         const int q = threadIdx.x;
-        d.chunkTrees[size_t(blockIdx.x) * 128 + q * 16 + 10] = (v[q][0] + v[q][1]) + (v[q][2] + v[q][3]);
+        d.chunkTrees[size_t(wg) * 128 + q * 16 + 10] = (v[q][0] + v[q][1]) + (v[q][2] + v[q][3]);
     }
+}
+
+__global__ __launch_bounds__(RFB) void k_rms_blocks(DiagArgs d, uint32_t nfull)
+{
+    __shared__ double v[8][RMS_CHUNK_BLOCKS];
+    rmsBlocksBody(d, nfull, blockIdx.x, v);
 }
 
 // Step epilogue (app/solver.d:122-247), one workgroup.  The chunk trees are
@@ -1745,13 +1928,20 @@ __global__ __launch_bounds__(RFB) void k_rms_blocks(DiagArgs d, uint32_t nfull)
 constexpr int FIN_T = 256;
 constexpr int FIN_CHUNKS = 1024;     // chunk trees merged in LDS per quantity
 
-__global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
+struct FinLds {
+    double v[8][FIN_CHUNKS];
+    double trees[8][48];
+    int ntree[8];
+    double tot[8];
+};
+
+DEV void finalizeBody(const DiagArgs& d, FinLds& L)
 {
     DevScalars* sc = d.sc;
-    __shared__ double v[8][FIN_CHUNKS];
-    __shared__ double trees[8][48];
-    __shared__ int ntree[8];
-    __shared__ double tot[8];
+    auto& v = L.v;
+    auto& trees = L.trees;
+    auto& ntree = L.ntree;
+    auto& tot = L.tot;
     const int tid = threadIdx.x;
     const uint64_t nb = uint64_t(d.L) / 16;
     const uint64_t nfull = nb / RMS_CHUNK_BLOCKS, part = nb % RMS_CHUNK_BLOCKS;
@@ -1881,7 +2071,8 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
         else if (inc >= 1) inc--;
         if (inc >= 2000) { CFL *= 0.99; inc = 0; }
     }
-    const int64_t slot = s0.steps - d.histBase;
+    // the history row: by absolute step within the host's batch, or a ring (batched runs)
+    const int64_t slot = d.histRing ? s0.steps % d.histCap : s0.steps - d.histBase;
     if (slot >= 0 && slot < d.histCap) {
         double* h = d.hist + slot * HIST_FIELDS;
         h[0] = s0.t; h[1] = dt; h[2] = rms[0]; h[3] = rms[1]; h[4] = rms[2]; h[5] = rms[3];
@@ -1905,9 +2096,35 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
     }
     sc->t = t1;
     sc->steps = steps1;
+    // convergence stop (extensions.rmsDrop): the rho RMS residual of app/solver.d:153-168 has
+    // dropped by the given factor below its value on the first step
+    const double rms0 = s0.steps == 0 ? rms[0] : s0.rms0;
+    sc->rms0 = rms0;
+    const bool converged = d.rmsDrop > 0 && rms[0] <= d.rmsDrop * rms0;
+    if (converged) sc->converged = 1;
     if (!(t1 < d.tEnd) || (d.maxSteps >= 0 && steps1 >= d.maxSteps) || s0.errEdge != ~0ull || s0.errCell != ~0ull ||
-        s0.errBC != ~0ull)
+        s0.errBC != ~0ull || converged)
         sc->active = 0;
+}
+
+__global__ __launch_bounds__(FIN_T) void k_finalize(DiagArgs d)
+{
+    __shared__ FinLds L;
+    finalizeBody(d, L);
+}
+
+// FP32 shadow of a state array (mixed path): after set-up, an upload or a load
+__global__ __launch_bounds__(BLK) void k_shadow(const double4* __restrict__ q, float4* __restrict__ q32, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const double4 v = q[i];
+    q32[i] = make_float4(float(v.x), float(v.y), float(v.z), float(v.w));
+}
+
+void launchShadow(const double4* q, float4* q32, uint32_t n, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_shadow, dim3((n + BLK - 1) / BLK), dim3(BLK), 0, st, q, q32, n);
 }
 
 __global__ void k_init_scalars(DevScalars* sc, double dt, double t, double CFL)
@@ -1924,7 +2141,8 @@ __global__ void k_init_scalars(DevScalars* sc, double dt, double t, double CFL)
     sc->incIters = 0;
     sc->steps = 0;
     sc->active = 1;
-    sc->pad = 0;
+    sc->converged = 0;
+    sc->rms0 = 0;
     for (int v = 0; v < 4; v++) sc->rms[v] = 0;
     sc->force[0] = sc->force[1] = 0;
     sc->fsum[0] = sc->fsum[1] = 0;
@@ -1942,11 +2160,19 @@ static void gradS(const DevMesh& m, const PhysParams& p, const StageArgs& a, hip
     const uint32_t g = nblk(m.N) + (a.force.n ? 1u : 0u);   // + the force workgroup
     if (g == 0) return;
     const int mode = p.order == 1 ? 0 : (!(p.limited && a.limit) ? 1 : (p.lpThresh > 0 ? 3 : 2));
+    if (p.mixed) {
+        switch (mode) {
+        case 0: hipLaunchKernelGGL((k_gradient<S, 0, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+        case 1: hipLaunchKernelGGL((k_gradient<S, 1, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+        default: hipLaunchKernelGGL((k_gradient<S, 2, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+        }
+        return;
+    }
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_gradient<S, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
-    case 1: hipLaunchKernelGGL((k_gradient<S, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
-    case 2: hipLaunchKernelGGL((k_gradient<S, 2>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
-    default: hipLaunchKernelGGL((k_gradient<S, 3>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    case 0: hipLaunchKernelGGL((k_gradient<S, 0, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    case 1: hipLaunchKernelGGL((k_gradient<S, 1, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    case 2: hipLaunchKernelGGL((k_gradient<S, 2, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    default: hipLaunchKernelGGL((k_gradient<S, 3, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     }
 }
 
@@ -1957,25 +2183,31 @@ void launchGradient(const DevMesh& m, const PhysParams& p, const StageArgs& a, i
     else gradS<6>(m, p, a, st);
 }
 
-template <int FLUX, int VISC, int ORDER, int LP>
+template <int FLUX, int VISC, int ORDER, int LP, int MIX>
 static void facesT(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
 {
     const uint32_t nbi = nblk(m.Fi), nbb = nblk(m.Fb);
     if (nbi + nbb == 0) return;
     if (a.estate)
-        hipLaunchKernelGGL((k_faces<FLUX, VISC, ORDER, LP, 1>), dim3(nbi + nbb), dim3(BLK), 0, st, m, p, a, nbi);
+        hipLaunchKernelGGL((k_faces<FLUX, VISC, ORDER, LP, 1, MIX>), dim3(nbi + nbb), dim3(BLK), 0, st, m, p, a, nbi);
     else
-        hipLaunchKernelGGL((k_faces<FLUX, VISC, ORDER, LP, 0>), dim3(nbi + nbb), dim3(BLK), 0, st, m, p, a, nbi);
+        hipLaunchKernelGGL((k_faces<FLUX, VISC, ORDER, LP, 0, MIX>), dim3(nbi + nbb), dim3(BLK), 0, st, m, p, a, nbi);
 }
 
 template <int FLUX, int VISC>
 static void facesO(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
 {
     const bool lp = p.order > 1 && p.limited && p.lpThresh > 0;
-    if (p.order == 1) facesT<FLUX, VISC, 1, 0>(m, p, a, st);
-    else if (!(p.limited && a.limit)) facesT<FLUX, VISC, 2, 2>(m, p, a, st);
-    else if (lp) facesT<FLUX, VISC, 2, 1>(m, p, a, st);
-    else facesT<FLUX, VISC, 2, 0>(m, p, a, st);
+    if (p.mixed) {
+        if (p.order == 1) facesT<FLUX, VISC, 1, 0, 1>(m, p, a, st);
+        else if (!(p.limited && a.limit)) facesT<FLUX, VISC, 2, 2, 1>(m, p, a, st);
+        else facesT<FLUX, VISC, 2, 0, 1>(m, p, a, st);
+        return;
+    }
+    if (p.order == 1) facesT<FLUX, VISC, 1, 0, 0>(m, p, a, st);
+    else if (!(p.limited && a.limit)) facesT<FLUX, VISC, 2, 2, 0>(m, p, a, st);
+    else if (lp) facesT<FLUX, VISC, 2, 1, 0>(m, p, a, st);
+    else facesT<FLUX, VISC, 2, 0, 0>(m, p, a, st);
 }
 
 void launchFaces(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t st)
@@ -2042,6 +2274,163 @@ void launchFinalize(const DiagArgs& d, hipStream_t st)
 void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStream_t st)
 {
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, st, sc, dt, t, CFL);
+}
+
+// ---------------------------------------------------------------- batched launches
+// One launch per kernel for many meshes.  Workgroup w takes item xcdBlock(w), so each
+// XCD gets a contiguous run of items (items are listed mesh by mesh, cell blocks and tiles
+// in order); the item names the mesh record and the block, tile or RMS workgroup, and the
+// workgroup runs the solo kernel's code on it.
+template <int S, int MODE, int MIX>
+__global__ __launch_bounds__(BLK) void k_gradient_batch(const MemberArgs* __restrict__ mem, const uint2* __restrict__ items,
+                                                        uint32_t parity, uint32_t stage)
+{
+    __shared__ GradLds<MODE> L;
+    const uint2 it = items[xcdBlock(blockIdx.x, gridDim.x)];
+    const MemberArgs& M = mem[it.x];
+    const StageArgs& a = M.st[parity ^ M.parityFlip][stage];
+    if (it.y == FORCE_ITEM) {
+        pendingForce<BLK, FORCE_LDS>(a.force, a.sc, reinterpret_cast<double2*>(&L.sg[0][0]));
+        return;
+    }
+    gradientBody<S, MODE, MIX>(M.m, M.p, a, it.y, L);
+}
+
+template <int S, int FLUX, int VISC, int ORDER, int LP, int MIX>
+__global__ __launch_bounds__(BLK) void k_face_tile_batch(const MemberArgs* __restrict__ mem,
+                                                         const uint2* __restrict__ items, uint32_t parity, uint32_t stage)
+{
+    extern __shared__ double fsd[];
+    const uint2 it = items[xcdBlock(blockIdx.x, gridDim.x)];
+    const MemberArgs& M = mem[it.x];
+    faceTileBody<S, FLUX, VISC, ORDER, LP, MIX>(M.m, M.p, M.st[parity ^ M.parityFlip][stage], M.t, it.y, it.y, fsd);
+}
+
+__global__ __launch_bounds__(RFB) void k_rms_batch(const MemberArgs* __restrict__ mem, const uint2* __restrict__ items,
+                                                   uint32_t parity)
+{
+    __shared__ double v[8][RMS_CHUNK_BLOCKS];
+    const uint2 it = items[blockIdx.x];
+    const MemberArgs& M = mem[it.x];
+    rmsBlocksBody(M.d[parity ^ M.parityFlip], M.nRmsFull, it.y, v);
+}
+
+__global__ __launch_bounds__(FIN_T) void k_finalize_batch(const MemberArgs* __restrict__ mem,
+                                                          const uint32_t* __restrict__ members, uint32_t parity)
+{
+    __shared__ FinLds L;
+    const MemberArgs& M = mem[members[blockIdx.x]];
+    finalizeBody(M.d[parity ^ M.parityFlip], L);
+}
+
+__global__ __launch_bounds__(BLK) void k_pending_force_batch(const MemberArgs* __restrict__ mem,
+                                                             const uint32_t* __restrict__ members)
+{
+    __shared__ double2 fsh[FORCE_LDS];
+    const MemberArgs& M = mem[members[blockIdx.x]];
+    pendingForce<BLK, FORCE_LDS>(M.f, M.d[0].sc, fsh);
+}
+
+__global__ void k_gather_scalars(const MemberArgs* __restrict__ mem, const uint32_t* __restrict__ members, uint32_t n,
+                                 DevScalars* __restrict__ out)
+{
+    // one thread per 8-byte word of a DevScalars record
+    constexpr uint32_t W = sizeof(DevScalars) / 8;
+    static_assert(sizeof(DevScalars) % 8 == 0, "DevScalars is copied in 8-byte words");
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * W) return;
+    const uint32_t k = g / W, w = g % W;
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(mem[members[k]].d[0].sc);
+    reinterpret_cast<uint64_t*>(out + k)[w] = src[w];
+}
+
+template <int S, int MIX>
+static void gradBatchS(int mode, const MemberArgs* mem, const uint2* items, uint32_t n, uint32_t par, uint32_t st,
+                       hipStream_t s)
+{
+    const dim3 g(n), b(BLK);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_gradient_batch<S, 0, MIX>), g, b, 0, s, mem, items, par, st); break;
+    case 1: hipLaunchKernelGGL((k_gradient_batch<S, 1, MIX>), g, b, 0, s, mem, items, par, st); break;
+    default: hipLaunchKernelGGL((k_gradient_batch<S, 2, MIX>), g, b, 0, s, mem, items, par, st); break;
+    }
+}
+
+void launchGradientBatch(const GroupShape& gs, int limitStage, const MemberArgs* mem, const uint2* items,
+                         uint32_t nItems, uint32_t parity, uint32_t stage, hipStream_t st)
+{
+    if (nItems == 0) return;
+    // batched groups run without the LP vector limiter (mode 3)
+    const int mode = gs.order == 1 ? 0 : (!(gs.limited && limitStage) ? 1 : 2);
+    if (gs.mixed) {
+        if (gs.S == 3) gradBatchS<3, 1>(mode, mem, items, nItems, parity, stage, st);
+        else if (gs.S == 4) gradBatchS<4, 1>(mode, mem, items, nItems, parity, stage, st);
+        else gradBatchS<6, 1>(mode, mem, items, nItems, parity, stage, st);
+    } else {
+        if (gs.S == 3) gradBatchS<3, 0>(mode, mem, items, nItems, parity, stage, st);
+        else if (gs.S == 4) gradBatchS<4, 0>(mode, mem, items, nItems, parity, stage, st);
+        else gradBatchS<6, 0>(mode, mem, items, nItems, parity, stage, st);
+    }
+}
+
+template <int S, int FLUX, int VISC, int MIX>
+static void faceBatchO(const GroupShape& gs, int limitStage, const MemberArgs* mem, const uint2* items, uint32_t n,
+                       uint32_t lds, uint32_t par, uint32_t stg, hipStream_t s)
+{
+    const dim3 g(n), b(BLK);
+    if (gs.order == 1) hipLaunchKernelGGL((k_face_tile_batch<S, FLUX, VISC, 1, 0, MIX>), g, b, lds, s, mem, items, par, stg);
+    else if (!(gs.limited && limitStage))
+        hipLaunchKernelGGL((k_face_tile_batch<S, FLUX, VISC, 2, 2, MIX>), g, b, lds, s, mem, items, par, stg);
+    else hipLaunchKernelGGL((k_face_tile_batch<S, FLUX, VISC, 2, 0, MIX>), g, b, lds, s, mem, items, par, stg);
+}
+
+template <int S, int MIX>
+static void faceBatchS(const GroupShape& gs, int limitStage, const MemberArgs* mem, const uint2* items, uint32_t n,
+                       uint32_t lds, uint32_t par, uint32_t stg, hipStream_t s)
+{
+    if (gs.flux == 1) {
+        if (gs.viscous) faceBatchO<S, 1, 1, MIX>(gs, limitStage, mem, items, n, lds, par, stg, s);
+        else faceBatchO<S, 1, 0, MIX>(gs, limitStage, mem, items, n, lds, par, stg, s);
+    } else {
+        if (gs.viscous) faceBatchO<S, 0, 1, MIX>(gs, limitStage, mem, items, n, lds, par, stg, s);
+        else faceBatchO<S, 0, 0, MIX>(gs, limitStage, mem, items, n, lds, par, stg, s);
+    }
+}
+
+void launchFaceTilesBatch(const GroupShape& gs, int limitStage, const MemberArgs* mem, const uint2* items,
+                          uint32_t nItems, uint32_t ldsBytes, uint32_t parity, uint32_t stage, hipStream_t st)
+{
+    if (nItems == 0) return;
+    if (gs.mixed) {
+        if (gs.S == 3) faceBatchS<3, 1>(gs, limitStage, mem, items, nItems, ldsBytes, parity, stage, st);
+        else if (gs.S == 4) faceBatchS<4, 1>(gs, limitStage, mem, items, nItems, ldsBytes, parity, stage, st);
+        else faceBatchS<6, 1>(gs, limitStage, mem, items, nItems, ldsBytes, parity, stage, st);
+    } else {
+        if (gs.S == 3) faceBatchS<3, 0>(gs, limitStage, mem, items, nItems, ldsBytes, parity, stage, st);
+        else if (gs.S == 4) faceBatchS<4, 0>(gs, limitStage, mem, items, nItems, ldsBytes, parity, stage, st);
+        else faceBatchS<6, 0>(gs, limitStage, mem, items, nItems, ldsBytes, parity, stage, st);
+    }
+}
+
+void launchRmsBatch(const MemberArgs* mem, const uint2* items, uint32_t nItems, uint32_t parity, hipStream_t st)
+{
+    if (nItems) hipLaunchKernelGGL(k_rms_batch, dim3(nItems), dim3(RFB), 0, st, mem, items, parity);
+}
+
+void launchFinalizeBatch(const MemberArgs* mem, const uint32_t* members, uint32_t n, uint32_t parity, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_finalize_batch, dim3(n), dim3(FIN_T), 0, st, mem, members, parity);
+}
+
+void launchPendingForceBatch(const MemberArgs* mem, const uint32_t* members, uint32_t n, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_pending_force_batch, dim3(n), dim3(BLK), 0, st, mem, members);
+}
+
+void launchGatherScalars(const MemberArgs* mem, const uint32_t* members, uint32_t n, DevScalars* out, hipStream_t st)
+{
+    const uint32_t words = n * uint32_t(sizeof(DevScalars) / 8);
+    if (words) hipLaunchKernelGGL(k_gather_scalars, dim3((words + 255) / 256), dim3(256), 0, st, mem, members, n, out);
 }
 
 }  // namespace ebb

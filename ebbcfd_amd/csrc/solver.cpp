@@ -17,60 +17,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ebbcfd.h"
-#include "dev.hpp"
-#include "host.hpp"
-#include "tiles.hpp"
+#include "solver_impl.hpp"
 
-using namespace ebb;
-
-struct ebb_mesh {
-    RawMesh raw;
-    BuiltMesh b;
-};
-
-struct ebb_config {
-    Config c;
-};
+namespace ebb {
+thread_local std::string lastDeviceError;
+}  // namespace ebb
 
 namespace {
-
-#define HIPCHK(x)                                                     \
-    do {                                                              \
-        hipError_t e_ = (x);                                          \
-        if (e_ != hipSuccess) {                                       \
-            lastDeviceError = std::string(#x) + ": " + hipGetErrorString(e_); \
-            return EBB_ERR_DEVICE;                                    \
-        }                                                             \
-    } while (0)
-
-thread_local std::string lastDeviceError;
-
-template <typename T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t alloc(size_t count)
-    {
-        n = count;
-        if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e == hipSuccess) e = hipMemset(p, 0, count * sizeof(T));
-        return e;
-    }
-    hipError_t upload(const std::vector<T>& v)
-    {
-        hipError_t e = alloc(v.size());
-        if (e == hipSuccess && !v.empty()) e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-        return e;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
 
 void putBE32(std::vector<uint8_t>& b, uint32_t v)
 {
@@ -117,78 +70,7 @@ uint32_t crc32(const uint8_t* p, size_t n)
 }
 }  // namespace ebb
 
-struct ebb_solver {
-    // host copies
-    RawMesh raw;
-    BuiltMesh b;
-    Config cfg;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t N = 0, G = 0, Fi = 0, Fb = 0, S = 3;
-    std::vector<uint32_t> faceOfEdge;   // reference edge -> device face
-    std::vector<uint32_t> edgeOfFace;   // device face -> reference edge
-    std::vector<BFace> bfaces;
-    PhysParams phys{};
-    DevMesh dm{};
-    // device buffers
-    DBuf<uint32_t> d_nbr, d_cface, d_refId, d_forceFaces;
-    DBuf<double> d_nia, d_cd, d_cper, d_flen, d_grad, d_lim, d_lenF, d_lenS, d_celldt, d_hist, d_chunkTrees;
-    DBuf<double> d_partial;
-    DBuf<double2> d_cen, d_fn, d_fmid;
-    DBuf<uint2> d_fcell;
-    DBuf<BFace> d_bface;
-    DBuf<double4> d_src, d_qA, d_qB, d_T, d_T2, d_acc, d_R, d_estate;
-    DBuf<uint32_t> d_chunkMask;
-    // fused-stage tiles
-    DBuf<uint4> d_tDesc, d_tBlob;
-    TileArgs tiles{};
-    bool useTiles = false;
-    uint32_t tileCells = 0;
-    DBuf<uint4> d_ftDesc, d_ftRef;
-    DBuf<uint32_t> d_ftIds, d_ftFid;
-    DBuf<uint16_t> d_ftCf;
-    DBuf<double2> d_node;
-    FaceTileArgs ftiles{};
-    bool useFaceTiles = false;
-    // device cell numbering of the tiled paths: device cell d is reference cell
-    // refOf[d] (empty: identical numbering); d_devOf maps back for the RMS sums
-    std::vector<uint32_t> refOf, devOf;
-    DBuf<uint32_t> d_devOf;
-    // RMS leaf sums computed by the stage that completes a step (split path, and
-    // face tiles cut along aligned 16-cell blocks)
-    bool rmsFused = false;
-    DBuf<double> d_rmsBlocks;
-    DevScalars* d_sc = nullptr;
-    DevScalars* h_sc = nullptr;  // pinned
-    double* h_hist = nullptr;    // pinned
-    int64_t histCap = 256;
-    // time-loop bookkeeping
-    int cur = 0;                 // 0: qA holds the current state, 1: qB
-    double4* lastInput = nullptr;
-    int lastLimit = 1;
-    bool haveStage = false;
-    uint32_t stageCounter = 0;
-    // the step of each stage launched since the error words were last cleared (stage
-    // stageBase + k ran in step stageStep[k]): edge and boundary errors carry their stage
-    uint32_t stageBase = 0;
-    std::vector<int64_t> stageStep;
-    // stage partials: pairs per stage (the stage kernel's workgroups), region stride,
-    // and the stage of the current time-loop step
-    uint32_t nPart = 0, pstride = 0, stageInStep = 0;
-    int64_t stepsTotal = 0;
-    double t0 = 0, dt0 = 0;
-    std::vector<double> history;     // HIST_FIELDS per completed step of ebb_run
-    // errors
-    int errKind = 0;
-    int64_t errIndex = -1, errStep = -1;
-    std::string errMsg;
-
-    double4* qcur() { return cur == 0 ? d_qA.p : d_qB.p; }
-    double4* qnext() { return cur == 0 ? d_qB.p : d_qA.p; }
-    int64_t deviceBytes = 0;
-};
-
-namespace {
+namespace ebb {
 
 int fail(ebb_solver* s, int code, const std::string& msg)
 {
@@ -232,27 +114,76 @@ ForceArgs forceArgs(const ebb_solver* s)
     return f;
 }
 
+// The FP32 shadow paired with a state buffer (mixed precision), null otherwise.
+float4* shadowOf(ebb_solver* s, const double4* q)
+{
+    if (!s->phys.mixed || !q) return nullptr;
+    if (q == s->d_qA.p) return s->d_qA32.p;
+    if (q == s->d_qB.p) return s->d_qB32.p;
+    if (q == s->d_T.p) return s->d_T32.p;
+    if (q == s->d_T2.p) return s->d_T232.p;
+    return nullptr;
+}
+
+// the mixed-precision pointers of a stage reading qin and writing qout
+void mixArgs(ebb_solver* s, StageArgs& a, const double4* qin, const double4* qout)
+{
+    a.qin32 = shadowOf(s, qin);
+    a.qout32 = shadowOf(s, qout);
+    a.grad32 = s->phys.mixed ? s->d_grad32.p : nullptr;
+    a.lim32 = s->phys.mixed ? s->d_lim32.p : nullptr;
+}
+
+// refresh the FP32 shadow of the current state (after an upload, a load or set-up)
+void refreshShadow(ebb_solver* s)
+{
+    if (s->phys.mixed) launchShadow(s->qcur(), shadowOf(s, s->qcur()), uint32_t(s->b.nCells), s->stream);
+}
+
+// the arguments every stage shares
+static StageArgs baseStageArgs(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qout, int limit,
+                               int dtUpdate, int first, int last, uint32_t stageId, uint32_t stepId)
+{
+    StageArgs a{};
+    a.qin = qin; a.qn = qn; a.qout = qout; a.acc = s->d_acc.p;
+    a.grad = s->d_grad.p; a.lim = s->d_lim.p; a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p;
+    a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
+    a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
+    a.recordErrors = 1; a.stageId = stageId; a.stepId = stepId;
+    mixArgs(s, a, qin, qout);
+    return a;
+}
+
+StageArgs loopStageArgs(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qout, int limit,
+                        int dtUpdate, int first, int last, uint32_t stageInStep, uint32_t stageId, uint32_t stepId)
+{
+    StageArgs a = baseStageArgs(s, kind, qin, qn, qout, limit, dtUpdate, first, last, stageId, stepId);
+    a.checkActive = 1;
+    a.rmsBlocks = (last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
+    // time loop: each stage leaves its partials in its own region; the step's RMS
+    // kernel reduces them (nothing in the step reads newDt or Rmax before that)
+    a.deferReduce = 1;
+    a.partial = s->d_partial.p + size_t(stageInStep) * s->pstride;
+    // the gradient kernels of the time loop also sum the previous step's force
+    if (deferForce(s)) a.force = forceArgs(s);
+    return a;
+}
+
 // one residual evaluation + stage update
 void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qout, int limit, int dtUpdate,
            int first, int last, int checkActive, double4* R, double4* estate, uint32_t stepId)
 {
-    StageArgs a{};
-    a.qin = qin; a.qn = qn; a.qout = qout; a.acc = s->d_acc.p; a.R = R;
-    a.grad = s->d_grad.p; a.lim = s->d_lim.p; a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p;
-    a.estate = estate; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
-    a.rmsBlocks = (last && checkActive && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
-    a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
-    a.checkActive = checkActive; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = stepId;
-    s->stageStep.push_back(stepId);
+    const uint32_t stageId = s->stageCounter++;
+    StageArgs a;
     if (checkActive) {
-        // time loop: each stage leaves its partials in its own region; the step's RMS
-        // kernel reduces them (nothing in the step reads newDt or Rmax before that)
-        a.deferReduce = 1;
-        a.partial = s->d_partial.p + size_t(s->stageInStep) * s->pstride;
+        a = loopStageArgs(s, kind, qin, qn, qout, limit, dtUpdate, first, last, s->stageInStep, stageId, stepId);
         s->stageInStep++;
-        // the gradient kernels of the time loop also sum the previous step's force
-        if (deferForce(s)) a.force = forceArgs(s);
+    } else {
+        a = baseStageArgs(s, kind, qin, qn, qout, limit, dtUpdate, first, last, stageId, stepId);
     }
+    a.R = R;
+    a.estate = estate;
+    s->stageStep.push_back(stepId);
     if (s->useTiles && !estate && kind != SK_RESIDUAL) {
         launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
     } else if (s->useFaceTiles && !estate && kind != SK_RESIDUAL) {
@@ -268,32 +199,31 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
     s->haveStage = true;
 }
 
-// Integrator.step: all stages of one step from qcur() into qnext()
-void stepStages(ebb_solver* s, int checkActive, uint32_t stepId)
+// Integrator.step (source/integrators.d:22-270) from qcur() into qnext(): each stage's
+// kind, input and output buffers, limiter and local-dt flags
+std::vector<StageDesc> stepSequence(ebb_solver* s)
 {
     const Config& c = s->cfg;
     const int msl = c.multistageLimiting ? 1 : 0, lts = c.localTimestep ? 1 : 0;
     double4 *q = s->qcur(), *nx = s->qnext(), *T = s->d_T.p, *T2 = s->d_T2.p;
-    s->stageInStep = 0;
     switch (c.integratorKind()) {
-    case IntEuler:
-        stage(s, SK_EULER, q, q, nx, 1, 1, 1, 1, checkActive, nullptr, nullptr, stepId);
-        break;
-    case IntRK2:
-        stage(s, SK_RK2_1, q, q, T, 1, 1, 1, 0, checkActive, nullptr, nullptr, stepId);
-        stage(s, SK_RK2_2, T, q, nx, msl, 0, 0, 1, checkActive, nullptr, nullptr, stepId);
-        break;
-    case IntRK2TVD:
-        stage(s, SK_TVD_1, q, q, T, 1, lts, 1, 0, checkActive, nullptr, nullptr, stepId);
-        stage(s, SK_TVD_2, T, q, nx, 0, lts ? 0 : 1, 0, 1, checkActive, nullptr, nullptr, stepId);
-        break;
-    case IntRK4:
-        stage(s, SK_RK4_1, q, q, T, 1, 1, 1, 0, checkActive, nullptr, nullptr, stepId);
-        stage(s, SK_RK4_2, T, q, T2, msl, 0, 0, 0, checkActive, nullptr, nullptr, stepId);
-        stage(s, SK_RK4_3, T2, q, T, msl, 0, 0, 0, checkActive, nullptr, nullptr, stepId);
-        stage(s, SK_RK4_4, T, q, nx, msl, 0, 0, 1, checkActive, nullptr, nullptr, stepId);
-        break;
+    case IntEuler: return {{SK_EULER, q, nx, 1, 1, 1, 1}};
+    case IntRK2: return {{SK_RK2_1, q, T, 1, 1, 1, 0}, {SK_RK2_2, T, nx, msl, 0, 0, 1}};
+    case IntRK2TVD:   // stage 2 unlimited (integrators.d:190)
+        return {{SK_TVD_1, q, T, 1, lts, 1, 0}, {SK_TVD_2, T, nx, 0, lts ? 0 : 1, 0, 1}};
+    default:
+        return {{SK_RK4_1, q, T, 1, 1, 1, 0}, {SK_RK4_2, T, T2, msl, 0, 0, 0}, {SK_RK4_3, T2, T, msl, 0, 0, 0},
+                {SK_RK4_4, T, nx, msl, 0, 0, 1}};
     }
+}
+
+// Integrator.step: all stages of one step from qcur() into qnext()
+void stepStages(ebb_solver* s, int checkActive, uint32_t stepId)
+{
+    s->stageInStep = 0;
+    const double4* q = s->qcur();
+    for (const StageDesc& x : stepSequence(s))
+        stage(s, x.kind, x.in, q, x.out, x.limit, x.dtUpdate, x.first, x.last, checkActive, nullptr, nullptr, stepId);
 }
 
 int stagesPerStep(const Config& c)
@@ -320,6 +250,8 @@ DiagArgs diagArgs(ebb_solver* s, const double4* qnew, const double4* qold, int64
     d.maxSteps = maxSteps;
     d.partial = s->d_partial.p; d.nPart = s->nPart; d.pstride = s->pstride; d.nStages = s->stageInStep;
     d.deferForce = deferForce(s) ? 1 : 0;
+    d.histRing = 0;
+    d.rmsDrop = s->cfg.rmsDrop;
     d.sc = s->d_sc;
     return d;
 }
@@ -504,6 +436,72 @@ int buildTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& orde
     return EBB_OK;
 }
 
+// Cut the face tiles again with `tileCells` cells per tile (-1: automatic) and upload them
+// in place of the current ones; the stage partials grow with the tile count.  Only the
+// face-tile path in the reference numbering (the batched runs' layout) is re-tiled.
+int retileFaceTiles(ebb_solver* s, long long tileCells)
+{
+    if (!s->useFaceTiles || !s->refOf.empty()) return fail(s, EBB_ERR_STATE, "only face tiles in the reference numbering can be re-tiled");
+    auto drop = [&](auto& b) { s->deviceBytes -= int64_t((b.n ? b.n : 1) * sizeof(*b.p)); b.release(); };
+    drop(s->d_ftDesc); drop(s->d_ftRef); drop(s->d_ftIds); drop(s->d_ftCf); drop(s->d_ftFid);
+    s->cfg.tileCells = tileCells;
+    Layout ref, lay;
+    makeLayout(s->b, ref);
+    std::vector<uint32_t> start, ord;
+    int rc = planTiles(s, ref, lay, start, ord);
+    if (rc) return rc;
+    rc = buildTiles(s, lay, ord, start);
+    if (rc) return rc;
+    if (!s->useFaceTiles) return fail(s, EBB_ERR_MESH, "face tiles do not fit the workgroup's LDS");
+    s->nPart = s->ftiles.nTiles;
+    const uint32_t need = 2 * uint32_t(std::max<size_t>({(s->N + 255) / 256, s->ftiles.nTiles})) + 2;
+    if (need > s->pstride) {
+        s->deviceBytes -= int64_t(s->d_partial.n * sizeof(double));
+        s->d_partial.release();
+        s->pstride = need;
+        HIPCHK((hipError_t)trackAlloc(s, s->d_partial, size_t(s->pstride) * 4));
+    }
+    return EBB_OK;
+}
+
+// Device bytes buildDevice allocates for this mesh and config (the tile arrays estimated
+// at their usual fill), without allocating: the admission test of batched runs.
+int64_t estimateDeviceBytes(const BuiltMesh& b, const Config& c)
+{
+    const int64_t N = b.nElem, C = b.nCells, G = C - N, E = b.nEdges, F = E;
+    int maxE = 3;
+    for (uint32_t i = 0; i < b.nElem; i++) maxE = std::max<int>(maxE, b.nEdgesCell[i]);
+    const int64_t S = maxE <= 3 ? 3 : (maxE == 4 ? 4 : 6);
+    const int nst = stagesPerStep(c);
+    const int k = c.integratorKind();
+    const bool lp = c.order > 1 && c.limited && c.lpThresh > 0;
+    int64_t bytes = 0;
+    bytes += N * S * 4 * 2 + N * 4;                        // nbr, cface, refId
+    bytes += int64_t(b.nCells) * 0 + int64_t(b.nEdges) * 0;
+    bytes += 3 * N * 8 + C * 16;                           // nia, cd, cper; centroids
+    bytes += F * (8 + 16 + 16 + 8);                        // fcell, fn, fmid, flen
+    bytes += G * int64_t(sizeof(BFace));
+    if (c.mms) bytes += N * 32;
+    int64_t nodes = 0;
+    for (uint32_t e = 0; e < b.nEdges; e++) nodes = std::max<int64_t>(nodes, std::max(b.eNode0[e], b.eNode1[e]) + 1);
+    bytes += nodes * 16;
+    bytes += C * 32 * (2 + (nst > 1 ? 1 : 0) + (k == IntRK4 ? 1 : 0));   // qA, qB, T, T2
+    if (k == IntRK2 || k == IntRK4) bytes += N * 32;                       // acc
+    if (c.precision) bytes += C * 16 * (2 + (nst > 1 ? 1 : 0) + (k == IntRK4 ? 1 : 0)) + N * 48;
+    else bytes += N * 64 + N * (lp ? 64 : 32);
+    bytes += (F + 1) * 40;                                 // lenF, lenS
+    if (c.localTimestep) bytes += N * 8;
+    // face tiles: ~1.1 cells' and ~1.7 faces' worth of records per cell, plus the padding
+    bytes += int64_t(1.1 * double(N)) * (4 + 2 * S) + int64_t(1.7 * double(N)) * (16 + 4) + 2 * 256 * 20;
+    bytes += ((N + 15) / 16) * 64;                         // RMS leaf sums
+    const int64_t tiles = N / 700 + 1;
+    bytes += (2 * std::max<int64_t>((N + 255) / 256, tiles) + 2) * 4 * 8;   // stage partials
+    bytes += 256 * HIST_FIELDS * 8;
+    const int64_t nchunks = (C / 16 + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
+    bytes += nchunks * (128 * 8 + 32);
+    return bytes;
+}
+
 // Interior-cell arrays between the reference numbering and the device numbering.
 template <typename T>
 void toDevice(const ebb_solver* s, const T* ref, T* dev, size_t per = 1)
@@ -653,9 +651,18 @@ int buildDevice(ebb_solver* s)
     if (nst > 1) HIPCHK((hipError_t)trackAlloc(s, s->d_T, C));
     if (c.integratorKind() == IntRK4) HIPCHK((hipError_t)trackAlloc(s, s->d_T2, C));
     if (c.integratorKind() == IntRK2 || c.integratorKind() == IntRK4) HIPCHK((hipError_t)trackAlloc(s, s->d_acc, N));
-    HIPCHK((hipError_t)trackAlloc(s, s->d_grad, size_t(N) * 8));
     const bool lp = c.order > 1 && c.limited && c.lpThresh > 0;
-    HIPCHK((hipError_t)trackAlloc(s, s->d_lim, size_t(N) * (lp ? 8 : 4)));
+    if (p.mixed) {
+        HIPCHK((hipError_t)trackAlloc(s, s->d_qA32, C));
+        HIPCHK((hipError_t)trackAlloc(s, s->d_qB32, C));
+        if (nst > 1) HIPCHK((hipError_t)trackAlloc(s, s->d_T32, C));
+        if (c.integratorKind() == IntRK4) HIPCHK((hipError_t)trackAlloc(s, s->d_T232, C));
+        HIPCHK((hipError_t)trackAlloc(s, s->d_grad32, size_t(N) * 8));
+        HIPCHK((hipError_t)trackAlloc(s, s->d_lim32, size_t(N) * 4));
+    } else {
+        HIPCHK((hipError_t)trackAlloc(s, s->d_grad, size_t(N) * 8));
+        HIPCHK((hipError_t)trackAlloc(s, s->d_lim, size_t(N) * (lp ? 8 : 4)));
+    }
     HIPCHK((hipError_t)trackAlloc(s, s->d_lenF, size_t(F + 1) * 4));
     HIPCHK((hipError_t)trackAlloc(s, s->d_lenS, size_t(F + 1)));
     if (c.localTimestep) HIPCHK((hipError_t)trackAlloc(s, s->d_celldt, N));
@@ -701,6 +708,7 @@ int buildDevice(ebb_solver* s)
     }
     HIPCHK(hipMemcpy(s->d_qA.p, q0.data(), C * sizeof(double4), hipMemcpyHostToDevice));
     s->cur = 0;
+    refreshShadow(s);
     s->t0 = 0;
     s->dt0 = c.dt;
     launchInitScalars(s->d_sc, c.dt, 0.0, c.CFL, s->stream);
@@ -709,25 +717,41 @@ int buildDevice(ebb_solver* s)
     return EBB_OK;
 }
 
-}  // namespace
-
-// Every C entry point that parses, allocates or touches the device runs its body
-// through guarded(): an exception (std::bad_alloc, a length_error from a size read
-// out of a file, ...) becomes a status code instead of unwinding into the caller.
-template <typename F>
-auto guarded(F&& f) -> decltype(f())
+int createSolver(const ebb_mesh* m, const Config& cfg, int device, ebb_solver** out)
 {
-    try {
-        return f();
-    } catch (const std::bad_alloc&) {
-        return EBB_ERR_ALLOC;
-    } catch (const std::exception& e) {
-        std::fprintf(stderr, "ebbcfd: %s\n", e.what());
-        return EBB_ERR_PARSE;
-    } catch (...) {
-        return EBB_ERR_PARSE;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+        std::fprintf(stderr, "ebb_solver_create: no HIP device %d\n", device);
+        return EBB_ERR_DEVICE;
     }
+    // released to the caller on success; destroyed (device buffers included) on any failure
+    std::unique_ptr<ebb_solver, void (*)(ebb_solver*)> s(new ebb_solver(), ebb_solver_destroy);
+    s->raw = m->raw;
+    s->b = m->b;
+    s->cfg = cfg;
+    s->device = device;
+    if (s->cfg.precision != 0) {
+        // mixed precision runs the gradient and face kernels in FP32: not the LP vector
+        // limiter (an FP64 simplex) and not the single fused stage kernel
+        const Config& c = s->cfg;
+        const char* why = nullptr;
+        if (c.order > 1 && c.limited && c.lpThresh > 0) why = "the LP vector limiter (lpThresh > 0)";
+        else if (c.stagePath == 2) why = "stagePath \"fused\"";
+        if (why) {
+            std::fprintf(stderr, "ebb_solver_create: precision \"mixed\" does not support %s\n", why);
+            return EBB_ERR_ARG;
+        }
+    }
+    int rc = buildDevice(s.get());
+    if (rc != EBB_OK) {
+        std::fprintf(stderr, "ebb_solver_create: %s %s\n", s->errMsg.c_str(), lastDeviceError.c_str());
+        return rc;
+    }
+    *out = s.release();
+    return EBB_OK;
 }
+
+}  // namespace ebb
 
 // ==================================================================== C ABI
 extern "C" {
@@ -892,6 +916,7 @@ static double* cfgField(Config& c, const std::string& k, bool& isInt, long long*
     if (k == "R") return &c.R;
     if (k == "Re") return &c.Re;
     if (k == "L") return &c.L;
+    if (k == "rmsDrop") return &c.rmsDrop;
     if (k.size() == 3 && k[0] == 'i' && k[1] == 'c' && k[2] >= '0' && k[2] <= '3') return &c.ic[k[2] - '0'];
     isInt = true;
     if (k == "saveIter") ip = &c.saveIter;
@@ -956,28 +981,7 @@ int ebb_solver_create(const ebb_mesh* m, const ebb_config* c, int device, ebb_so
 {
     return guarded([&]() -> int {
         if (!m || !c || !out) return EBB_ERR_ARG;
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
-            std::fprintf(stderr, "ebb_solver_create: no HIP device %d\n", device);
-            return EBB_ERR_DEVICE;
-        }
-        // released to the caller on success; destroyed (device buffers included) on any failure
-        std::unique_ptr<ebb_solver, void (*)(ebb_solver*)> s(new ebb_solver(), ebb_solver_destroy);
-        s->raw = m->raw;
-        s->b = m->b;
-        s->cfg = c->c;
-        s->device = device;
-        if (s->cfg.precision != 0) {
-            std::fprintf(stderr, "ebb_solver_create: mixed precision is not available in this library build\n");
-            return EBB_ERR_ARG;
-        }
-        int rc = buildDevice(s.get());
-        if (rc != EBB_OK) {
-            std::fprintf(stderr, "ebb_solver_create: %s %s\n", s->errMsg.c_str(), lastDeviceError.c_str());
-            return rc;
-        }
-        *out = s.release();
-        return EBB_OK;
+        return createSolver(m, c->c, device, out);
     });
 }
 
@@ -1013,12 +1017,29 @@ static int downloadCells(ebb_solver* s, const void* devPtr, double* host, size_t
     return EBB_OK;
 }
 
+// The same for FP32 rows (mixed precision), widened to double.
+static int downloadCellsF(ebb_solver* s, const float* devPtr, double* host, size_t per)
+{
+    const size_t n = size_t(s->N) * per;
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpyAsync(f.data(), devPtr, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::vector<double> w(f.begin(), f.end());
+    toReference(s, w.data(), host, per);
+    return EBB_OK;
+}
+
 int ebb_state_upload(ebb_solver* s, const double* q)
 {
     return guarded([&]() -> int {
         if (!s || !q) return EBB_ERR_ARG;
         HIPCHK(hipSetDevice(s->device));
-        return uploadCells(s, q, s->qcur(), 4);
+        int rc = uploadCells(s, q, s->qcur(), 4);
+        if (rc) return rc;
+        refreshShadow(s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return EBB_OK;
     });
 }
 
@@ -1218,7 +1239,8 @@ static int refreshEdgeStates(ebb_solver* s)
     StageArgs a{};
     a.qin = s->lastInput; a.qn = s->lastInput; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
     a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.estate = s->d_estate.p; a.celldt = s->d_celldt.p;
-    a.sc = s->d_sc; a.kind = SK_RESIDUAL; a.limit = s->lastLimit; a.first = 0; a.checkActive = 0; a.recordErrors = 0;
+        a.sc = s->d_sc; a.kind = SK_RESIDUAL; a.limit = s->lastLimit; a.first = 0; a.checkActive = 0; a.recordErrors = 0;
+        mixArgs(s, a, s->lastInput, nullptr);
     // keep the device accumulators: the gradient kernel resets newDt, restore it afterwards
     { int rc_ = readScalars(s); if (rc_) return rc_; }
     DevScalars saved = *s->h_sc;
@@ -1267,14 +1289,21 @@ int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
         HIPCHK(hipSetDevice(s->device));
         // the fused stage kernel keeps gradients in LDS: recompute them for the last stage input
         if (s->useTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
-        if (grad) { int rc_ = downloadCells(s, s->d_grad.p, grad, 8); if (rc_) return rc_; }
+        if (grad) {
+            int rc_ = s->phys.mixed ? downloadCellsF(s, s->d_grad32.p, grad, 8) : downloadCells(s, s->d_grad.p, grad, 8);
+            if (rc_) return rc_;
+        }
         if (lim && !(s->phys.limited && s->lastLimit)) {
             // an unlimited evaluation resets every limiter to 1 (solve.d:324-340) and stores none
             std::fill(lim, lim + size_t(s->N) * 8, 1.0);
         } else if (lim) {
             const bool lp = s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0;
             std::vector<double> l(size_t(s->N) * (lp ? 8 : 4));
-            { int rc_ = downloadCells(s, s->d_lim.p, l.data(), lp ? 8 : 4); if (rc_) return rc_; }
+            {
+                int rc_ = s->phys.mixed ? downloadCellsF(s, s->d_lim32.p, l.data(), 4)
+                                        : downloadCells(s, s->d_lim.p, l.data(), lp ? 8 : 4);
+                if (rc_) return rc_;
+            }
             for (uint32_t i = 0; i < s->N; i++)
                 for (int k = 0; k < 4; k++) {
                     lim[size_t(i) * 8 + 2 * k] = lp ? l[size_t(i) * 8 + 2 * k] : l[size_t(i) * 4 + k];
@@ -1388,6 +1417,7 @@ int ebb_load_solution(ebb_solver* s, const char* path)
         }
         HIPCHK(hipSetDevice(s->device));
         { int rc_ = uploadCells(s, reinterpret_cast<const double*>(q.data()), s->qcur(), 4); if (rc_) return rc_; }
+        refreshShadow(s);
         return ebb_run_reset(s, t, dt);
     });
 }
@@ -1437,25 +1467,17 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
         // same launch sequence as stepStages, with an event after each kernel
         HIPCHK(hipEventRecord(ev[0], s->stream));
         size_t k = 1;
-        const Config& c = s->cfg;
-        const int msl = c.multistageLimiting ? 1 : 0, lts = c.localTimestep ? 1 : 0;
         for (int st = 0; st < steps; st++) {
-            double4 *q = s->qcur(), *nx = s->qnext(), *T = s->d_T.p, *T2 = s->d_T2.p;
-            struct St { int kind; double4 *in, *out; int limit, dtu, first, last; };
-            std::vector<St> seq;
-            switch (c.integratorKind()) {
-            case IntEuler: seq = {{SK_EULER, q, nx, 1, 1, 1, 1}}; break;
-            case IntRK2: seq = {{SK_RK2_1, q, T, 1, 1, 1, 0}, {SK_RK2_2, T, nx, msl, 0, 0, 1}}; break;
-            case IntRK2TVD: seq = {{SK_TVD_1, q, T, 1, lts, 1, 0}, {SK_TVD_2, T, nx, 0, lts ? 0 : 1, 0, 1}}; break;
-            default: seq = {{SK_RK4_1, q, T, 1, 1, 1, 0}, {SK_RK4_2, T, T2, msl, 0, 0, 0}, {SK_RK4_3, T2, T, msl, 0, 0, 0}, {SK_RK4_4, T, nx, msl, 0, 0, 1}};
-            }
+            double4* q = s->qcur();
+            const std::vector<StageDesc> seq = stepSequence(s);
             for (auto& x : seq) {
                 StageArgs a{};
                 a.qin = x.in; a.qn = q; a.qout = x.out; a.acc = s->d_acc.p; a.grad = s->d_grad.p; a.lim = s->d_lim.p;
                 a.lenF = s->d_lenF.p; a.lenS = s->d_lenS.p; a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
-                a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtu; a.first = x.first; a.last = x.last;
+                a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtUpdate; a.first = x.first; a.last = x.last;
                 a.rmsBlocks = (x.last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
                 a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
+            mixArgs(s, a, x.in, x.out);
                 s->stageStep.push_back(s->stepsTotal);
                 if (s->useTiles) {
                     launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
@@ -1522,6 +1544,8 @@ void ebb_solver_destroy(ebb_solver* s)
     s->d_tDesc.release(); s->d_tBlob.release();
     s->d_ftDesc.release(); s->d_ftRef.release(); s->d_ftIds.release(); s->d_ftCf.release(); s->d_ftFid.release(); s->d_node.release();
     s->d_devOf.release(); s->d_rmsBlocks.release();
+    s->d_qA32.release(); s->d_qB32.release(); s->d_T32.release(); s->d_T232.release();
+    s->d_grad32.release(); s->d_lim32.release();
     if (s->d_sc) (void)hipFree(s->d_sc);
     if (s->h_sc) (void)hipHostFree(s->h_sc);
     if (s->h_hist) (void)hipHostFree(s->h_hist);

@@ -251,3 +251,135 @@ class Solver:
     @property
     def device_bytes(self):
         return lib().ebb_solver_device_bytes(self.h)
+
+
+def estimate_device_bytes(mesh, config):
+    """Device bytes a Solver for this mesh and config would hold (admission by footprint)."""
+    if not isinstance(config, Config):
+        config = Config(config)
+    v = C.c_int64(0)
+    _check(lib().ebb_estimate_device_bytes(mesh.h, config.h, C.byref(v)), "estimate device bytes")
+    return v.value
+
+
+BATCH_STATUS = ("t", "dt", "steps", "running", "converged", "rho_rms_first", "rho_rms", "error", "error_index",
+                "device_bytes")
+
+
+class Batch:
+    """Many solves advanced by the same launches (no counterpart in the reference, whose
+    integrator scratch is process-static: source/integrators.d:56-75).  Each solve's states
+    and history equal its solo Solver.run's."""
+
+    def __init__(self, device=0, device_bytes_limit=200 << 30, graph_steps=8, keep_history=False):
+        self.h = C.c_void_p()
+        _check(lib().ebb_batch_create(int(device), int(device_bytes_limit), int(graph_steps), int(keep_history),
+                                      C.byref(self.h)), "batch create")
+        self.meshes = {}
+
+    def add(self, mesh, config):
+        """Admit a solve; returns its id.  Raises EbbError (code EBB_ERR_ALLOC = 10) when its
+        device footprint does not fit what the limit leaves."""
+        if not isinstance(config, Config):
+            config = Config(config)
+        i = C.c_int64(0)
+        _check(lib().ebb_batch_add(self.h, mesh.h, config.h, C.byref(i)), "batch add")
+        self.meshes[i.value] = mesh
+        return i.value
+
+    def remove(self, sid):
+        _check(lib().ebb_batch_remove(self.h, int(sid)), "batch remove")
+        self.meshes.pop(sid, None)
+
+    def run(self, steps):
+        """Advance every running solve by `steps` steps; returns how many are still running."""
+        r = C.c_int64(0)
+        _check(lib().ebb_batch_run(self.h, int(steps), C.byref(r)), "batch run")
+        return r.value
+
+    def status(self, sid):
+        out = np.zeros(10)
+        _check(lib().ebb_batch_status(self.h, int(sid), _p(out)), "batch status")
+        return dict(zip(BATCH_STATUS, out.tolist()))
+
+    def q(self, sid):
+        out = np.zeros((self.meshes[sid].n_interior, 4))
+        _check(lib().ebb_batch_state_download(self.h, int(sid), _p(out)), "batch state")
+        return out
+
+    def history(self, sid):
+        n = lib().ebb_batch_history_length(self.h, int(sid))
+        out = np.zeros((n, len(HIST_FIELDS)))
+        if n:
+            _check(lib().ebb_batch_history(self.h, int(sid), 0, int(n), _p(out)), "batch history")
+        return out
+
+    def info(self):
+        out = np.zeros(6, np.int64)
+        _check(lib().ebb_batch_info(self.h, _p(out, C.c_int64)), "batch info")
+        return dict(zip(("solves", "running", "device_bytes", "limit", "tile_cells", "graph_steps"), out.tolist()))
+
+    def tune(self, steps=16):
+        """Autotune tile size and graph length on the running solves (their states are kept)."""
+        out = np.zeros(4)
+        _check(lib().ebb_batch_tune(self.h, int(steps), _p(out)), "batch tune")
+        return dict(tile_cells=int(out[0]), graph_steps=int(out[1]), ms_per_step=out[2], default_ms_per_step=out[3])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ebb_batch_destroy(self.h)
+            self.h = None
+
+
+SERVER_STATS = ("submitted", "done", "failed", "running", "queued", "p50_s", "p99_s", "solves_per_s", "device_bytes",
+                "batched_run_s", "rounds", "age_s")
+
+
+class Server:
+    """Serving loop over a Batch (library-side loop thread; no counterpart in the reference):
+    requests queue as they arrive, are admitted by device footprint, and are stepped together
+    with the running solves until each stops (extensions.rmsDrop, maxSteps, tEnd)."""
+
+    def __init__(self, device=0, device_bytes_limit=200 << 30, max_solves=256, steps_per_round=100, graph_steps=8,
+                 keep_results=False):
+        self.h = C.c_void_p()
+        _check(lib().ebb_server_create(int(device), int(device_bytes_limit), int(max_solves), int(steps_per_round),
+                                       int(graph_steps), int(keep_results), C.byref(self.h)), "server create")
+        self._keep = {}
+
+    def submit(self, mesh, config):
+        if not isinstance(config, Config):
+            config = Config(config)
+        t = C.c_int64(0)
+        _check(lib().ebb_server_submit(self.h, mesh.h, config.h, C.byref(t)), "server submit")
+        self._keep[t.value] = (mesh, config)     # the request's mesh and config outlive it
+        return t.value
+
+    def poll(self, ticket):
+        out = np.zeros(16)
+        _check(lib().ebb_server_poll(self.h, int(ticket), _p(out)), "server poll")
+        d = dict(zip(("state", "error", "arrival_s", "admit_s", "done_s", "cells"), out[:6].tolist()))
+        d.update(zip(BATCH_STATUS, out[6:].tolist()))
+        return d
+
+    def result(self, ticket):
+        mesh = self._keep[ticket][0]
+        q = np.zeros((mesh.n_interior, 4))
+        _check(lib().ebb_server_result(self.h, int(ticket), _p(q)), "server result")
+        return q
+
+    def wait(self, n_done, timeout_s=600.0):
+        return lib().ebb_server_wait(self.h, int(n_done), float(timeout_s)) == 0
+
+    def stats(self):
+        out = np.zeros(12)
+        _check(lib().ebb_server_stats(self.h, _p(out)), "server stats")
+        return dict(zip(SERVER_STATS, out.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ebb_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

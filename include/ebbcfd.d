@@ -48,3 +48,27 @@ int ebb_save_solution_as(ebb_solver* s, const(char)* path, double t, double dt);
 int ebb_write_forces(ebb_solver* s, const(char)* path, long first, long count, int append);
 int ebb_time_steps(ebb_solver* s, int steps, double* out6);
 long ebb_solver_device_bytes(const(ebb_solver)* s);
+
+// batched solves (many meshes per launch) and the serving loop
+struct ebb_batch; struct ebb_server;
+int ebb_estimate_device_bytes(const(ebb_mesh)* m, const(ebb_config)* c, long* bytes);
+int ebb_batch_create(int device, long deviceBytesLimit, int graphSteps, int keepHistory, ebb_batch** b);
+int ebb_batch_add(ebb_batch* b, const(ebb_mesh)* m, const(ebb_config)* c, long* id);
+int ebb_batch_remove(ebb_batch* b, long id);
+int ebb_batch_run(ebb_batch* b, long steps, long* running);
+int ebb_batch_status(ebb_batch* b, long id, double* out10);
+int ebb_batch_state_download(ebb_batch* b, long id, double* q);
+long ebb_batch_history_length(ebb_batch* b, long id);
+int ebb_batch_history(ebb_batch* b, long id, long first, long count, double* rows);
+int ebb_batch_info(ebb_batch* b, long* info6);
+int ebb_batch_tune(ebb_batch* b, int steps, double* out4);
+long ebb_batch_device_bytes(const(ebb_batch)* b);
+void ebb_batch_destroy(ebb_batch* b);
+int ebb_server_create(int device, long deviceBytesLimit, int maxSolves, int stepsPerRound, int graphSteps,
+                      int keepResults, ebb_server** sv);
+int ebb_server_submit(ebb_server* sv, const(ebb_mesh)* m, const(ebb_config)* c, long* ticket);
+int ebb_server_poll(ebb_server* sv, long ticket, double* out16);
+int ebb_server_result(ebb_server* sv, long ticket, double* q);
+int ebb_server_wait(ebb_server* sv, long nDone, double timeoutS);
+int ebb_server_stats(ebb_server* sv, double* out12);
+void ebb_server_destroy(ebb_server* sv);

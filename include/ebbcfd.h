@@ -183,6 +183,90 @@ int64_t ebb_solver_device_bytes(const ebb_solver* s);
 
 void ebb_solver_destroy(ebb_solver* s);
 
+/* ---------------------------------------------------------------- batched solves
+ * No counterpart in the reference, whose integrator scratch is process-static and sized by
+ * the first mesh (source/integrators.d:56-75), so one process holds one mesh.  A batch
+ * holds many solvers (each the result of ufvmSetup on its own mesh and config, state
+ * resident on the device) and advances all of them with one launch per kernel: per RK
+ * stage one gradient launch and one face/update launch over every mesh's blocks and tiles,
+ * then one RMS launch and one epilogue launch, the whole step sequence captured as a HIP
+ * graph.  Each mesh's states and history equal its solo ebb_run's bit for bit.  Members
+ * use the face-tile stage path in the reference cell numbering; the LP vector limiter and
+ * the fused stage path are not batched. */
+typedef struct ebb_batch ebb_batch;
+
+/* Device bytes ebb_solver_create would allocate for this mesh and config (an estimate
+ * within a few percent; admission by footprint uses it). */
+int ebb_estimate_device_bytes(const ebb_mesh* m, const ebb_config* c, int64_t* bytes);
+
+/* A batch on HIP device `device` admitting solvers while their device bytes stay within
+ * device_bytes_limit.  graph_steps: steps per captured HIP graph (even; 0: plain
+ * launches).  keep_history: also copy each step's history row to the host. */
+int ebb_batch_create(int device, int64_t device_bytes_limit, int graph_steps, int keep_history, ebb_batch** out);
+
+/* Admit a solve (ufvmSetup on its own device state).  EBB_ERR_ALLOC when its footprint
+ * does not fit what the limit leaves; *id names it in the calls below. */
+int ebb_batch_add(ebb_batch* b, const ebb_mesh* m, const ebb_config* c, int64_t* id);
+
+/* Release a solve's device state (finished or not). */
+int ebb_batch_remove(ebb_batch* b, int64_t id);
+
+/* Advance every running solve by `steps` time-loop steps (app/solver.d:119-250); a solve
+ * stops by itself at tEnd, maxSteps, the rho-RMS drop of extensions.rmsDrop or an error.
+ * *running: solves still stepping afterwards. */
+int ebb_batch_run(ebb_batch* b, int64_t steps, int64_t* running);
+
+/* out[0..9]: t, dt, steps, running (1/0), converged (1/0), first-step rho RMS, last rho
+ * RMS, error kind (EBB_ERR_*, 0: none), reference edge/cell of the error, device bytes. */
+int ebb_batch_status(ebb_batch* b, int64_t id, double out[10]);
+/* Interior states of a solve, reference cell order, n_interior x 4. */
+int ebb_batch_state_download(ebb_batch* b, int64_t id, double* q);
+/* History rows of a solve (as ebb_history), kept when the batch was created with keep_history. */
+int64_t ebb_batch_history_length(ebb_batch* b, int64_t id);
+int ebb_batch_history(ebb_batch* b, int64_t id, int64_t first, int64_t count, double* out);
+/* info[0..5]: solves held, solves running, device bytes admitted, limit, cells per tile
+ * (-1: automatic), graph steps. */
+int ebb_batch_info(ebb_batch* b, int64_t info[6]);
+
+/* Autotuner: times `steps` steps of the running solves for each tile size (automatic, 512,
+ * 384, 256 cells) and graph length (0, 2, 8, 32 steps), keeps the fastest, and restores
+ * every solve's state as it was.  out: best tile cells, best graph steps, its ms per step,
+ * ms per step of the default (automatic tiles, 8-step graphs). */
+int ebb_batch_tune(ebb_batch* b, int steps, double out[4]);
+
+int64_t ebb_batch_device_bytes(const ebb_batch* b);
+void ebb_batch_destroy(ebb_batch* b);
+
+/* ---------------------------------------------------------------- serving loop
+ * No counterpart in the reference (one mesh per process).  A server owns a batch and a
+ * loop thread: requests (mesh + config) are queued as they arrive; each round the loop
+ * admits queued requests in arrival order while their device footprint fits
+ * device_bytes_limit and fewer than max_solves run, advances every running solve by
+ * steps_per_round steps with the batched launches, and retires the solves that stopped
+ * (extensions.rmsDrop, maxSteps, tEnd, error), freeing their memory.  Requests arriving
+ * during a round join at the next round boundary.  The mesh and config of a request must
+ * stay valid until the request is done.  Submission, polling and statistics may be called
+ * from any thread. */
+typedef struct ebb_server ebb_server;
+int ebb_server_create(int device, int64_t device_bytes_limit, int max_solves, int steps_per_round, int graph_steps,
+                      int keep_results, ebb_server** out);
+/* Queue a request; *ticket names it.  Its arrival time is now. */
+int ebb_server_submit(ebb_server* sv, const ebb_mesh* m, const ebb_config* c, int64_t* ticket);
+/* out[0..15]: state (0 queued, 1 running, 2 done, 3 failed), error code, arrival, admission
+ * and completion times (s since the server started; -1 before), interior cells, then the
+ * ten ebb_batch_status fields at completion. */
+int ebb_server_poll(ebb_server* sv, int64_t ticket, double out[16]);
+/* Final interior states of a done request (server created with keep_results), n_interior x 4. */
+int ebb_server_result(ebb_server* sv, int64_t ticket, double* q);
+/* Block until at least n_done requests are done or failed (EBB_ERR_STATE on timeout). */
+int ebb_server_wait(ebb_server* sv, int64_t n_done, double timeout_s);
+/* out[0..11]: submitted, done, failed, running, queued, p50 and p99 time to converged (s,
+ * arrival to completion, done requests), solves/s (done / span from the first arrival to
+ * the last completion), device bytes in use, seconds in batched runs, rounds, server age (s). */
+int ebb_server_stats(ebb_server* sv, double out[12]);
+/* Stops the loop after its current round and releases every solve. */
+void ebb_server_destroy(ebb_server* sv);
+
 #ifdef __cplusplus
 }
 #endif

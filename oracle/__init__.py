@@ -40,6 +40,15 @@ class OrcConfig(C.Structure):
     ]
 
 
+def set_threads(n):
+    """OpenMP threads of the residual and stage loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
 def build(force=False):
     """Compile liboracle.so with the committed Makefile."""
     if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(os.path.join(_HERE, "ebb_oracle.c")):
@@ -78,6 +87,8 @@ def lib():
         L.orc_convective_flux.argtypes = [d, d, d, d, d, P(d), P(d)]
         L.orc_diffusive_flux.argtypes = [d, d, P(d), P(d), P(d), P(d)]
         L.orc_solution.argtypes = [d, d, P(d)]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         L.orc_solution_gradient.argtypes = [d, d, P(d)]
         L.orc_source_term.argtypes = [d, d, C.c_int, d, d, d, P(d)]
         L.orc_limiter1d.restype = d

@@ -35,6 +35,16 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#include <omp.h>
+
+/* The per-cell and per-edge loops of one residual evaluation and of the stage updates run on
+   OpenMP threads: each iteration writes only its own cell or edge, min/max reductions are exact
+   in any order, and an error reports the first failing index in loop order, so the results are
+   those of the serial loops bit for bit (tests/test_oracle_threads.py checks it). */
+#define ORC_FIRST(v, kv, x, kind) do { _Pragma("omp critical(orc_first)") { if ((x) < (v)) { (v) = (x); (kv) = (kind); } } } while (0)
+
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 #define MAXE 6
 #define GAMMA 1.4
@@ -911,6 +921,9 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
 {
     const double PrC = cfg->Pr, mu = cfg->mu;
     /* ghost cell states, solve.d:520-689 */
+    uint32_t firstG = UINT32_MAX;
+    int kindG = 0;
+#pragma omp parallel for schedule(static)
     for (uint32_t b = 0; b < m->nBoundaryEdges; b++) {
         uint32_t gi = m->nElem + b;
         const OEdge *e = &m->edges[m->cells[gi].edges[0]];
@@ -971,7 +984,7 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
             if (M1 >= 0 && M2 >= 0) Mb = fmin(M1, M2);
             else if (M1 >= 0 && M2 < 0) Mb = M1;
             else if (M1 < 0 && M2 >= 0) Mb = M2;
-            else { m->errKind = ORC_ERR_BC; m->errIndex = m->cells[gi].edges[0]; return ORC_ERR_BC; }
+            else { ORC_FIRST(firstG, kindG, b, ORC_ERR_BC); continue; }
             double Tb = Tt / (1 + 0.5 * (GAMMA - 1) * (Mb * Mb));
             double pb = Pt * pow(Tb / Tt, GAMMA / (GAMMA - 1));
             double rhob = pb / (Rg * Tb);
@@ -983,13 +996,18 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
             break;
         }
         default:
-            m->errKind = ORC_ERR_BC; m->errIndex = m->cells[gi].edges[0];
-            return ORC_ERR_BC;
+            ORC_FIRST(firstG, kindG, b, ORC_ERR_BC);
+            continue;
         }
+    }
+    if (firstG != UINT32_MAX) {
+        m->errKind = kindG; m->errIndex = m->cells[m->nElem + firstG].edges[0];
+        return kindG;
     }
 
     /* buildGradients, solve.d:324-508 */
     if (cfg->order > 1) {
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < m->nElem; i++) {
             OCell *c = &m->cells[i];
             double du[4][MAXE];
@@ -1051,6 +1069,9 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
     }
 
     /* boundary edges, solve.d:769-968 */
+    uint32_t firstB = UINT32_MAX;
+    int kindB = 0;
+#pragma omp parallel for schedule(static)
     for (uint32_t bi = 0; bi < m->nBoundaryEdges; bi++) {
         uint32_t i = m->boundaryEdges[bi];
         OEdge *e = &m->edges[i];
@@ -1181,13 +1202,20 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
             break;
         }
         default:
-            m->errKind = ORC_ERR_BC; m->errIndex = i;
-            return ORC_ERR_BC;
+            ORC_FIRST(firstB, kindB, bi, ORC_ERR_BC);
+            continue;
         }
-        if (has_nan4(e->flux)) { m->errKind = ORC_ERR_EDGE_NAN; m->errIndex = i; return ORC_ERR_EDGE_NAN; }
+        if (has_nan4(e->flux)) ORC_FIRST(firstB, kindB, bi, ORC_ERR_EDGE_NAN);
+    }
+    if (firstB != UINT32_MAX) {
+        m->errKind = kindB; m->errIndex = m->boundaryEdges[firstB];
+        return kindB;
     }
 
     /* interior edges, solve.d:970-1033 */
+    uint32_t firstI = UINT32_MAX;
+    int kindI = 0;
+#pragma omp parallel for schedule(static)
     for (uint32_t ii = 0; ii < m->nInteriorEdges; ii++) {
         uint32_t i = m->interiorEdges[ii];
         OEdge *e = &m->edges[i];
@@ -1220,11 +1248,20 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
         } else {
             for (int k = 0; k < 4; k++) e->flux[k] = F[k];
         }
-        if (has_nan4(e->flux)) { m->errKind = ORC_ERR_EDGE_NAN; m->errIndex = i; return ORC_ERR_EDGE_NAN; }
+        if (has_nan4(e->flux)) ORC_FIRST(firstI, kindI, ii, ORC_ERR_EDGE_NAN);
+    }
+    if (firstI != UINT32_MAX) {
+        m->errKind = kindI; m->errIndex = m->interiorEdges[firstI];
+        return kindI;
     }
 
     /* residual, solve.d:1080-1118 */
-    *newDt = INFINITY;
+    double ndAll = INFINITY, rmAll = *Rmax;
+#pragma omp parallel
+    {
+    double ndT = INFINITY, rmT = 0;
+    double *newDtT = &ndT, *RmaxT = &rmT;
+#pragma omp for schedule(static)
     for (uint32_t i = 0; i < m->nElem; i++) {
         OCell *c = &m->cells[i];
         double r[4] = {0, 0, 0, 0};
@@ -1236,15 +1273,23 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
             sAve += e->len * e->sMax;
         }
         sAve /= c->perim;
-        *newDt = fmin(*newDt, (cfg->CFL * c->d) / sAve);
+        *newDtT = fmin(*newDtT, (cfg->CFL * c->d) / sAve);
         if (cfg->localTimestep && dtUpdate) {
             c->dt = cfg->CFL * c->d / sAve;
             if (isnan(c->dt)) c->dt = 0;
         }
-        for (int k = 0; k < 4; k++) if (fabs(r[k]) > *Rmax) *Rmax = fabs(r[k]);
+        for (int k = 0; k < 4; k++) if (fabs(r[k]) > *RmaxT) *RmaxT = fabs(r[k]);
         double s = -(1.0 / c->area);
         for (int k = 0; k < 4; k++) R[i][k] = r[k] * s;
     }
+#pragma omp critical(orc_reduce)
+    {
+        ndAll = fmin(ndAll, ndT);
+        if (rmT > rmAll) rmAll = rmT;
+    }
+    }
+    *newDt = ndAll;
+    *Rmax = rmAll;
     return ORC_OK;
 }
 
@@ -1256,6 +1301,7 @@ static int stage_residual(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], doub
     int rc = orc_ufvm_solver(m, cfg, R, q, newDt, Rmax, limit, dtUpdate);
     if (rc) return rc;
     if (cfg->mms)
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < m->nElem; i++)
             for (int k = 0; k < 4; k++) R[i][k] = R[i][k] + m->src[i][k];
     return ORC_OK;
@@ -1279,6 +1325,7 @@ int orc_step(OrcMesh *m, const OrcConfig *cfg, double *dt, double *Rmax)
     switch (cfg->integrator) {
     case INT_EULER:
         if ((rc = stage_residual(m, cfg, m->R, q, &newDt, Rmax, 1, 1))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) {
             double h = lts ? m->cells[i].dt : *dt;
             for (int k = 0; k < 4; k++) q[i][k] = q[i][k] + h * m->R[i][k];
@@ -1287,12 +1334,16 @@ int orc_step(OrcMesh *m, const OrcConfig *cfg, double *dt, double *Rmax)
     case INT_RK4: {
         const double h = *dt;
         if ((rc = stage_residual(m, cfg, m->k1, q, &newDt, Rmax, 1, 1))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) for (int k = 0; k < 4; k++) m->tmp[i][k] = q[i][k] + ((h / 2.0) * m->k1[i][k]);
         if ((rc = stage_residual(m, cfg, m->k2, m->tmp, &newDt, Rmax, cfg->multistageLimiting, 0))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) for (int k = 0; k < 4; k++) m->tmp[i][k] = q[i][k] + ((h / 2.0) * m->k2[i][k]);
         if ((rc = stage_residual(m, cfg, m->k3, m->tmp, &newDt, Rmax, cfg->multistageLimiting, 0))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) for (int k = 0; k < 4; k++) m->tmp[i][k] = q[i][k] + (h * m->k3[i][k]);
         if ((rc = stage_residual(m, cfg, m->k4, m->tmp, &newDt, Rmax, cfg->multistageLimiting, 0))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) {
             double hh = lts ? m->cells[i].dt : h;
             for (int k = 0; k < 4; k++)
@@ -1303,11 +1354,13 @@ int orc_step(OrcMesh *m, const OrcConfig *cfg, double *dt, double *Rmax)
     case INT_RK2TVD: {
         const double h = *dt;
         if ((rc = stage_residual(m, cfg, m->R, q, &newDt, Rmax, 1, lts))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) {
             double hh = lts ? m->cells[i].dt : h;
             for (int k = 0; k < 4; k++) m->k1[i][k] = q[i][k] + hh * m->R[i][k];  /* qFE */
         }
         if ((rc = stage_residual(m, cfg, m->k2, m->k1, &newDt, Rmax, 0, !lts))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) {
             double hh = lts ? m->cells[i].dt : h;
             for (int k = 0; k < 4; k++) q[i][k] = 0.5 * (q[i][k] + m->k1[i][k] + hh * m->k2[i][k]);
@@ -1317,8 +1370,10 @@ int orc_step(OrcMesh *m, const OrcConfig *cfg, double *dt, double *Rmax)
     case INT_RK2: {
         const double h = *dt;
         if ((rc = stage_residual(m, cfg, m->k1, q, &newDt, Rmax, 1, 1))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) for (int k = 0; k < 4; k++) m->tmp[i][k] = q[i][k] + (h * m->k1[i][k]);
         if ((rc = stage_residual(m, cfg, m->k2, m->tmp, &newDt, Rmax, cfg->multistageLimiting, 0))) return rc;
+#pragma omp parallel for schedule(static)
         for (uint32_t i = 0; i < N; i++) {
             double hh = lts ? m->cells[i].dt : h;
             for (int k = 0; k < 4; k++) q[i][k] = q[i][k] + (hh / 2.0) * (m->k1[i][k] + m->k2[i][k]);

@@ -1,0 +1,129 @@
+"""Batched solves (many meshes per launch) against solo runs.
+
+Each solve in a batch runs the solo kernels' code on its own (mesh, block/tile) work
+items, so its per-step history and its states must equal a solo Solver.run of the same
+mesh and config bit for bit -- whatever else is in the batch, with or without graph
+capture, and after the autotuner (which must also restore every state).  Admission is
+by device footprint.
+"""
+import numpy as np
+import pytest
+
+from ebbcfd_amd import Batch, Solver, build_mesh, estimate_device_bytes, meshgen
+
+from helpers import channel_case, cylinder_case, mms_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _channel(nx, ny, h, seed, M, aoa, **over):
+    raw = meshgen.channel_bump(nx, ny, h, seed=seed)
+    q = [M, aoa, 1.0, 1.0]
+    bcs = [{"tag": "Inflow", "type": "fullState", "q": q}, {"tag": "Outflow", "type": "fullState", "q": q},
+           {"tag": "Lower", "type": "inviscidWall", "q": q}, {"tag": "Upper", "type": "inviscidWall", "q": q}]
+    from helpers import base_config
+    cfg = base_config(initialConditions=q, boudaryConditions=bcs, forceBoundary="Lower", flux="roeFlux",
+                      integrator="RK4", **over)
+    return raw, cfg
+
+
+def _cases():
+    rng = np.random.Generator(np.random.PCG64(11))
+    out = []
+    for k in range(6):
+        nx = int(rng.integers(12, 90))
+        out.append(_channel(nx, max(4, nx // 3), float(rng.uniform(0.02, 0.1)), int(rng.integers(1, 1000)),
+                            float(rng.uniform(0.3, 1.2)), float(rng.uniform(-5, 5))))
+    out.append(mms_case(n=24))                                        # Dirichlet + source, RK4
+    out.append(cylinder_case(n_theta=64, n_r=32, CFL=0.05))           # viscous, Rusanov, TVD-RK2, force faces
+    raw, cfg = channel_case(integrator="RK2", localTimestep=True)     # local time steps, RK2
+    out.append((raw, cfg))
+    return out
+
+
+@pytest.mark.parametrize("graph_steps", [0, 8])
+def test_batch_equals_solo_runs(graph_steps):
+    cases = _cases()
+    steps = 120
+    b = Batch(graph_steps=graph_steps, keep_history=True)
+    ids = [b.add(build_mesh(raw), cfg) for raw, cfg in cases]
+    assert b.run(steps) == len(cases)
+    for (raw, cfg), sid in zip(cases, ids):
+        s = Solver(build_mesh(raw), cfg)
+        hs, t, dt = s.run(steps)
+        hb = b.history(sid)
+        st = b.status(sid)
+        assert len(hb) == steps and st["steps"] == steps
+        np.testing.assert_array_equal(hb, hs)
+        np.testing.assert_array_equal(b.q(sid), s.q)
+        assert (st["t"], st["dt"]) == (t, dt)
+
+
+def test_batch_members_join_and_leave():
+    """Continuous batching: a solve that joins late, or whose neighbours leave, still equals
+    its solo run; stopped solves drop out."""
+    cases = _cases()[:5]
+    b = Batch(graph_steps=8, keep_history=True)
+    first = [b.add(build_mesh(raw), cfg) for raw, cfg in cases[:3]]
+    b.run(40)
+    b.remove(first[1])
+    late = [b.add(build_mesh(raw), cfg) for raw, cfg in cases[3:]]
+    b.run(60)
+    plan = {first[0]: (cases[0], 100), first[2]: (cases[2], 100), late[0]: (cases[3], 60), late[1]: (cases[4], 60)}
+    for sid, ((raw, cfg), n) in plan.items():
+        s = Solver(build_mesh(raw), cfg)
+        hs, _, _ = s.run(n)
+        np.testing.assert_array_equal(b.history(sid), hs)
+        np.testing.assert_array_equal(b.q(sid), s.q)
+
+
+def test_batch_convergence_stop_and_max_steps():
+    raw, cfg = _channel(30, 10, 0.05, 3, 0.5, 1.0, limited=False, localTimestep=True)
+    cfg["extensions"] = {"rmsDrop": 1e-3}
+    raw2, cfg2 = _channel(20, 8, 0.05, 4, 0.6, 0.0)
+    cfg2["extensions"] = {"maxSteps": 37}
+    b = Batch(graph_steps=8, keep_history=True)
+    a = b.add(build_mesh(raw), cfg)
+    c = b.add(build_mesh(raw2), cfg2)
+    left = 2
+    for _ in range(100):
+        left = b.run(400)
+        if left == 0:
+            break
+    sa, sc = b.status(a), b.status(c)
+    assert sc["steps"] == 37 and not sc["running"]
+    assert sa["converged"] == 1 and sa["rho_rms"] <= 1e-3 * sa["rho_rms_first"]
+    s = Solver(build_mesh(raw), cfg)
+    hs, _, _ = s.run()
+    assert len(hs) == sa["steps"]
+    np.testing.assert_array_equal(b.q(a), s.q)
+
+
+def test_batch_admission_by_footprint():
+    raw, cfg = _channel(60, 20, 0.05, 5, 0.5, 0.0)
+    m = build_mesh(raw)
+    need = estimate_device_bytes(m, cfg)
+    s = Solver(m, cfg)
+    assert 0.85 * s.device_bytes <= need <= 1.25 * s.device_bytes, (need, s.device_bytes)
+    b = Batch(device_bytes_limit=int(2.5 * need))
+    b.add(m, cfg)
+    b.add(m, cfg)
+    with pytest.raises(Exception) as e:
+        b.add(m, cfg)
+    assert getattr(e.value, "code", None) == 10
+    assert b.info()["solves"] == 2
+
+
+def test_batch_tuner_keeps_states():
+    cases = _cases()[:4]
+    b = Batch(graph_steps=8, keep_history=True)
+    ids = [b.add(build_mesh(raw), cfg) for raw, cfg in cases]
+    b.run(16)
+    res = b.tune(steps=8)
+    assert res["tile_cells"] in (-1, 512, 384, 256) and res["graph_steps"] in (0, 2, 8, 32)
+    b.run(24)
+    for (raw, cfg), sid in zip(cases, ids):
+        s = Solver(build_mesh(raw), cfg)
+        hs, _, _ = s.run(40)
+        np.testing.assert_array_equal(b.history(sid), hs)
+        np.testing.assert_array_equal(b.q(sid), s.q)
