@@ -53,7 +53,7 @@ GroupKey groupKey(const ebb_solver* s, const std::vector<StageDesc>& seq)
     int lim[4] = {0, 0, 0, 0};
     for (size_t k = 0; k < seq.size() && k < 4; k++) lim[k] = seq[k].limit;
     return GroupKey(int(s->S), s->phys.flux, s->phys.viscous, s->phys.order, s->phys.limited, s->phys.mixed,
-                    int(seq.size()), lim[0], lim[1], lim[2], lim[3], 0);
+                    int(seq.size()), lim[0], lim[1], lim[2], lim[3], s->phys.venkat);
 }
 
 template <typename T>
@@ -161,8 +161,8 @@ int rebuild(ebb_batch* b)
         auto it = gidx.find(key);
         if (it == gidx.end()) {
             BatchGroup g;
-            g.shape = GroupShape{int(s->S), s->phys.flux, s->phys.viscous, s->phys.order, s->phys.limited, 0,
-                                 s->phys.mixed};
+            g.shape = GroupShape{int(s->S), s->phys.flux, s->phys.viscous, s->phys.order, s->phys.limited,
+                                 s->phys.venkat, s->phys.mixed};
             g.nst = int(seq.size());
             for (size_t q = 0; q < seq.size() && q < 4; q++) g.limit[q] = seq[q].limit;
             it = gidx.emplace(key, b->groups.size()).first;

@@ -249,6 +249,15 @@ bool parseConfig(const std::string& json, Config& c, std::string& err)
             else if (e->s == "blocks") c.tileOrder = 2;
             else { err = "unknown tileOrder " + e->s; return false; }
         }
+        if ((e = v->get("limiter")) && e->kind == JVal::Str) {
+            if (e->s == "scalar") c.limiter = 0;
+            else if (e->s == "venkatakrishnan") c.limiter = 1;
+            else { err = "unknown limiter " + e->s; return false; }
+        }
+        if ((e = v->get("venkatK")) && e->isNum()) {
+            if (!(e->num() >= 0)) { err = "venkatK must be >= 0"; return false; }
+            c.venkatK = e->num();
+        }
         if ((e = v->get("precision")) && e->kind == JVal::Str) {
             if (e->s == "mixed") c.precision = 1;
             else if (e->s == "fp64") c.precision = 0;

@@ -90,6 +90,8 @@ struct PhysParams {
     double Pr, mu, Rgas, lpThresh;
     int localTimestep;
     int mixed;           // 1: fp32 flux/reconstruction/limiter
+    int venkat;          // 1: Venkatakrishnan limiter in place of the reference's scalar limiter
+    double venkatK;      // its constant K: eps^2 = (K d)^3 with d = 2 area / perimeter of the cell
 };
 
 // One stage of an integrator.  `kind` selects the update applied to the residual:
@@ -284,7 +286,7 @@ void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStrea
 // shape (S, flux, viscous, order, limiter mode, precision) is the group's; items are
 // (mesh record, local index).  parity selects each mesh's state buffers (see MemberArgs).
 struct GroupShape {
-    int S, flux, viscous, order, limited, lp, mixed;
+    int S, flux, viscous, order, limited, venkat, mixed;
 };
 void launchGradientBatch(const GroupShape& g, int limitStage, const MemberArgs* mem, const uint2* items,
                          uint32_t nItems, uint32_t parity, uint32_t stage, hipStream_t stream);

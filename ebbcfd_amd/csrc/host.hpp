@@ -80,6 +80,8 @@ struct Config {
     double rmsDrop = 0;               // stop once the rho RMS residual drops by this factor below its
                                       // first-step value (0: never)
     int precision = 0;                // 0 = fp64, 1 = mixed fp32 flux/reconstruction/limiter
+    int limiter = 0;                  // 0: the reference's scalar limiter; 1: Venkatakrishnan
+    double venkatK = 1.0;             // Venkatakrishnan constant K (eps^2 = (K d)^3)
     int stagePath = 1;                // 0: gradient, face, update kernels; 1: gradient + face/update tiles;
                                       // 2: one fused kernel per stage
     long long tileCells = -1;         // cells per tile of the tiled paths (-1: automatic)

@@ -380,10 +380,14 @@ DEV void lsqWeights(double2 ci, const double2 cn[S], double w[2][S])
 // gm: stored weights ((dir, nbr)-major, stride gstride between entries), or null
 // to compute them from the centroids.  Weights and centroid offsets are geometry,
 // computed in FP64 and converted to R; the gradient, bounds and limiter are R.
+// MODE 4: the Venkatakrishnan limiter at the same test points (neighbour centroids): for
+// the unlimited increment t = grad . (x_j - x_i) and the bound difference d = max - q_i (t > 0)
+// or min - q_i (t < 0), phi_j = [(d^2 + eps2) t + 2 t^2 d] / (t [d^2 + 2 t^2 + d t + eps2]),
+// phi = min(1, min_j phi_j); eps2 = (K d_cell)^3 (FP64, passed in).
 template <int S, int MODE, typename R>
 DEV void gradLimit(const R qi[4], const R qn[S][4], const double* gm, uint32_t gstride, double2 ci,
                    const double2 cn[S], const bool real[S], int limit, double lpThresh, R g[4][2], R lx[4],
-                   R ly[4])
+                   R ly[4], double eps2d = 0.0)
 {
     static_assert(MODE != 3 || sizeof(R) == 8, "the LP vector limiter runs in FP64 only");
     R w[2][S];
@@ -424,7 +428,26 @@ DEV void gradLimit(const R qi[4], const R qn[S][4], const double* gm, uint32_t g
         }
 #pragma unroll
     for (int k = 0; k < 4; k++) { lx[k] = R(1.0); ly[k] = R(1.0); }
-    if (MODE >= 2 && limit) {
+    if (MODE == 4 && limit) {
+        const R eps2 = R(eps2d), two = R(2.0);
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const R dx = R(cn[j].x - ci.x), dy = R(cn[j].y - ci.y);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const R t = g[k][0] * dx + g[k][1] * dy;
+                R s = R(1.0);
+                if (t != R(0.0)) {
+                    const R d = t > R(0.0) ? mx[k] - qi[k] : mn[k] - qi[k];
+                    s = ((d * d + eps2) * t + two * (t * t) * d) / (t * (d * d + two * (t * t) + d * t + eps2));
+                }
+                lx[k] = fmin(lx[k], s);
+                ly[k] = lx[k];
+            }
+        }
+        return;
+    }
+    if (MODE >= 2 && MODE <= 3 && limit) {
         R dxs[S], dys[S];
 #pragma unroll
         for (int j = 0; j < S; j++) {
@@ -564,7 +587,7 @@ DEV void pendingForce(const ForceArgs& f, DevScalars* sc, double2* fp)
 // and of the limiter rows
 template <int MODE>
 struct GradLds {
-    static constexpr int LWS = MODE == 3 ? 8 : (MODE == 2 ? 4 : 0);
+    static constexpr int LWS = MODE == 3 ? 8 : ((MODE == 2 || MODE == 4) ? 4 : 0);
     double sg[BLK][9];
     double sl[BLK][LWS + 1];
 };
@@ -642,7 +665,12 @@ DEV void gradientBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
             bool real[S];
 #pragma unroll
             for (int j = 0; j < S; j++) real[j] = nb[j] != i;
-            gradLimit<S, MODE, R>(qi, qn, nullptr, 0, ci, cn, real, a.limit, p.lpThresh, g, lx, ly);
+            double eps2 = 0.0;
+            if constexpr (MODE == 4) {
+                const double kd = p.venkatK * m.cd[i];
+                eps2 = kd * kd * kd;
+            }
+            gradLimit<S, MODE, R>(qi, qn, nullptr, 0, ci, cn, real, a.limit, p.lpThresh, g, lx, ly, eps2);
         }
     }
     if (MODE == 0) return;
@@ -656,7 +684,7 @@ DEV void gradientBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
         if (have) {
 #pragma unroll
             for (int c = 0; c < 4; c++) { fg[8 * tid + 2 * c] = g[c][0]; fg[8 * tid + 2 * c + 1] = g[c][1]; }
-            if (MODE == 2) {
+            if (MODE == 2 || MODE == 4) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) fl[4 * tid + c] = lx[c];
             }
@@ -665,7 +693,7 @@ DEV void gradientBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
         float4* gd = reinterpret_cast<float4*>(a.grad32 + size_t(base) * 8);
         for (uint32_t c = tid; c < 2 * n; c += BLK)
             __builtin_nontemporal_store(*reinterpret_cast<const ntf4*>(fg + 4 * c), reinterpret_cast<ntf4*>(gd) + c);
-        if (MODE == 2) {
+        if (MODE == 2 || MODE == 4) {
             float4* ld = reinterpret_cast<float4*>(a.lim32 + size_t(base) * 4);
             for (uint32_t c = tid; c < n; c += BLK)
                 __builtin_nontemporal_store(*reinterpret_cast<const ntf4*>(fl + 4 * c), reinterpret_cast<ntf4*>(ld) + c);
@@ -678,7 +706,7 @@ DEV void gradientBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
             if (MODE == 3) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) { sl[tid][2 * c] = lx[c]; sl[tid][2 * c + 1] = ly[c]; }
-            } else if (MODE == 2) {
+            } else if (MODE == 2 || MODE == 4) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) sl[tid][c] = lx[c];
             }
@@ -2159,11 +2187,12 @@ static void gradS(const DevMesh& m, const PhysParams& p, const StageArgs& a, hip
 {
     const uint32_t g = nblk(m.N) + (a.force.n ? 1u : 0u);   // + the force workgroup
     if (g == 0) return;
-    const int mode = p.order == 1 ? 0 : (!(p.limited && a.limit) ? 1 : (p.lpThresh > 0 ? 3 : 2));
+    const int mode = p.order == 1 ? 0 : (!(p.limited && a.limit) ? 1 : (p.venkat ? 4 : (p.lpThresh > 0 ? 3 : 2)));
     if (p.mixed) {
         switch (mode) {
         case 0: hipLaunchKernelGGL((k_gradient<S, 0, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
         case 1: hipLaunchKernelGGL((k_gradient<S, 1, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+        case 4: hipLaunchKernelGGL((k_gradient<S, 4, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
         default: hipLaunchKernelGGL((k_gradient<S, 2, 1>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
         }
         return;
@@ -2172,6 +2201,7 @@ static void gradS(const DevMesh& m, const PhysParams& p, const StageArgs& a, hip
     case 0: hipLaunchKernelGGL((k_gradient<S, 0, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     case 1: hipLaunchKernelGGL((k_gradient<S, 1, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     case 2: hipLaunchKernelGGL((k_gradient<S, 2, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
+    case 4: hipLaunchKernelGGL((k_gradient<S, 4, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     default: hipLaunchKernelGGL((k_gradient<S, 3, 0>), dim3(g), dim3(BLK), 0, st, m, p, a); break;
     }
 }
@@ -2352,6 +2382,7 @@ static void gradBatchS(int mode, const MemberArgs* mem, const uint2* items, uint
     switch (mode) {
     case 0: hipLaunchKernelGGL((k_gradient_batch<S, 0, MIX>), g, b, 0, s, mem, items, par, st); break;
     case 1: hipLaunchKernelGGL((k_gradient_batch<S, 1, MIX>), g, b, 0, s, mem, items, par, st); break;
+    case 4: hipLaunchKernelGGL((k_gradient_batch<S, 4, MIX>), g, b, 0, s, mem, items, par, st); break;
     default: hipLaunchKernelGGL((k_gradient_batch<S, 2, MIX>), g, b, 0, s, mem, items, par, st); break;
     }
 }
@@ -2361,7 +2392,7 @@ void launchGradientBatch(const GroupShape& gs, int limitStage, const MemberArgs*
 {
     if (nItems == 0) return;
     // batched groups run without the LP vector limiter (mode 3)
-    const int mode = gs.order == 1 ? 0 : (!(gs.limited && limitStage) ? 1 : 2);
+    const int mode = gs.order == 1 ? 0 : (!(gs.limited && limitStage) ? 1 : (gs.venkat ? 4 : 2));
     if (gs.mixed) {
         if (gs.S == 3) gradBatchS<3, 1>(mode, mem, items, nItems, parity, stage, st);
         else if (gs.S == 4) gradBatchS<4, 1>(mode, mem, items, nItems, parity, stage, st);

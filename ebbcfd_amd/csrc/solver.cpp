@@ -604,6 +604,8 @@ int buildDevice(ebb_solver* s)
     p.lpThresh = c.lpThresh;
     p.localTimestep = c.localTimestep ? 1 : 0;
     p.mixed = c.precision;
+    p.venkat = (c.limiter == 1) ? 1 : 0;
+    p.venkatK = c.venkatK;
     // MMS source at centroids
     std::vector<double4> src;
     if (c.mms) {
@@ -730,6 +732,11 @@ int createSolver(const ebb_mesh* m, const Config& cfg, int device, ebb_solver** 
     s->b = m->b;
     s->cfg = cfg;
     s->device = device;
+    if (s->cfg.limiter == 1 && (s->cfg.stagePath == 2 || (s->cfg.order > 1 && s->cfg.limited && s->cfg.lpThresh > 0))) {
+        std::fprintf(stderr, "ebb_solver_create: the Venkatakrishnan limiter runs on the face-tile and split stage "
+                             "paths, without the LP vector limiter\n");
+        return EBB_ERR_ARG;
+    }
     if (s->cfg.precision != 0) {
         // mixed precision runs the gradient and face kernels in FP32: not the LP vector
         // limiter (an FP64 simplex) and not the single fused stage kernel
@@ -917,6 +924,7 @@ static double* cfgField(Config& c, const std::string& k, bool& isInt, long long*
     if (k == "Re") return &c.Re;
     if (k == "L") return &c.L;
     if (k == "rmsDrop") return &c.rmsDrop;
+    if (k == "venkatK") return &c.venkatK;
     if (k.size() == 3 && k[0] == 'i' && k[1] == 'c' && k[2] >= '0' && k[2] <= '3') return &c.ic[k[2] - '0'];
     isInt = true;
     if (k == "saveIter") ip = &c.saveIter;

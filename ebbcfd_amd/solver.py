@@ -358,7 +358,7 @@ class Server:
     def poll(self, ticket):
         out = np.zeros(16)
         _check(lib().ebb_server_poll(self.h, int(ticket), _p(out)), "server poll")
-        d = dict(zip(("state", "error", "arrival_s", "admit_s", "done_s", "cells"), out[:6].tolist()))
+        d = dict(zip(("state", "code", "arrival_s", "admit_s", "done_s", "cells"), out[:6].tolist()))
         d.update(zip(BATCH_STATUS, out[6:].tolist()))
         return d
 

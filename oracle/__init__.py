@@ -37,6 +37,7 @@ class OrcConfig(C.Structure):
         ("Pr", C.c_double), ("R", C.c_double), ("Re", C.c_double), ("L", C.c_double), ("mu", C.c_double),
         ("nBC", C.c_int), ("bcType", C.c_int * 32), ("bcData", (C.c_double * 4) * 32), ("bcNData", C.c_int * 32),
         ("tagToBc", C.c_int * 32), ("nForce", C.c_int), ("forceTags", C.c_int * 32),
+        ("limiter", C.c_int), ("venkatK", C.c_double),
     ]
 
 
@@ -203,6 +204,9 @@ def make_config(cfg, mesh_tags):
     c.tEnd = float(cfg["tEnd"])
     c.CFL = float(cfg.get("CFL", 0.5))
     c.lpThresh = float(cfg.get("lpThresh", -1))
+    ext = cfg.get("extensions", {})
+    c.limiter = {"scalar": 0, "venkatakrishnan": 1}[ext.get("limiter", "scalar")]
+    c.venkatK = float(ext.get("venkatK", 1.0))
     for i in range(4):
         c.ic[i] = float(cfg["initialConditions"][i])
     pc = cfg.get("physicalConfig")

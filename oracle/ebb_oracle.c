@@ -96,6 +96,8 @@ typedef struct {
     int tagToBc[32];      /* mesh boundary tag index -> config boundary index */
     int nForce;
     int forceTags[32];    /* mesh boundary tag indices, in forceBoundary order */
+    int limiter;          /* 0: the reference's scalar limiter; 1: Venkatakrishnan (build option) */
+    double venkatK;       /* Venkatakrishnan constant: eps^2 = (K d)^3, d = 2 area / perimeter */
 } OrcConfig;
 
 typedef struct {
@@ -1027,7 +1029,29 @@ int orc_ufvm_solver(OrcMesh *m, const OrcConfig *cfg, double (*R)[4], double (*q
                     for (int k = 0; k < MAXE; k++) s += c->gradMat[r][k] * du[j][k];
                     c->gradient[j][r] = s;
                 }
-            if (cfg->limited && limit) {
+            if (cfg->limited && limit && cfg->limiter == 1) {
+                /* Venkatakrishnan, J. Comput. Phys. 118 (1995), eq. 15 with the smooth min():
+                   for the unlimited change D2 to a test point and the bound difference D1,
+                   phi = [(D1^2 + eps^2) D2 + 2 D2^2 D1] / [D2 (D1^2 + 2 D2^2 + D1 D2 + eps^2)],
+                   at the neighbour centroids the reference's scalar limiter uses; no counterpart
+                   in the reference */
+                const double kd = cfg->venkatK * c->d, eps2 = kd * kd * kd;
+                for (uint32_t j = 0; j < c->nNeighborCells; j++) {
+                    const double *cn = m->cells[c->neighborCells[j]].centroid;
+                    double dx = cn[0] - c->centroid[0], dy = cn[1] - c->centroid[1];
+                    for (int k = 0; k < 4; k++) {
+                        const double d2 = c->gradient[k][0] * dx + c->gradient[k][1] * dy;
+                        double phi = 1.0;
+                        if (d2 != 0.0) {
+                            const double d1 = d2 > 0.0 ? c->maxQ[k] - q[i][k] : c->minQ[k] - q[i][k];
+                            phi = ((d1 * d1 + eps2) * d2 + 2.0 * (d2 * d2) * d1) /
+                                  (d2 * (d1 * d1 + 2.0 * (d2 * d2) + d1 * d2 + eps2));
+                        }
+                        c->lim[k][0] = fmin(c->lim[k][0], phi);
+                        c->lim[k][1] = c->lim[k][0];
+                    }
+                }
+            } else if (cfg->limited && limit) {
                 for (uint32_t j = 0; j < c->nNeighborCells; j++) {
                     const double *cn = m->cells[c->neighborCells[j]].centroid;
                     double dx = cn[0] - c->centroid[0], dy = cn[1] - c->centroid[1];

@@ -97,6 +97,11 @@ STEP_CASES = {
     "pentagons_rk2tvd_viscous": lambda: (hanging_node_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
     "mixed_rk2tvd_viscous": lambda: (mixed_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
     "mixed_rk4_lp": lambda: (mixed_mesh(seed=8), channel_case(integrator="RK4", lpThresh=1.0)[1]),
+    # the Venkatakrishnan limiter (an extensions option; the oracle restates it, pinned by
+    # tests/test_oracle_mesh.py's property checks)
+    "venkat_rk4_roe": lambda: channel_case(integrator="RK4", extensions={"limiter": "venkatakrishnan", "venkatK": 0.5}),
+    "venkat_cyl_ns_rk2tvd": lambda: cylinder_case(extensions={"limiter": "venkatakrishnan", "venkatK": 1.0}),
+    "venkat_quads_mixed": lambda: (mixed_mesh(seed=3), channel_case(integrator="RK2", extensions={"limiter": "venkatakrishnan"})[1]),
 }
 
 

@@ -70,5 +70,5 @@ def test_admission_waits_for_memory():
     bm = build_mesh(big[0])
     t = sv.submit(bm, big[1])
     assert sv.wait(5, 60.0)
-    assert sv.poll(t)["state"] == 3 and sv.poll(t)["error"] == 10
+    assert sv.poll(t)["state"] == 3 and sv.poll(t)["code"] == 10
     sv.close()

@@ -258,3 +258,31 @@ def test_gri_reader_matches_oracle(q, tmp_path):
     # same geometry as the mesh it was written from (q=2 only adds edge midpoints)
     ob = oracle.Mesh(raw)
     np.testing.assert_array_equal(om.cells()["area"], ob.cells()["area"])
+
+
+def test_host_mms_values_equal_the_oracle():
+    """The manufactured solution and source the library precomputes on the host (for the
+    device's Dirichlet ghosts, edge states and per-cell source) equal the oracle's bit for
+    bit on the 8,192 jittered centroids of the 64 x 64 square.  Both call sin and cos
+    separately, as the reference does; a sincos fusion by either compiler changes the last
+    bit of cos for some of these arguments."""
+    import ctypes as C
+    from ebbcfd_amd import _lib
+    L = C.CDLL(_lib.LIB_PATH)
+    src = L._ZN3ebb13mmsSourceTermEddbdddPd
+    src.argtypes = [C.c_double, C.c_double, C.c_bool, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    sol = L._ZN3ebb11mmsSolutionEddPd
+    sol.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
+    raw = meshgen.mms_square(64, 0.1, 1)
+    m = oracle.Mesh(raw)
+    cen = m.cells()["centroid"]
+    out = (C.c_double * 4)()
+    bad = 0
+    for x, y in cen:
+        src(x, y, False, 1.0, 0.0, 0.71, out)
+        bad += list(out) != oracle.source_term(x, y, viscosity=False).tolist()
+        src(x, y, True, 1.0, 0.01, 0.71, out)
+        bad += list(out) != oracle.source_term(x, y, viscosity=True, R=1.0, mu=0.01, Pr=0.71).tolist()
+        sol(x, y, out)
+        bad += list(out) != oracle.solution(x, y).tolist()
+    assert bad == 0

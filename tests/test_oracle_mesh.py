@@ -263,3 +263,60 @@ def test_limiter_is_tight():
                 gap.append(min(abs(qe - lo[k]), abs(qe - hi[k])))
             assert min(gap) <= 1e-12 * max(1.0, abs(q[i, k])), (i, k, l[i, k, 0], min(gap))
     assert limited > 50
+
+
+def _venkat_state(K, smooth=False, seed=1):
+    """limiter values and the data it was computed on, for the channel with a perturbed state
+    (or, smooth=True, a smooth linear-plus-quadratic field)"""
+    raw, cfg = channel_case(limited=True)
+    cfg["extensions"] = {"limiter": "venkatakrishnan", "venkatK": K}
+    m = oracle.Mesh(raw).setup(oracle_cfg(cfg))
+    q = m.q
+    N = m.n_interior
+    c = m.cells()
+    if smooth:
+        x, y = c["centroid"][:N, 0], c["centroid"][:N, 1]
+        base = np.array([1.0, 0.4, 0.1, 2.5])
+        q[:N] = base * (1 + 0.05 * (x[:, None] - 1.5) + 0.02 * (y[:, None] - 0.5) ** 2)
+    else:
+        rng = np.random.default_rng(seed)
+        q[:N] *= 1 + 0.1 * rng.uniform(-1, 1, (N, 4))
+    m.q = q
+    m.residual(limit=True)
+    g, l, _ = m.cell_data()
+    return m, m.q, g, l, c, N
+
+
+def test_venkatakrishnan_phi_in_unit_interval():
+    for K in (0.0, 0.3, 1.0, 5.0):
+        _, q, g, l, c, N = _venkat_state(K)
+        assert np.all(l[:N] >= 0.0) and np.all(l[:N] <= 1.0), K
+        np.testing.assert_array_equal(l[:N, :, 0], l[:N, :, 1])
+
+
+def test_venkatakrishnan_bounds_reconstruction_as_k_vanishes():
+    """With eps = 0 the limited change to every neighbour centroid stays inside the cell's
+    min/max (the reference scalar limiter's property): phi D2 / D1 = (y + 2) / (y^2 + y + 2)
+    <= 1 for y = D1 / D2 >= 0."""
+    m, q, g, l, c, N = _venkat_state(0.0)
+    limited = 0
+    for i in range(N):
+        nb = c["nbrs"][i, : int(c["nEdges"][i])]
+        lo, hi = np.minimum(q[i], q[nb].min(axis=0)), np.maximum(q[i], q[nb].max(axis=0))
+        for j in nb:
+            dx = c["centroid"][j] - c["centroid"][i]
+            qe = q[i] + l[i, :, 0] * (g[i, :, 0] * dx[0] + g[i, :, 1] * dx[1])
+            assert np.all(qe >= lo - 1e-12 * np.abs(lo)) and np.all(qe <= hi + 1e-12 * np.abs(hi))
+        limited += int(np.sum(l[i, :, 0] < 1))
+    assert limited > 50
+
+
+def test_venkatakrishnan_relaxes_on_smooth_data_as_k_grows():
+    """On smooth data the limiter switches off as K grows (eps^2 = (K d)^3 dominates the
+    differences), where the scalar limiter keeps clipping the extrema."""
+    means = []
+    for K in (0.1, 1.0, 10.0, 100.0):
+        _, q, g, l, c, N = _venkat_state(K, smooth=True)
+        means.append(float(l[:N].mean()))
+    assert all(b >= a - 1e-12 for a, b in zip(means, means[1:])), means
+    assert means[-1] > 0.999 and means[0] < means[-1], means
