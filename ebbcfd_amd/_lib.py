@@ -13,13 +13,13 @@ LIB_PATH = os.path.join(_HERE, "libebbcfd.so")
 # every symbol include/ebbcfd.h declares
 SYMBOLS = (
     "ebb_error_string", "ebb_mesh_load", "ebb_mesh_create", "ebb_mesh_info", "ebb_mesh_cell_geometry",
-    "ebb_mesh_edge_geometry", "ebb_mesh_tile_stats", "ebb_mesh_destroy", "ebb_config_parse", "ebb_config_get", "ebb_config_set",
+    "ebb_mesh_edge_geometry", "ebb_mesh_tile_stats", "ebb_mesh_save_mmsh", "ebb_mesh_destroy", "ebb_config_parse", "ebb_config_get", "ebb_config_set",
     "ebb_config_mesh_file", "ebb_config_destroy", "ebb_solver_create", "ebb_state_upload", "ebb_state_download",
     "ebb_state_download_all", "ebb_residual", "ebb_step", "ebb_run", "ebb_run_state", "ebb_run_reset",
     "ebb_history", "ebb_history_length", "ebb_edge_states", "ebb_cell_gradients", "ebb_save_solution",
     "ebb_save_solution_as",
     "ebb_load_solution", "ebb_write_forces", "ebb_last_error", "ebb_time_steps", "ebb_solver_device_bytes",
-    "ebb_solver_destroy", "ebb_estimate_device_bytes", "ebb_batch_create", "ebb_batch_add", "ebb_batch_remove",
+    "ebb_solver_destroy", "ebb_limiter1d", "ebb_estimate_device_bytes", "ebb_batch_create", "ebb_batch_add", "ebb_batch_remove",
     "ebb_batch_run", "ebb_batch_status", "ebb_batch_state_download", "ebb_batch_history_length", "ebb_batch_history",
     "ebb_batch_info", "ebb_batch_tune", "ebb_batch_device_bytes", "ebb_batch_destroy", "ebb_server_create",
     "ebb_server_submit", "ebb_server_poll", "ebb_server_result", "ebb_server_wait", "ebb_server_stats",
@@ -55,6 +55,7 @@ def lib():
     L.ebb_mesh_cell_geometry.argtypes = [vp, P(d), P(d), P(d), P(d), P(d), P(u32), P(C.c_int8), P(u32), P(C.c_uint8)]
     L.ebb_mesh_edge_geometry.argtypes = [vp, P(u32), P(u32), P(d), P(d), P(d), P(C.c_int32)]
     L.ebb_mesh_tile_stats.argtypes = [vp, C.c_int, C.c_int, P(d)]
+    L.ebb_mesh_save_mmsh.argtypes = [vp, C.c_char_p]
     L.ebb_mesh_destroy.argtypes = [vp]
     L.ebb_mesh_destroy.restype = None
     L.ebb_config_parse.argtypes = [C.c_char_p, P(vp)]
@@ -87,6 +88,7 @@ def lib():
     L.ebb_solver_device_bytes.restype = i64
     L.ebb_solver_destroy.argtypes = [vp]
     L.ebb_solver_destroy.restype = None
+    L.ebb_limiter1d.argtypes = [C.c_int, C.c_int, i64, P(d), P(d), P(d)]
     L.ebb_estimate_device_bytes.argtypes = [vp, vp, P(i64)]
     L.ebb_batch_create.argtypes = [C.c_int, i64, C.c_int, C.c_int, P(vp)]
     L.ebb_batch_add.argtypes = [vp, vp, vp, P(i64)]

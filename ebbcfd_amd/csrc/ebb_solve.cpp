@@ -1,6 +1,7 @@
 // ebb-solve: command-line driver with the reference's interface (app/solver.d:430-470):
 //   ebb-solve -c <config.json> [-s <save prefix>] [--device N]
-// Outputs as the reference: stdout lines every plotIter steps (app/solver.d:220-230),
+// Outputs as the reference: <meshbase>_0.mmsh at start-up (app/solver.d:364-372), stdout
+// lines every plotIter steps (app/solver.d:220-230),
 // save_<n>_0.esln every saveIter steps, final_0.esln, boundaryForces.frc, and the
 // timing summary of app/solver.d:380-390.  SIGINT/SIGUSR1 stop the loop cleanly
 // and still write final_0.esln (app/solver.d:30-36).
@@ -54,6 +55,12 @@ int main(int argc, char** argv)
     if (!saveFile.empty()) saveFile += "0.esln";
     ebb_mesh* mesh = nullptr;
     if ((rc = ebb_mesh_load(meshFile, &mesh))) return die("mesh", rc);
+    {
+        // the display mesh, named as config.meshFile.split('.')[0] ~ "_<rank>.mmsh" (app/solver.d:372)
+        const std::string mf(meshFile);
+        const std::string mmsh = mf.substr(0, mf.find('.')) + "_0.mmsh";
+        if ((rc = ebb_mesh_save_mmsh(mesh, mmsh.c_str()))) return die("mmsh", rc);
+    }
     ebb_solver* s = nullptr;
     if ((rc = ebb_solver_create(mesh, cfg, device, &s))) return die("solver", rc);
     if (!saveFile.empty() && (rc = ebb_load_solution(s, saveFile.c_str()))) return die("Failed to load solution", rc);

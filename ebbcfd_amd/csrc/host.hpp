@@ -102,6 +102,8 @@ struct Config {
 bool parseGmsh(const std::string& path, RawMesh& out, std::string& err);
 bool parseGri(const std::string& path, RawMesh& out, std::string& err);
 bool buildMesh(const RawMesh& raw, BuiltMesh& out, std::string& err);
+// triangulate + buildMesh + saveMatlabMesh (source/mesh.d:678-715, source/io.d:534-643)
+bool saveMatlabMesh(const RawMesh& raw, const std::string& path, std::string& err);
 
 // config.cpp
 bool parseConfig(const std::string& json, Config& out, std::string& err);

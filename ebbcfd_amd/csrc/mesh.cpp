@@ -662,4 +662,79 @@ bool buildMesh(const RawMesh& raw, BuiltMesh& m, std::string& err)
     return true;
 }
 
+// The display mesh ebb-solve writes at start-up (app/solver.d:364-372): every cell split
+// into a fan of triangles (element[0], element[j+1], element[j+2]) as source/mesh.d:678-715
+// triangulate does, that mesh built again (buildMesh, its own edge numbering), and its
+// nodes, triangles, interior edges (nodes, both cells) and boundary edges (nodes, owner,
+// boundary tag) written as source/io.d:534-643 saveMatlabMesh does: big-endian u64 counts,
+// every index 1-based and stored as a double, the tag names as u32 length + bytes.
+bool saveMatlabMesh(const RawMesh& raw, const std::string& path, std::string& err)
+{
+    RawMesh t;
+    t.nodes = raw.nodes;
+    t.bnodes = raw.bnodes;
+    t.btags = raw.btags;
+    t.bgroupStart = raw.bgroupStart;
+    t.eoff.assign(1, 0);
+    const size_t ne = raw.eoff.empty() ? 0 : raw.eoff.size() - 1;
+    for (size_t e = 0; e < ne; e++) {
+        const uint32_t* n = raw.enodes.data() + raw.eoff[e];
+        const uint32_t m = raw.eoff[e + 1] - raw.eoff[e];
+        if (m == 3) {
+            t.enodes.insert(t.enodes.end(), n, n + 3);
+            t.eoff.push_back(uint32_t(t.enodes.size()));
+        } else {
+            for (uint32_t j = 0; j + 2 < m; j++) {
+                t.enodes.push_back(n[0]);
+                t.enodes.push_back(n[j + 1]);
+                t.enodes.push_back(n[j + 2]);
+                t.eoff.push_back(uint32_t(t.enodes.size()));
+            }
+        }
+    }
+    BuiltMesh b;
+    if (!buildMesh(t, b, err)) return false;
+    std::vector<uint8_t> buf;
+    auto u64 = [&](uint64_t v) { for (int i = 7; i >= 0; i--) buf.push_back(uint8_t(v >> (8 * i))); };
+    auto f64 = [&](double d) { uint64_t v; std::memcpy(&v, &d, 8); u64(v); };
+    const size_t nn = t.nodes.size() / 2, nt = t.eoff.size() - 1;
+    buf.reserve(40 + nn * 16 + nt * 24 + size_t(b.nEdges) * 32);
+    u64(nn);
+    for (size_t i = 0; i < nn; i++) { f64(t.nodes[2 * i]); f64(t.nodes[2 * i + 1]); }
+    u64(nt);
+    for (size_t e = 0; e < nt; e++)
+        for (int k = 0; k < 3; k++) f64(double(t.enodes[3 * e + size_t(k)]));   // elements are 1-based already
+    uint64_t nie = 0;
+    for (uint32_t e = 0; e < b.nEdges; e++) nie += b.eBGroup[e] < 0 ? 1 : 0;
+    u64(nie);
+    for (uint32_t e = 0; e < b.nEdges; e++)
+        if (b.eBGroup[e] < 0) {
+            f64(double(b.eNode0[e]) + 1); f64(double(b.eNode1[e]) + 1);
+            f64(double(b.eCell0[e]) + 1); f64(double(b.eCell1[e]) + 1);
+        }
+    u64(uint64_t(b.nEdges) - nie);
+    for (uint32_t e = 0; e < b.nEdges; e++)
+        if (b.eBGroup[e] >= 0) {
+            f64(double(b.eNode0[e]) + 1); f64(double(b.eNode1[e]) + 1);
+            f64(double(b.eCell0[e]) + 1);
+            // the index of the edge's tag among the tags (countUntil) + 1
+            const std::string& tag = t.btags[size_t(b.eBGroup[e])];
+            size_t ti = 0;
+            while (ti < t.btags.size() && t.btags[ti] != tag) ti++;
+            f64(double(ti) + 1);
+        }
+    u64(t.btags.size());
+    for (const std::string& tag : t.btags) {
+        const uint32_t L = uint32_t(tag.size());
+        for (int i = 3; i >= 0; i--) buf.push_back(uint8_t(L >> (8 * i)));
+        buf.insert(buf.end(), tag.begin(), tag.end());
+    }
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { err = "cannot write " + path; return false; }
+    const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+    std::fclose(f);
+    if (w != buf.size()) { err = "short write to " + path; return false; }
+    return true;
+}
+
 }  // namespace ebb

@@ -898,6 +898,19 @@ int ebb_mesh_tile_stats(const ebb_mesh* m, int tile_cells, int order, double out
     });
 }
 
+int ebb_mesh_save_mmsh(const ebb_mesh* m, const char* path)
+{
+    return guarded([&]() -> int {
+        if (!m || !path) return EBB_ERR_ARG;
+        std::string err;
+        if (!saveMatlabMesh(m->raw, path, err)) {
+            std::fprintf(stderr, "ebb_mesh_save_mmsh: %s\n", err.c_str());
+            return err.rfind("cannot write", 0) == 0 || err.rfind("short write", 0) == 0 ? EBB_ERR_IO : EBB_ERR_MESH;
+        }
+        return EBB_OK;
+    });
+}
+
 void ebb_mesh_destroy(ebb_mesh* m) { delete m; }
 
 int ebb_config_parse(const char* json, ebb_config** out)

@@ -127,6 +127,10 @@ class UMesh2:
                 "max_local", "lds_bytes")
         return dict(zip(keys, out.tolist()))
 
+    def save_mmsh(self, path):
+        """triangulate + buildMesh + saveMatlabMesh (source/mesh.d:678-715, source/io.d:534-643)."""
+        _check(lib().ebb_mesh_save_mmsh(self.h, str(path).encode()), "saveMatlabMesh")
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().ebb_mesh_destroy(self.h)
@@ -383,3 +387,16 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+LIMITERS_1D = ("firstOrderUpwindS", "laxWendroffS", "minmodS", "harmonicS", "geometricS", "superbeeS")
+
+
+def limiter1d(kind, sp, sm, device=0):
+    """source/limiters.d:7-37 on the device, elementwise over the pairs (sp, sm)."""
+    k = LIMITERS_1D.index(kind) if isinstance(kind, str) else int(kind)
+    a = np.ascontiguousarray(sp, np.float64).ravel()
+    b = np.ascontiguousarray(sm, np.float64).ravel()
+    out = np.empty_like(a)
+    _check(lib().ebb_limiter1d(int(device), k, a.size, _p(a), _p(b), _p(out)), "limiter1d")
+    return out

@@ -35,6 +35,7 @@ int ebb_mesh_cell_geometry(const(ebb_mesh)* m, double* area, double* d, double* 
 int ebb_mesh_edge_geometry(const(ebb_mesh)* m, uint* nodes, uint* cells, double* len, double* normal,
                            double* mid, int* bgroup);
 int ebb_mesh_tile_stats(const(ebb_mesh)* m, int tileCells, int order, double* out8);
+int ebb_mesh_save_mmsh(const(ebb_mesh)* m, const(char)* path);          // <meshbase>_<rank>.mmsh
 int ebb_config_get(const(ebb_config)* c, const(char)* key, double* value);
 int ebb_config_set(ebb_config* c, const(char)* key, double value);
 int ebb_config_mesh_file(const(ebb_config)* c, char* buf, size_t len);
@@ -48,6 +49,7 @@ int ebb_save_solution_as(ebb_solver* s, const(char)* path, double t, double dt);
 int ebb_write_forces(ebb_solver* s, const(char)* path, long first, long count, int append);
 int ebb_time_steps(ebb_solver* s, int steps, double* out6);
 long ebb_solver_device_bytes(const(ebb_solver)* s);
+int ebb_limiter1d(int device, int kind, long n, const(double)* sp, const(double)* sm, double* result);
 
 // batched solves (many meshes per launch) and the serving loop
 struct ebb_batch; struct ebb_server;

@@ -82,6 +82,11 @@ int ebb_mesh_edge_geometry(const ebb_mesh* m, uint32_t* nodes, uint32_t* cells, 
  * workgroup (scalar limiter layout). */
 int ebb_mesh_tile_stats(const ebb_mesh* m, int tile_cells, int order, double out[8]);
 
+/* The display mesh ebb-solve writes at start-up as <meshbase>_<rank>.mmsh
+ * (app/solver.d:364-372): triangulate (source/mesh.d:678-715), buildMesh on the triangles,
+ * saveMatlabMesh (source/io.d:534-643; big-endian, indices 1-based as doubles). */
+int ebb_mesh_save_mmsh(const ebb_mesh* m, const char* path);
+
 void ebb_mesh_destroy(ebb_mesh* m);
 
 /* ---------------------------------------------------------------- config */
@@ -182,6 +187,12 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6]);
 int64_t ebb_solver_device_bytes(const ebb_solver* s);
 
 void ebb_solver_destroy(ebb_solver* s);
+
+/* The 1-D slope limiters of source/limiters.d:7-37 (kind 0 firstOrderUpwindS,
+ * 1 laxWendroffS, 2 minmodS, 3 harmonicS, 4 geometricS, 5 superbeeS) applied on HIP device
+ * `device` to n pairs: out[i] = limiter(sp[i], sm[i]).  Host arrays; ebb-solve never calls
+ * these limiters (the solver uses the scalar and LP limiters of source/solve.d). */
+int ebb_limiter1d(int device, int kind, int64_t n, const double* sp, const double* sm, double* out);
 
 /* ---------------------------------------------------------------- batched solves
  * No counterpart in the reference, whose integrator scratch is process-static and sized by

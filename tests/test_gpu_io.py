@@ -121,6 +121,8 @@ def test_cli_matches_library(tmp_path):
     raw, cfg = _write_case(tmp_path, dict(dynamicDt=False, dt=2.0e-4, tEnd=0.00205, saveIter=4, plotIter=3))
     r = subprocess.run([CLI, "-c", "cfg.json"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    # the display mesh of app/solver.d:364-372, named after the mesh file
+    assert os.path.getsize(tmp_path / "bump_0.mmsh") > 0
     s = Solver(import_mesh(str(tmp_path / "bump.msh")), cfg)
     h, t, dt = s.run(-1)
     assert len(h) == 11
