@@ -230,6 +230,25 @@ def serve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def dropin_leg(s, N, stages, ms_per_stage_loop):
+    """The drop-in at the reference's operator boundary (source/integrators.d calling
+    ufvmSolver once per stage): per stage, the caller's state goes to the device
+    (ebb_state_upload), ebb_residual runs and R comes back to the host.  Host buffers are
+    the caller's (pageable numpy), so this is the PCIe-inclusive rate of that use."""
+    q = s.q
+    s.ufvm_solver()
+    t0 = time.perf_counter()
+    for _ in range(stages):
+        s.q = q
+        s.ufvm_solver()
+    el = (time.perf_counter() - t0) / stages
+    log("drop-in at ufvmSolver: %.2f ms per stage (upload q + residual + download R), vs %.3f ms per stage "
+        "inside the device loop" % (el * 1e3, ms_per_stage_loop))
+    return {"ms_per_stage": round(el * 1e3, 3), "Mcell_stage_per_s": round(N / el / 1e6, 1),
+            "bytes_over_pcie_per_stage": int(N * 64), "stages": stages,
+            "loop_ms_per_stage": round(ms_per_stage_loop, 4)}
+
+
 def c1_leg(steps):
     """Latency of the manufactured-solution Euler workload: 8,192 jittered triangles,
     Roe, scalar limiter, RK4, CFL 0.5 -- cache-resident, reported for latency."""
@@ -345,6 +364,8 @@ def main():
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision form of the workload (0: skip)")
     ap.add_argument("--c1-steps", type=int, default=1000,
                     help="steps of the 8,192-cell manufactured-solution latency leg (0: skip)")
+    ap.add_argument("--dropin-stages", type=int, default=10,
+                    help="stages timed at the ufvmSolver boundary (state upload + ebb_residual + R download; 0: skip)")
     ap.add_argument("--batched-meshes", type=int, default=256, help="meshes of the batched leg (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=64, help="steps per solve timed in the batched leg")
     ap.add_argument("--batched-seed", type=int, default=1)
@@ -426,6 +447,7 @@ def main():
     traffic = MEASURED_TRAFFIC.get((kernel, workload, args.stage_path, args.tile_cells, args.tile_order,
                                     args.cell_order))
 
+    dropin = dropin_leg(s, N, args.dropin_stages, stage_ms) if args.dropin_stages > 0 else None
     mixed = None
     if args.mixed:
         mixed = mixed_leg(mesh, cfg, s, args, N, value, ms_per_step)
@@ -476,6 +498,7 @@ def main():
                            "unit": "GB/s"},
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "dropin_ufvmSolver": dropin,
         "c1": c1,
         "batched": batched,
     }
