@@ -15,7 +15,7 @@ SYMBOLS = (
     "ebb_error_string", "ebb_mesh_load", "ebb_mesh_create", "ebb_mesh_info", "ebb_mesh_cell_geometry",
     "ebb_mesh_edge_geometry", "ebb_mesh_tile_stats", "ebb_mesh_save_mmsh", "ebb_mesh_destroy", "ebb_config_parse", "ebb_config_get", "ebb_config_set",
     "ebb_config_mesh_file", "ebb_config_destroy", "ebb_solver_create", "ebb_state_upload", "ebb_state_download",
-    "ebb_state_download_all", "ebb_residual", "ebb_step", "ebb_run", "ebb_run_state", "ebb_run_reset",
+    "ebb_state_download_all", "ebb_residual", "ebb_residual_noalloc", "ebb_step", "ebb_run", "ebb_run_state", "ebb_run_reset",
     "ebb_history", "ebb_history_length", "ebb_edge_states", "ebb_cell_gradients", "ebb_save_solution",
     "ebb_save_solution_as",
     "ebb_load_solution", "ebb_write_forces", "ebb_last_error", "ebb_time_steps", "ebb_solver_device_bytes",
@@ -69,6 +69,7 @@ def lib():
     L.ebb_state_download.argtypes = [vp, P(d)]
     L.ebb_state_download_all.argtypes = [vp, P(d)]
     L.ebb_residual.argtypes = [vp, C.c_int, C.c_int, P(d), P(d), P(d)]
+    L.ebb_residual_noalloc.argtypes = [vp, C.c_int, C.c_int, vp, P(d), P(d)]
     L.ebb_step.argtypes = [vp, P(d), P(d)]
     L.ebb_run.argtypes = [vp, i64, P(d), P(d), P(i64)]
     L.ebb_run_state.argtypes = [vp, P(d), P(d), P(d), P(i64)]

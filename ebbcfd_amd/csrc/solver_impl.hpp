@@ -162,6 +162,8 @@ ForceArgs forceArgs(const ebb_solver* s);
 bool deferForce(const ebb_solver* s);
 int stagesPerStep(const Config& c);
 void refreshShadow(ebb_solver* s);
+// the mixed-precision pointers of a stage reading qin and writing qout (null on the FP64 path)
+void mixArgs(ebb_solver* s, StageArgs& a, const double4* qin, const double4* qout);
 int readScalars(ebb_solver* s);
 int checkErrors(ebb_solver* s);
 // device bytes a solver for this mesh and config would allocate (admission by footprint)

@@ -45,6 +45,7 @@ int ebb_run_reset(ebb_solver* s, double t, double dt);
 long ebb_history_length(ebb_solver* s);
 int ebb_edge_states(ebb_solver* s, double* eq, double* flux, double* smax); // Edge.q[2], flux, sMax
 int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim);
+int ebb_residual_noalloc(ebb_solver* s, int limit, int dtUpdate, double* RDev, double* newDt, double* Rmax);
 int ebb_save_solution_as(ebb_solver* s, const(char)* path, double t, double dt);
 int ebb_write_forces(ebb_solver* s, const(char)* path, long first, long count, int append);
 int ebb_time_steps(ebb_solver* s, int steps, double* out6);

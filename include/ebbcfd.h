@@ -131,6 +131,19 @@ int ebb_state_download_all(ebb_solver* s, double* q);
  * configured), new_dt (min CFL*d/sAve) and max-accumulates *rmax. */
 int ebb_residual(ebb_solver* s, int limit, int dt_update, double* R, double* new_dt, double* rmax);
 
+/* ufvmSolver as ebb_residual computes it, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid it); it has no
+ * counterpart in the reference.  It allocates nothing: R_dev is a device buffer of
+ * n_interior x 4 doubles the caller owns (reference cell order; solvers that renumber their
+ * cells are rejected), no edge states are stored, and the update kernel's per-workgroup
+ * partials live in a fixed device array of 65,536 pairs, so meshes above 16,777,216
+ * interior cells are rejected (EBB_ERR_ALLOC).  That array is shared by every call on the
+ * GPU: calls that overlap on one GPU race.  With new_dt and rmax both NULL the call only
+ * queues work on the solver's stream (no synchronisation; results stay in the solver's
+ * device scalars); otherwise it waits and returns them as ebb_residual does.  The first
+ * call on a device looks the array's address up: make it before any capture. */
+int ebb_residual_noalloc(ebb_solver* s, int limit, int dt_update, double* R_dev, double* new_dt, double* rmax);
+
 /* Integrator.step (source/integrators.d:22-270): one step with *dt; on return
  * *dt is the last stage's new time step and *rmax the step's max |R|. */
 int ebb_step(ebb_solver* s, double* dt, double* rmax);
