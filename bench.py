@@ -4,7 +4,8 @@
 Workload (default): FP64 Navier-Stokes flow past a cylinder, O-grid of
 2048 azimuthal x 1024 radial cells (geometric stretching 1.006, R=0.5 to 30),
 each quad split into 2 triangles along a seeded diagonal -> 4,194,304 cells;
-M=0.2, Re=100, Rusanov flux, scalar limiter, TVD-RK2 (2 stages per step),
+M=0.2, Re=100, Rusanov flux, Venkatakrishnan limiter (K = 1; --limiter scalar
+for the reference's own limiter, solve.d:360-403), TVD-RK2 (2 stages per step),
 CFL 0.05.  (The reference's CFL time step, solve.d:1092-1094, has no viscous
 limit; with 3.9e-4 wall spacing and nu = 0.01 it diverges within three steps
 at CFL 0.5 (the CPU restatement shows the same) and within a hundred steps
@@ -77,11 +78,24 @@ def cpu_baseline(raw, cfg, steps, n_cells, threads):
     return n_cells * 2 * steps / el / 1e6, el
 
 
-def mixed_leg(mesh, cfg, s64, args, N, value64, ms64):
+def state_error(q64, q32):
+    """Per-cell state error of a mixed-precision run against the FP64 one: for each variable
+    the largest |difference| over the cells divided by that variable's largest |FP64 value|
+    (a near-zero component, such as v in the far field, would make a per-cell ratio
+    meaningless), then the worst variable; and the same in the 2-norm."""
+    d = np.abs(q64 - q32)
+    scale = np.abs(q64).max(axis=0)
+    mx = float((d.max(axis=0) / scale).max())
+    l2 = float((np.sqrt((d * d).sum(axis=0)) / np.sqrt((q64 * q64).sum(axis=0))).max())
+    return mx, l2
+
+
+def mixed_leg(mesh, cfg, args, N, value64):
     """The same workload in mixed precision (FP32 flux, reconstruction and limiter; FP64
-    geometry, accumulation and norms), timed like the FP64 line, and its error against the
-    FP64 solver after the same steps from the same initial state (the FP64 path equals the
-    oracle bit for bit)."""
+    geometry, accumulation and norms), timed like the FP64 line.  Its error is measured on
+    the workload's full run: `--c2-steps` steps (5000) of the FP64 and the mixed solver from
+    the same initial state (the FP64 path equals the oracle bit for bit), comparing the
+    states, drag and lift."""
     from ebbcfd_amd import Solver
     c = dict(cfg)
     c["extensions"] = dict(cfg["extensions"], precision="mixed")
@@ -94,22 +108,37 @@ def mixed_leg(mesh, cfg, s64, args, N, value64, ms64):
     ms = el / args.steps * 1e3
     v = N * 2 * args.steps / el / 1e6
     tk = s.time_steps(args.time_steps)
-    h64 = s64.history()
-    h32 = s.history()
-    n = min(len(h64), len(h32))
-    q64, q32 = s64.q, s.q
-    st = float((np.abs(q64 - q32) / np.abs(q64)).max())
-    drag = abs(h64[n - 1, 7] - h32[n - 1, 7]) / abs(h64[n - 1, 7])
-    lift = abs(h64[n - 1, 8] - h32[n - 1, 8]) / max(abs(h64[n - 1, 8]), abs(h64[n - 1, 7]))
-    log("mixed: %.3f ms/step, %.1f Mcell-stage/s (x%.3f of FP64); tile %.3f ms, gradient %.3f ms; "
-        "after %d steps: state %.3g, drag %.3g, lift %.3g" % (ms, v, v / value64, tk["fused_ms"], tk["gradient_ms"], n,
-                                                               st, drag, lift))
     del s
-    return {"value": round(v, 3), "unit": "Mcell-stage/s", "ms_per_step": round(ms, 4), "dtype": "mixed",
-            "speedup_vs_fp64": round(v / value64, 4),
-            "kernels_ms": {"k_face_tile": round(tk["fused_ms"], 4), "k_gradient": round(tk["gradient_ms"], 4)},
-            "error_vs_fp64": {"steps": int(n), "max_rel_state": st, "drag_rel": drag, "lift_rel": lift},
-            "precision": "FP32 flux/reconstruction/gradient/limiter; FP64 geometry, residual sum, update, norms"}
+    log("mixed: %.3f ms/step, %.1f Mcell-stage/s (x%.3f of FP64); tile %.3f ms, gradient %.3f ms"
+        % (ms, v, v / value64, tk["fused_ms"], tk["gradient_ms"]))
+    out = {"value": round(v, 3), "unit": "Mcell-stage/s", "ms_per_step": round(ms, 4), "dtype": "mixed",
+           "speedup_vs_fp64": round(v / value64, 4),
+           "kernels_ms": {"tile": round(tk["fused_ms"], 4), "gradient": round(tk["gradient_ms"], 4)},
+           "precision": "FP32 flux/reconstruction/gradient/limiter; FP64 geometry, residual sum, update, norms"}
+    if args.c2_steps > 0:
+        n = args.c2_steps
+        runs = {}
+        for prec, cc in (("fp64", cfg), ("mixed", c)):
+            x = Solver(mesh, cc)
+            t0 = time.perf_counter()
+            h, t_end, _ = x.run(n)
+            runs[prec] = (h, x.q, time.perf_counter() - t0, t_end)
+            del x
+        (h64, q64, s64, t64), (h32, q32, s32, t32) = runs["fp64"], runs["mixed"]
+        assert len(h64) == n and len(h32) == n, "a full-length run stopped early"
+        mx, l2 = state_error(q64, q32)
+        drag = abs(h64[-1, 7] - h32[-1, 7]) / abs(h64[-1, 7])
+        lift = abs(h64[-1, 8] - h32[-1, 8]) / max(abs(h64[-1, 8]), abs(h64[-1, 7]))
+        drag_hist = float(np.max(np.abs(h64[:, 7] - h32[:, 7])) / np.max(np.abs(h64[:, 7])))
+        log("C2/C3 full runs: %d steps to t = %.4g, FP64 %.2f s, mixed %.2f s; mixed vs FP64: state %.3g (max), "
+            "%.3g (2-norm), drag %.3g, lift %.3g (of drag), drag over the history %.3g"
+            % (n, t64, s64, s32, mx, l2, drag, lift, drag_hist))
+        out["error_vs_fp64"] = {"steps": n, "t_end": t64, "max_rel_state": mx, "l2_rel_state": l2,
+                                "drag_rel": drag, "lift_rel_of_drag": lift, "drag_rel_max_over_history": drag_hist,
+                                "drag_fp64": h64[-1, 7], "lift_fp64": h64[-1, 8],
+                                "strouhal": "not reached: %d steps cover t = %.3g, a shedding period is ~30" % (n, t64)}
+        out["full_run_s"] = {"fp64": round(s64, 3), "mixed": round(s32, 3)}
+    return out
 
 
 def bump_request(rng, cells, M_range, seed, precision="mixed", **ext):
@@ -354,7 +383,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4,
                     help="oracle steps timed for the CPU baseline, about 13 s on one core (0: skip)")
     ap.add_argument("--time-steps", type=int, default=20, help="steps timed per kernel with HIP events")
-    ap.add_argument("--stage-path", default="faceTiles", choices=("split", "faceTiles", "fused"),
+    ap.add_argument("--stage-path", default="faceTiles", choices=("split", "faceTiles", "fused", "gradTiles"),
                     help="kernels of one stage: gradient/face/update, gradient + face/update tiles, or one fused kernel")
     ap.add_argument("--tile-cells", type=int, default=-1, help="cells per tile of the tiled paths (-1: automatic)")
     ap.add_argument("--tile-order", default="blocks", choices=("blocks", "grow", "morton"),
@@ -362,6 +391,10 @@ def main():
     ap.add_argument("--cell-order", default="reference", choices=("reference", "tiles"),
                     help="device numbering of the cells: the mesh's, or tile by tile")
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision form of the workload (0: skip)")
+    ap.add_argument("--c2-steps", type=int, default=5000,
+                    help="full-length FP64 and mixed runs of the workload compared for the mixed error (0: skip)")
+    ap.add_argument("--limiter", default="venkatakrishnan", choices=("venkatakrishnan", "scalar"))
+    ap.add_argument("--venkat-k", type=float, default=1.0)
     ap.add_argument("--c1-steps", type=int, default=1000,
                     help="steps of the 8,192-cell manufactured-solution latency leg (0: skip)")
     ap.add_argument("--dropin-stages", type=int, default=10,
@@ -405,7 +438,7 @@ def main():
     raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1, **kw)
     cfg = cylinder_config()
     cfg["extensions"] = {"stagePath": args.stage_path, "tileCells": args.tile_cells, "tileOrder": args.tile_order,
-                         "cellOrder": args.cell_order}
+                         "cellOrder": args.cell_order, "limiter": args.limiter, "venkatK": args.venkat_k}
     mesh = build_mesh(raw)
     N = mesh.n_interior
     log("mesh built: %d cells, %d edges" % (N, mesh.n_edges))
@@ -450,7 +483,7 @@ def main():
     dropin = dropin_leg(s, N, args.dropin_stages, stage_ms) if args.dropin_stages > 0 else None
     mixed = None
     if args.mixed:
-        mixed = mixed_leg(mesh, cfg, s, args, N, value, ms_per_step)
+        mixed = mixed_leg(mesh, cfg, args, N, value)
     c1 = None
     if args.c1_steps > 0:
         c1 = c1_leg(args.c1_steps)
@@ -487,7 +520,9 @@ def main():
         "config": {"workload": workload, "cells": N,
                    "stage_path": args.stage_path, "tile_cells": args.tile_cells, "tile_order": args.tile_order,
                    "cell_order": args.cell_order,
-                   "edges": F, "flux": "rusanovFlux", "limiter": "scalar (solve.d:360-403)",
+                   "edges": F, "flux": "rusanovFlux",
+                   "limiter": ("venkatakrishnan (K=%g)" % args.venkat_k) if args.limiter == "venkatakrishnan"
+                   else "scalar (solve.d:360-403)",
                    "integrator": "RK2_TVD", "stages_per_step": stages, "M": 0.2, "Re": 100, "CFL": cfg["CFL"]},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -46,14 +46,14 @@ struct BatchGroup {
     DBuf<uint2> dGrad, dTile;
 };
 
-using GroupKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int>;
+using GroupKey = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int>;
 
 GroupKey groupKey(const ebb_solver* s, const std::vector<StageDesc>& seq)
 {
     int lim[4] = {0, 0, 0, 0};
     for (size_t k = 0; k < seq.size() && k < 4; k++) lim[k] = seq[k].limit;
     return GroupKey(int(s->S), s->phys.flux, s->phys.viscous, s->phys.order, s->phys.limited, s->phys.mixed,
-                    int(seq.size()), lim[0], lim[1], lim[2], lim[3], s->phys.venkat);
+                    int(seq.size()), lim[0], lim[1], lim[2], lim[3], s->phys.venkat, s->useGradTiles ? 1 : 0);
 }
 
 template <typename T>
@@ -162,7 +162,7 @@ int rebuild(ebb_batch* b)
         if (it == gidx.end()) {
             BatchGroup g;
             g.shape = GroupShape{int(s->S), s->phys.flux, s->phys.viscous, s->phys.order, s->phys.limited,
-                                 s->phys.venkat, s->phys.mixed};
+                                 s->phys.venkat, s->phys.mixed, s->useGradTiles ? 1 : 0};
             g.nst = int(seq.size());
             for (size_t q = 0; q < seq.size() && q < 4; q++) g.limit[q] = seq[q].limit;
             it = gidx.emplace(key, b->groups.size()).first;
@@ -175,7 +175,9 @@ int rebuild(ebb_batch* b)
             g.grad.push_back(make_uint2(slot, FORCE_ITEM));
             b->force.push_back(slot);
         }
-        const uint32_t nblk = (s->N + 255) / 256;
+        // gradient items: cell blocks, or on gradient tiles the border blocks (at least one,
+        // whose block 0 resets the step's Rmax)
+        const uint32_t nblk = s->useGradTiles ? std::max<uint32_t>(1u, (s->ftiles.nBorder + 255) / 256) : (s->N + 255) / 256;
         for (uint32_t x = 0; x < nblk; x++) g.grad.push_back(make_uint2(slot, x));
         for (uint32_t x = 0; x < s->ftiles.nTiles; x++) g.tile.push_back(make_uint2(slot, x));
         for (uint32_t x = 0; x < b->hmem.back().nRmsFull + 3; x++) b->rms.push_back(make_uint2(slot, x));
@@ -401,11 +403,12 @@ int ebb_batch_add(ebb_batch* b, const ebb_mesh* m, const ebb_config* c, int64_t*
     return guarded([&]() -> int {
         if (!b || !m || !c || !id) return EBB_ERR_ARG;
         Config cfg = c->c;
-        // batched members run the face-tile stage path in the reference numbering with
-        // 16-cell-block tiles (the RMS leaf sums come from the last stage), no LP limiter
+        // batched members run a face-tile stage path (gradient tiles where asked for, else
+        // the gradient kernel and face tiles) in the reference numbering with 16-cell-block
+        // tiles (the RMS leaf sums come from the last stage), no LP limiter
         if (cfg.order > 1 && cfg.limited && cfg.lpThresh > 0) return EBB_ERR_ARG;
         if (cfg.stagePath == 2) return EBB_ERR_ARG;
-        cfg.stagePath = 1;
+        cfg.stagePath = cfg.stagePath == 3 ? 3 : 1;
         cfg.tileOrder = 2;
         cfg.renumber = false;
         if (b->tileCells != -1) cfg.tileCells = b->tileCells;

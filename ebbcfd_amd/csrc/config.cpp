@@ -237,6 +237,7 @@ bool parseConfig(const std::string& json, Config& c, std::string& err)
             if (e->s == "split") c.stagePath = 0;
             else if (e->s == "faceTiles") c.stagePath = 1;
             else if (e->s == "fused") c.stagePath = 2;
+            else if (e->s == "gradTiles") c.stagePath = 3;
             else { err = "unknown stagePath " + e->s; return false; }
         }
         if ((e = v->get("tileCells")) && e->kind == JVal::Int) {

@@ -219,6 +219,19 @@ struct FaceTileArgs {
     const uint4* fref;             // per face: cell a, cell b, node 0, node 1 (normal, midpoint and length are
                                    // recomputed from the nodes); boundary face b has the ghost N + b as cell b
     const uint32_t* fid;           // per face: global face id, read only to report a NaN flux
+    // Gradient tiles (the same tiles, gradients computed inside the tile kernel): each tile's
+    // own cells' gradient and limiter rows are computed into LDS slots 0..nC-1; the rows of
+    // its ring (the cells across its faces that other tiles own) are copied into slots
+    // nC..nC+nH-1 from what the border kernel computed.  Null on the face-tile path.
+    const uint32_t* fslot;         // per face: LDS row of cell a (low 16 bits) and of cell b (high 16 bits;
+                                   // a boundary face repeats cell a's)
+    const uint32_t* halo;          // per tile, from t * haloStride: the ring cells, in slot order
+    const uint32_t* nHalo;         // per tile: ring cells
+    const uint32_t* border;        // every cell in some tile's ring, ascending
+    const uint32_t* nbrT;          // S per own cell, laid out like ids: the cell's neighbour ids
+    uint32_t haloStride, nBorder;
+    uint32_t glDoubles;            // LDS offset, in doubles, of the face results after the gradient rows
+    uint32_t pad;
 };
 
 // Launchers (kernels.hip).  All run on `stream`.
@@ -226,6 +239,11 @@ void launchFaceTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, 
                      hipStream_t stream);
 void launchStageTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const TileArgs& t, int S,
                       hipStream_t stream);
+// gradient tiles: the border kernel (ring cells' gradients; the force and Rmax duties of the
+// gradient kernel) then the tile kernel (own gradients, faces and update); parts 1: the border
+// kernel only, 2: the tile kernel (and the stage reduction) only
+void launchGradTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, int S,
+                     hipStream_t stream, int parts = 3);
 void launchGradient(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t stream);
 void launchFaces(const DevMesh& m, const PhysParams& p, const StageArgs& a, hipStream_t stream);
 void launchUpdate(const DevMesh& m, const PhysParams& p, const StageArgs& a, int S, hipStream_t stream);
@@ -287,6 +305,7 @@ void launchInitScalars(DevScalars* sc, double dt, double t, double CFL, hipStrea
 // (mesh record, local index).  parity selects each mesh's state buffers (see MemberArgs).
 struct GroupShape {
     int S, flux, viscous, order, limited, venkat, mixed;
+    int gradTiles;   // 1: gradient items are border blocks and tiles compute their own gradients
 };
 void launchGradientBatch(const GroupShape& g, int limitStage, const MemberArgs* mem, const uint2* items,
                          uint32_t nItems, uint32_t parity, uint32_t stage, hipStream_t stream);

@@ -83,7 +83,7 @@ struct Config {
     int limiter = 0;                  // 0: the reference's scalar limiter; 1: Venkatakrishnan
     double venkatK = 1.0;             // Venkatakrishnan constant K (eps^2 = (K d)^3)
     int stagePath = 1;                // 0: gradient, face, update kernels; 1: gradient + face/update tiles;
-                                      // 2: one fused kernel per stage
+                                      // 2: one fused kernel per stage; 3: border gradients + gradient tiles
     long long tileCells = -1;         // cells per tile of the tiled paths (-1: automatic)
     int tileOrder = 2;                // 0: greedily grown regions, 1: Morton order of centroids,
                                       // 2: Morton order of aligned 16-cell blocks

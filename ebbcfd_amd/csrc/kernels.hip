@@ -592,12 +592,80 @@ struct GradLds {
     double sl[BLK][LWS + 1];
 };
 
+template <int MIX>
+using RealOf = typename std::conditional<MIX == 1, float, double>::type;
+
+// a stage-input row: the FP32 shadow on the mixed path
+template <int MIX>
+DEV void ldStage(const StageArgs& a, uint32_t c, RealOf<MIX> o[4])
+{
+    if constexpr (MIX == 1) ld4(a.qin32, c, o);
+    else ld4(a.qin, c, o);
+}
+// a ghost row of the stage input; the mixed path also widens it into the FP64 row
+template <int MIX>
+DEV void stGhostRow(const StageArgs& a, uint32_t c, const RealOf<MIX> o[4])
+{
+    if constexpr (MIX == 1) {
+        st4(a.qin32, c, o);
+        a.qin[c] = make_double4(o[0], o[1], o[2], o[3]);
+    } else {
+        st4(a.qin, c, o);
+    }
+}
+
+// Gradient and limiter of interior cell i (MODE 1-4) from the stage input, with the ghost
+// states of its boundary faces, which are stored to the stage buffer where storeGhosts;
+// nbrRow: its S neighbour ids (the mesh's row, or a tile's copy of it).
+// Every neighbour row is loaded before any ghost is written, so the loads of all
+// neighbours are in flight together.  A ghost slot belongs to this cell alone (the owner of
+// its boundary face), so its row still holds the stage buffer's previous ghost state, which
+// TempPresInflow keeps in the components it does not write.
+template <int S, int MODE, int MIX>
+DEV void cellGradient(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t i, const uint32_t* nbrRow,
+                      bool storeGhosts, RealOf<MIX> g[4][2], RealOf<MIX> lx[4], RealOf<MIX> ly[4])
+{
+    using R = RealOf<MIX>;
+    R qi[4];
+    ldStage<MIX>(a, i, qi);
+    uint32_t nb[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) nb[j] = nbrRow[j];
+    R qn[S][4];
+    double2 cn[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) { ldStage<MIX>(a, nb[j], qn[j]); cn[j] = m.cen[nb[j]]; }
+    const double2 ci = m.cen[i];
+#pragma unroll
+    for (int j = 0; j < S; j++)
+        if (nb[j] >= m.N) {
+            const BFace& bf = m.bface[nb[j] - m.N];
+            R old[4] = {0, 0, 0, 0};
+            if (bf.type == BT_TPIN) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) old[k] = qn[j][k];
+            }
+            if (!ghostState<R>(bf, qi, qn[j], old) && a.recordErrors)
+                atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
+            if (storeGhosts) stGhostRow<MIX>(a, nb[j], qn[j]);
+        }
+    bool real[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) real[j] = nb[j] != i;
+    double eps2 = 0.0;
+    if constexpr (MODE == 4) {
+        const double kd = p.venkatK * m.cd[i];
+        eps2 = kd * kd * kd;
+    }
+    gradLimit<S, MODE, R>(qi, qn, nullptr, 0, ci, cn, real, a.limit, p.lpThresh, g, lx, ly, eps2);
+}
+
 // One workgroup of the gradient kernel on cell block `bid` of its mesh (the solo and the
 // batched kernels run this same code).
 template <int S, int MODE, int MIX>
 DEV void gradientBody(const DevMesh& m, const PhysParams& p, const StageArgs& a, uint32_t bid, GradLds<MODE>& L)
 {
-    using R = typename std::conditional<MIX == 1, float, double>::type;
+    using R = RealOf<MIX>;
     constexpr int LWS = GradLds<MODE>::LWS;
     auto& sg = L.sg;
     auto& sl = L.sl;
@@ -607,70 +675,26 @@ DEV void gradientBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
     if (MODE == 0 && i >= m.N) return;
     const bool have = i < m.N;   // no early return below: the staged stores need every thread
     R g[4][2], lx[4], ly[4];
-    auto ldState = [&](uint32_t c, R o[4]) {
-        if constexpr (MIX == 1) ld4(a.qin32, c, o);
-        else ld4(a.qin, c, o);
-    };
-    auto stGhost = [&](uint32_t c, const R o[4]) {
-        if constexpr (MIX == 1) {
-            st4(a.qin32, c, o);
-            a.qin[c] = make_double4(o[0], o[1], o[2], o[3]);
-        } else {
-            st4(a.qin, c, o);
-        }
-    };
     if (have) {
-        R qi[4];
-        ldState(i, qi);
-        uint32_t nb[S];
-#pragma unroll
-        for (int j = 0; j < S; j++) nb[j] = m.nbr[i * S + j];
         if constexpr (MODE == 0) {
             // order 1: the ghost states are the kernel's only output
+            R qi[4];
+            ldStage<MIX>(a, i, qi);
 #pragma unroll
-            for (int j = 0; j < S; j++)
-                if (nb[j] >= m.N) {
-                    const BFace& bf = m.bface[nb[j] - m.N];
+            for (int j = 0; j < S; j++) {
+                const uint32_t nb = m.nbr[i * S + j];
+                if (nb >= m.N) {
+                    const BFace& bf = m.bface[nb - m.N];
                     R qg[4], old[4] = {0, 0, 0, 0};
-                    if (bf.type == BT_TPIN) ldState(nb[j], old);
+                    if (bf.type == BT_TPIN) ldStage<MIX>(a, nb, old);
                     if (!ghostState<R>(bf, qi, qg, old) && a.recordErrors)
-                        atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
-                    stGhost(nb[j], qg);
+                        atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb - m.N));
+                    stGhostRow<MIX>(a, nb, qg);
                 }
+            }
             return;
         } else {
-            // Every neighbour row is loaded before any ghost is written, so the loads of
-            // all neighbours are in flight together.  A ghost slot belongs to this cell
-            // alone (the owner of its boundary face), so its row still holds the stage
-            // buffer's previous ghost state, which TempPresInflow keeps in the components
-            // it does not write.
-            R qn[S][4];
-            double2 cn[S];
-#pragma unroll
-            for (int j = 0; j < S; j++) { ldState(nb[j], qn[j]); cn[j] = m.cen[nb[j]]; }
-            const double2 ci = m.cen[i];
-#pragma unroll
-            for (int j = 0; j < S; j++)
-                if (nb[j] >= m.N) {
-                    const BFace& bf = m.bface[nb[j] - m.N];
-                    R old[4] = {0, 0, 0, 0};
-                    if (bf.type == BT_TPIN) {
-#pragma unroll
-                        for (int k = 0; k < 4; k++) old[k] = qn[j][k];
-                    }
-                    if (!ghostState<R>(bf, qi, qn[j], old) && a.recordErrors)
-                        atomicMin(&a.sc->errBC, (uint64_t(a.stageId) << 32) | (nb[j] - m.N));
-                    stGhost(nb[j], qn[j]);
-                }
-            bool real[S];
-#pragma unroll
-            for (int j = 0; j < S; j++) real[j] = nb[j] != i;
-            double eps2 = 0.0;
-            if constexpr (MODE == 4) {
-                const double kd = p.venkatK * m.cd[i];
-                eps2 = kd * kd * kd;
-            }
-            gradLimit<S, MODE, R>(qi, qn, nullptr, 0, ci, cn, real, a.limit, p.lpThresh, g, lx, ly, eps2);
+            cellGradient<S, MODE, MIX>(m, p, a, i, m.nbr + size_t(i) * S, true, g, lx, ly);
         }
     }
     if (MODE == 0) return;
@@ -1488,6 +1512,88 @@ DEV void edgeGeometry(bool interior, double2 p0, double2 p1, double2& n, double2
     mid = make_double2(0.5 * (p0.x + p1.x), 0.5 * (p0.y + p1.y));
 }
 
+// The cell phase of a face tile: each own cell sums its faces' results from LDS in its own
+// face order and applies the stage update; the workgroup's (min dt, max |R|) pair goes to
+// partial slot `pslot`.  giNext/giNext2: the device ids of the first two cell rounds.
+template <int S>
+DEV void tileCellPhase(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, const uint4& d,
+                       uint32_t pslot, const double* fsd, uint32_t giNext, uint32_t giNext2)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nC = d.x;
+    // each cell round's per-cell inputs are loaded one round ahead (the first round's
+    // before the barrier, which waits for LDS only), from ids loaded two rounds ahead
+    // (a load issued after the next round's would wait for it: loads complete in order)
+    double metN[3] = {0, 0, 0}, qnN[4] = {0, 0, 0, 0}, qiN[4] = {0, 0, 0, 0}, acN[4] = {0, 0, 0, 0};
+    uint32_t refN = 0;
+    const bool wantQn = a.kind != SK_RESIDUAL, wantQi = a.kind == SK_TVD_2;
+    const bool wantAcc = a.kind == SK_RK2_2 || a.kind == SK_RK4_2 || a.kind == SK_RK4_3 || a.kind == SK_RK4_4;
+    const bool wantRef = a.rmsBlocks != nullptr;
+    uint32_t cfN[S] = {};
+    auto loadCellIn = [&](uint32_t c) {
+        metN[0] = m.cper[c]; metN[1] = m.cd[c]; metN[2] = m.nia[c];
+        if (wantQn) ld4(a.qn, c, qnN);
+        if (wantQi) ld4(a.qin, c, qiN);
+        if (wantAcc) ld4(a.acc, c, acN);
+        if (wantRef) refN = m.refId[c];
+    };
+    auto loadCf = [&](uint32_t k) {
+#pragma unroll
+        for (int j = 0; j < S; j++) cfN[j] = t.cf[size_t(d.z + k) * S + j];
+    };
+    if (tid < nC) { loadCellIn(giNext); loadCf(tid); }
+    ldsBarrier();
+    double dtCand = INFINITY, rCand = 0.0;
+    // uniform trip count: the RMS leaf sums need every lane of a 16-lane group
+    for (uint32_t k0 = 0; k0 < nC; k0 += BLK) {
+        const uint32_t k = k0 + tid;
+        const bool have = k < nC;
+        double qNew[4] = {0, 0, 0, 0}, qOld[4] = {0, 0, 0, 0};
+        uint32_t gi = giNext;
+        double met[3], qnC[4], qiC[4], acC[4];
+#pragma unroll
+        for (int x = 0; x < 3; x++) met[x] = metN[x];
+#pragma unroll
+        for (int x = 0; x < 4; x++) { qnC[x] = qnN[x]; qiC[x] = qiN[x]; acC[x] = acN[x]; }
+        const uint32_t refC = refN;
+        uint32_t cfC[S];
+#pragma unroll
+        for (int j = 0; j < S; j++) cfC[j] = cfN[j];
+        if (k + BLK < nC) {
+            giNext = giNext2;
+            if (k + 2 * BLK < nC) giNext2 = t.ids[d.z + k + 2 * BLK];
+            loadCellIn(giNext);
+            loadCf(k + BLK);
+        }
+        if (have) {
+            double r[4] = {0.0, 0.0, 0.0, 0.0};
+            double sAve = 0;
+#pragma unroll
+            for (int j = 0; j < S; j++) {
+                const uint32_t fe = cfC[j];
+                const double* lf = fsd + 5 * (fe & 0x7fffu);
+                if (fe >> 15) { r[0] = r[0] - lf[0]; r[1] = r[1] - lf[1]; r[2] = r[2] - lf[2]; r[3] = r[3] - lf[3]; }
+                else { r[0] = r[0] + lf[0]; r[1] = r[1] + lf[1]; r[2] = r[2] + lf[2]; r[3] = r[3] + lf[3]; }
+                sAve += lf[4];
+            }
+            switch (a.kind) {
+            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
+            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
+            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
+            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
+            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
+            default: cellStage<SK_RESIDUAL>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
+            }
+        }
+        if (a.rmsBlocks) rmsBlockLeaf(have, qNew, qOld, have ? refC : 0u, a.rmsBlocks);
+    }
+    blockPartial(dtCand, rCand, a.partial, pslot);
+}
+
 // One workgroup of the face/update kernel on tile `tile` of its mesh, leaving its (min dt,
 // max |R|) pair in partial slot `pslot` (the solo and the batched kernels run this same code).
 template <int S, int FLUX, int VISC, int ORDER, int LP, int MIX>
@@ -1560,78 +1666,8 @@ DEV void faceTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
         slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
         slot[4] = len * sMax;
     }
-    // each cell round's per-cell inputs are loaded one round ahead (the first round's
-    // before the barrier, which waits for LDS only), from ids loaded two rounds ahead
-    // (a load issued after the next round's would wait for it: loads complete in order)
-    double metN[3] = {0, 0, 0}, qnN[4] = {0, 0, 0, 0}, qiN[4] = {0, 0, 0, 0}, acN[4] = {0, 0, 0, 0};
-    uint32_t refN = 0;
-    const bool wantQn = a.kind != SK_RESIDUAL, wantQi = a.kind == SK_TVD_2;
-    const bool wantAcc = a.kind == SK_RK2_2 || a.kind == SK_RK4_2 || a.kind == SK_RK4_3 || a.kind == SK_RK4_4;
-    const bool wantRef = a.rmsBlocks != nullptr;
-    uint32_t cfN[S] = {};
-    auto loadCellIn = [&](uint32_t c) {
-        metN[0] = m.cper[c]; metN[1] = m.cd[c]; metN[2] = m.nia[c];
-        if (wantQn) ld4(a.qn, c, qnN);
-        if (wantQi) ld4(a.qin, c, qiN);
-        if (wantAcc) ld4(a.acc, c, acN);
-        if (wantRef) refN = m.refId[c];
-    };
-    auto loadCf = [&](uint32_t k) {
-#pragma unroll
-        for (int j = 0; j < S; j++) cfN[j] = t.cf[size_t(d.z + k) * S + j];
-    };
-    if (tid < nC) { loadCellIn(giNext); loadCf(tid); }
     if (tid < 5) fsd[5 * nF + tid] = 0.0;   // padding face
-    ldsBarrier();
-    double dtCand = INFINITY, rCand = 0.0;
-    // uniform trip count: the RMS leaf sums need every lane of a 16-lane group
-    for (uint32_t k0 = 0; k0 < nC; k0 += BLK) {
-        const uint32_t k = k0 + tid;
-        const bool have = k < nC;
-        double qNew[4] = {0, 0, 0, 0}, qOld[4] = {0, 0, 0, 0};
-        uint32_t gi = giNext;
-        double met[3], qnC[4], qiC[4], acC[4];
-#pragma unroll
-        for (int x = 0; x < 3; x++) met[x] = metN[x];
-#pragma unroll
-        for (int x = 0; x < 4; x++) { qnC[x] = qnN[x]; qiC[x] = qiN[x]; acC[x] = acN[x]; }
-        const uint32_t refC = refN;
-        uint32_t cfC[S];
-#pragma unroll
-        for (int j = 0; j < S; j++) cfC[j] = cfN[j];
-        if (k + BLK < nC) {
-            giNext = giNext2;
-            if (k + 2 * BLK < nC) giNext2 = t.ids[d.z + k + 2 * BLK];
-            loadCellIn(giNext);
-            loadCf(k + BLK);
-        }
-        if (have) {
-            double r[4] = {0.0, 0.0, 0.0, 0.0};
-            double sAve = 0;
-#pragma unroll
-            for (int j = 0; j < S; j++) {
-                const uint32_t fe = cfC[j];
-                const double* lf = fsd + 5 * (fe & 0x7fffu);
-                if (fe >> 15) { r[0] = r[0] - lf[0]; r[1] = r[1] - lf[1]; r[2] = r[2] - lf[2]; r[3] = r[3] - lf[3]; }
-                else { r[0] = r[0] + lf[0]; r[1] = r[1] + lf[1]; r[2] = r[2] + lf[2]; r[3] = r[3] + lf[3]; }
-                sAve += lf[4];
-            }
-            switch (a.kind) {
-            case SK_EULER: cellStage<SK_EULER>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
-            case SK_RK2_1: cellStage<SK_RK2_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
-            case SK_RK2_2: cellStage<SK_RK2_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
-            case SK_TVD_1: cellStage<SK_TVD_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
-            case SK_TVD_2: cellStage<SK_TVD_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
-            case SK_RK4_1: cellStage<SK_RK4_1>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
-            case SK_RK4_2: cellStage<SK_RK4_2>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
-            case SK_RK4_3: cellStage<SK_RK4_3>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC, acC); break;
-            case SK_RK4_4: cellStage<SK_RK4_4>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, qNew, qOld, qnC, acC); break;
-            default: cellStage<SK_RESIDUAL>(m, p, a, gi, r, sAve, qiC, met, dtCand, rCand, nullptr, nullptr, qnC); break;
-            }
-        }
-        if (a.rmsBlocks) rmsBlockLeaf(have, qNew, qOld, have ? refC : 0u, a.rmsBlocks);
-    }
-    blockPartial(dtCand, rCand, a.partial, pslot);
+    tileCellPhase<S>(m, p, a, t, d, pslot, fsd, giNext, giNext2);
 }
 
 template <int S, int FLUX, int VISC, int ORDER, int LP, int MIX>
@@ -1678,6 +1714,232 @@ void launchFaceTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, 
     else if (S == 4) faceTileS<4>(m, p, a, t, st);
     else faceTileS<6>(m, p, a, t, st);
     if (!a.deferReduce) hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nTiles, a.sc, a.checkActive);
+}
+
+// ---------------------------------------------------------------- gradient tiles
+// A stage in two launches that keeps most gradient and limiter rows on chip.
+//   k_border_grad  per ring cell (a cell across a face of some other tile): its gradient
+//                  and limiter rows to HBM; it also carries the gradient kernel's other
+//                  duties (the previous step's boundary force, the per-step Rmax reset);
+//   k_grad_tile    per tile: its own cells' ghosts, gradients and limiters into LDS, the
+//                  ring cells' rows copied in from the border kernel's output, then every
+//                  face of the tile and the cells' update as in k_face_tile.
+// Gradients come from cellGradient and faces from interiorFace/boundaryFace, the code of
+// the split kernels, on the same inputs, so the stage's results are bit-identical to the
+// other paths.  A boundary face takes its ghost state from its owner's row (ghostState)
+// instead of reading back the row this launch's gradient phase wrote.
+
+// LDS row of one cell: gradient (x,y per variable) then the 4 scalar limiters
+constexpr int GROW = 12;
+
+// One workgroup of the border kernel on border block `bid` of its mesh (the solo and the
+// batched kernels run this same code); block 0 also resets the step's Rmax.
+template <int S, int MODE, int MIX>
+DEV void borderGradBody(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, uint32_t bid)
+{
+    using R = RealOf<MIX>;
+    if (a.checkActive && !a.sc->active) return;
+    if (bid == 0 && threadIdx.x == 0 && a.first) a.sc->rmaxBits = 0;   // Rmax is per step
+    const uint32_t k = bid * BLK + threadIdx.x;
+    if (k >= t.nBorder) return;
+    const uint32_t i = t.border[k];
+    R g[4][2], lx[4], ly[4];
+    cellGradient<S, MODE, MIX>(m, p, a, i, m.nbr + size_t(i) * S, false, g, lx, ly);   // the owning tile stores the ghosts
+    if constexpr (MIX == 1) {
+        float4* gd = reinterpret_cast<float4*>(a.grad32 + size_t(i) * 8);
+        gd[0] = make_float4(g[0][0], g[0][1], g[1][0], g[1][1]);
+        gd[1] = make_float4(g[2][0], g[2][1], g[3][0], g[3][1]);
+        if (MODE != 1) *reinterpret_cast<float4*>(a.lim32 + size_t(i) * 4) = make_float4(lx[0], lx[1], lx[2], lx[3]);
+    } else {
+        double4* gd = reinterpret_cast<double4*>(a.grad + size_t(i) * 8);
+        gd[0] = make_double4(g[0][0], g[0][1], g[1][0], g[1][1]);
+        gd[1] = make_double4(g[2][0], g[2][1], g[3][0], g[3][1]);
+        if (MODE != 1) *reinterpret_cast<double4*>(a.lim + size_t(i) * 4) = make_double4(lx[0], lx[1], lx[2], lx[3]);
+    }
+}
+
+// Workgroup 0 sums a pending boundary force when there is one (as the gradient kernel's
+// force workgroup); the others are border blocks.
+template <int S, int MODE, int MIX>
+__global__ __launch_bounds__(BLK) void k_border_grad(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
+{
+    __shared__ double2 fstage[FORCE_LDS];
+    const uint32_t fw = a.force.n ? 1u : 0u;
+    if (fw && blockIdx.x == 0) {
+        pendingForce<BLK, FORCE_LDS>(a.force, a.sc, fstage);
+        return;
+    }
+    borderGradBody<S, MODE, MIX>(m, p, a, t, xcdBlock(blockIdx.x - fw, gridDim.x - fw));
+}
+
+// One workgroup of the gradient-tile kernel on tile `tile`.  MODE 1: an unlimited stage
+// (no limiter rows; the faces take unit limiters); 2: the scalar limiter; 4: Venkatakrishnan.
+template <int S, int FLUX, int VISC, int MODE, int MIX>
+DEV void gradTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t,
+                      uint32_t tile, uint32_t pslot, double* lds)
+{
+    using R = RealOf<MIX>;
+    constexpr int LP = MODE == 1 ? 2 : 0;
+    const FaceSrc<MIX> src{a};
+    const uint32_t tid = threadIdx.x;
+    // first loads beside the descriptor's (fixed strides; lanes past the counts read padding)
+    const uint32_t gi0 = t.ids[size_t(tile) * t.cellStride + tid];
+    uint4 rNext = t.fref[size_t(tile) * t.faceStride + tid];
+    uint32_t sNext = t.fslot[size_t(tile) * t.faceStride + tid];
+    const uint4 d = t.desc[tile];
+    const uint32_t nH = t.nHalo[tile];
+    if (a.checkActive && !a.sc->active) return;
+    const uint32_t nC = d.x, nF = d.y;
+    R* gl = reinterpret_cast<R*>(lds);
+    double* fsd = lds + t.glDoubles;
+    // the ring's rows first: their loads are in flight while the own gradients compute
+    for (uint32_t h = tid; h < nH; h += BLK) {
+        const uint32_t y = t.halo[size_t(tile) * t.haloStride + h];
+        R gg[8];
+        src.grad(y, gg);
+        R* row = gl + GROW * (nC + h);
+#pragma unroll
+        for (int c = 0; c < 8; c++) row[c] = gg[c];
+        if (MODE != 1) {
+            R lx[4], ly[4];
+            src.template lim<0>(y, lx, ly);
+#pragma unroll
+            for (int c = 0; c < 4; c++) row[8 + c] = lx[c];
+        }
+    }
+    // own cells (ghosts stored: this tile owns them), neighbour ids from the tile's copy of
+    // their rows, whose addresses do not wait for the cell ids
+    for (uint32_t k = tid; k < nC; k += BLK) {
+        const uint32_t i = k == tid ? gi0 : t.ids[d.z + k];
+        R g[4][2], lx[4], ly[4];
+        cellGradient<S, MODE, MIX>(m, p, a, i, t.nbrT + size_t(d.z + k) * S, true, g, lx, ly);
+        R* row = gl + GROW * k;
+#pragma unroll
+        for (int c = 0; c < 4; c++) { row[2 * c] = g[c][0]; row[2 * c + 1] = g[c][1]; }
+        if (MODE != 1) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) row[8 + c] = lx[c];
+        }
+    }
+    // the first cell rounds' ids for the cell phase
+    const uint32_t giNext = gi0;
+    const uint32_t giNext2 = t.ids[size_t(tile) * t.cellStride + tid + BLK];
+    ldsBarrier();
+    // face phase: as k_face_tile, with the rows from LDS
+    for (uint32_t k = tid; k < nF; k += BLK) {
+        const uint4 r = rNext;   // cell a, cell b, node 0, node 1
+        const uint32_t sl = sNext;
+        if (k + BLK < nF) { rNext = t.fref[d.w + k + BLK]; sNext = t.fslot[d.w + k + BLK]; }
+        const bool interior = r.y < m.N;
+        const double2 p0 = m.node[r.z], p1 = m.node[r.w];
+        R qa[4], qb[4];
+        src.q(r.x, qa);
+        src.q(r.y, qb);   // a boundary face: the ghost row, whose TempPresInflow components carry over
+        const double2 ca = m.cen[r.x], cb = m.cen[r.y];
+        const R* ra = gl + GROW * (sl & 0xffffu);
+        const R* rb = gl + GROW * (sl >> 16);
+        R ga[8], gb[8], lxa[4], lya[4], lxb[4], lyb[4];
+#pragma unroll
+        for (int c = 0; c < 8; c++) { ga[c] = ra[c]; gb[c] = rb[c]; }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            lxa[c] = lya[c] = MODE == 1 ? R(1.0) : ra[8 + c];
+            lxb[c] = lyb[c] = MODE == 1 ? R(1.0) : rb[8 + c];
+        }
+        double2 n, mid;
+        double len;
+        edgeGeometry(interior, p0, p1, n, mid, len);
+        R qL[4], qR[4], Fr[4], sMr;
+        bool isB = false;
+        uint32_t b = 0;
+        if (interior) {
+            interiorFace<FLUX, VISC, 2, LP, R>(qa, ga, lxa, lya, ca, qb, gb, lxb, lyb, cb, n, mid, p, qL, qR, Fr, sMr);
+        } else {
+            isB = true;
+            b = r.y - m.N;
+            const BFace& bf = m.bface[b];
+            R qg[4];
+            ghostState<R>(bf, qa, qg, qb);   // a failure was recorded by the gradient phase
+            boundaryFace<FLUX, VISC, 2, LP, R>(qa, ga, lxa, lya, ca, qg, bf, n, mid, p, qL, qR, Fr, sMr);
+        }
+        const double F[4] = {double(Fr[0]), double(Fr[1]), double(Fr[2]), double(Fr[3])}, sMax = double(sMr);
+        if (isB) {
+            const uint32_t f = m.Fi + b;
+            *reinterpret_cast<double4*>(a.lenF + size_t(f) * 4) =
+                make_double4(len * F[0], len * F[1], len * F[2], len * F[3]);
+            a.lenS[f] = len * sMax;
+        }
+        if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
+            atomicMin(&a.sc->errEdge,
+                      (uint64_t(a.stageId) << 32) | (isB ? uint64_t(b) : uint64_t(m.Fb) + t.fid[d.w + k]));
+        double* slot = fsd + 5 * k;
+        slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
+        slot[4] = len * sMax;
+    }
+    if (tid < 5) fsd[5 * nF + tid] = 0.0;   // padding face
+    tileCellPhase<S>(m, p, a, t, d, pslot, fsd, giNext, giNext2);
+}
+
+// FP64: at most 168 VGPRs, three waves per SIMD (the LDS of three workgroups fits a CU)
+template <int S, int FLUX, int VISC, int MODE, int MIX>
+__global__ __launch_bounds__(BLK, 3) void k_grad_tile(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
+{
+    extern __shared__ double glds[];
+    gradTileBody<S, FLUX, VISC, MODE, MIX>(m, p, a, t, xcdBlock(blockIdx.x, gridDim.x), blockIdx.x, glds);
+}
+
+template <int S, int FLUX, int VISC, int MIX>
+static void gradTileM(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, int mode,
+                      int parts, hipStream_t st)
+{
+    const dim3 g(t.nTiles), b(BLK);
+    const uint32_t nb = (t.nBorder + BLK - 1) / BLK;
+    const dim3 gb((nb ? nb : 1u) + (a.force.n ? 1u : 0u));   // at least one workgroup: the Rmax reset
+    const bool border = parts & 1, tiles = parts & 2;
+    switch (mode) {
+    case 1:
+        if (border) hipLaunchKernelGGL((k_border_grad<S, 1, MIX>), gb, b, 0, st, m, p, a, t);
+        if (tiles) hipLaunchKernelGGL((k_grad_tile<S, FLUX, VISC, 1, MIX>), g, b, t.ldsBytes, st, m, p, a, t);
+        break;
+    case 4:
+        if (border) hipLaunchKernelGGL((k_border_grad<S, 4, MIX>), gb, b, 0, st, m, p, a, t);
+        if (tiles) hipLaunchKernelGGL((k_grad_tile<S, FLUX, VISC, 4, MIX>), g, b, t.ldsBytes, st, m, p, a, t);
+        break;
+    default:
+        if (border) hipLaunchKernelGGL((k_border_grad<S, 2, MIX>), gb, b, 0, st, m, p, a, t);
+        if (tiles) hipLaunchKernelGGL((k_grad_tile<S, FLUX, VISC, 2, MIX>), g, b, t.ldsBytes, st, m, p, a, t);
+        break;
+    }
+}
+
+template <int S>
+static void gradTileS(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, int mode,
+                      int parts, hipStream_t st)
+{
+#define EBB_GT(F, V)                                                      \
+    do {                                                                  \
+        if (p.mixed) gradTileM<S, F, V, 1>(m, p, a, t, mode, parts, st);  \
+        else gradTileM<S, F, V, 0>(m, p, a, t, mode, parts, st);          \
+    } while (0)
+    if (p.flux == 1) {
+        if (p.viscous) EBB_GT(1, 1); else EBB_GT(1, 0);
+    } else {
+        if (p.viscous) EBB_GT(0, 1); else EBB_GT(0, 0);
+    }
+#undef EBB_GT
+}
+
+void launchGradTiles(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, int S,
+                     hipStream_t st, int parts)
+{
+    if (t.nTiles == 0) return;
+    // order 2 and no LP vector limiter (the set-up keeps other configurations on face tiles)
+    const int mode = !(p.limited && a.limit) ? 1 : (p.venkat ? 4 : 2);
+    if (S == 3) gradTileS<3>(m, p, a, t, mode, parts, st);
+    else if (S == 4) gradTileS<4>(m, p, a, t, mode, parts, st);
+    else gradTileS<6>(m, p, a, t, mode, parts, st);
+    if ((parts & 2) && !a.deferReduce)
+        hipLaunchKernelGGL(k_reduce_stage, dim3(1), dim3(1024), 0, st, a.partial, t.nTiles, a.sc, a.checkActive);
 }
 
 template <int S, int FLUX, int VISC, int LP>
@@ -2336,6 +2598,31 @@ __global__ __launch_bounds__(BLK) void k_face_tile_batch(const MemberArgs* __res
     faceTileBody<S, FLUX, VISC, ORDER, LP, MIX>(M.m, M.p, M.st[parity ^ M.parityFlip][stage], M.t, it.y, it.y, fsd);
 }
 
+template <int S, int MODE, int MIX>
+__global__ __launch_bounds__(BLK) void k_border_grad_batch(const MemberArgs* __restrict__ mem,
+                                                           const uint2* __restrict__ items, uint32_t parity, uint32_t stage)
+{
+    __shared__ double2 fstage[FORCE_LDS];
+    const uint2 it = items[xcdBlock(blockIdx.x, gridDim.x)];
+    const MemberArgs& M = mem[it.x];
+    const StageArgs& a = M.st[parity ^ M.parityFlip][stage];
+    if (it.y == FORCE_ITEM) {
+        pendingForce<BLK, FORCE_LDS>(a.force, a.sc, fstage);
+        return;
+    }
+    borderGradBody<S, MODE, MIX>(M.m, M.p, a, M.t, it.y);
+}
+
+template <int S, int FLUX, int VISC, int MODE, int MIX>
+__global__ __launch_bounds__(BLK, 3) void k_grad_tile_batch(const MemberArgs* __restrict__ mem,
+                                                         const uint2* __restrict__ items, uint32_t parity, uint32_t stage)
+{
+    extern __shared__ double glds[];
+    const uint2 it = items[xcdBlock(blockIdx.x, gridDim.x)];
+    const MemberArgs& M = mem[it.x];
+    gradTileBody<S, FLUX, VISC, MODE, MIX>(M.m, M.p, M.st[parity ^ M.parityFlip][stage], M.t, it.y, it.y, glds);
+}
+
 __global__ __launch_bounds__(RFB) void k_rms_batch(const MemberArgs* __restrict__ mem, const uint2* __restrict__ items,
                                                    uint32_t parity)
 {
@@ -2387,10 +2674,35 @@ static void gradBatchS(int mode, const MemberArgs* mem, const uint2* items, uint
     }
 }
 
+template <int S, int MIX>
+static void borderBatchS(int mode, const MemberArgs* mem, const uint2* items, uint32_t n, uint32_t par, uint32_t st,
+                         hipStream_t s)
+{
+    const dim3 g(n), b(BLK);
+    switch (mode) {
+    case 1: hipLaunchKernelGGL((k_border_grad_batch<S, 1, MIX>), g, b, 0, s, mem, items, par, st); break;
+    case 4: hipLaunchKernelGGL((k_border_grad_batch<S, 4, MIX>), g, b, 0, s, mem, items, par, st); break;
+    default: hipLaunchKernelGGL((k_border_grad_batch<S, 2, MIX>), g, b, 0, s, mem, items, par, st); break;
+    }
+}
+
 void launchGradientBatch(const GroupShape& gs, int limitStage, const MemberArgs* mem, const uint2* items,
                          uint32_t nItems, uint32_t parity, uint32_t stage, hipStream_t st)
 {
     if (nItems == 0) return;
+    if (gs.gradTiles) {   // border blocks; order 2, no LP vector limiter
+        const int mode = !(gs.limited && limitStage) ? 1 : (gs.venkat ? 4 : 2);
+        if (gs.mixed) {
+            if (gs.S == 3) borderBatchS<3, 1>(mode, mem, items, nItems, parity, stage, st);
+            else if (gs.S == 4) borderBatchS<4, 1>(mode, mem, items, nItems, parity, stage, st);
+            else borderBatchS<6, 1>(mode, mem, items, nItems, parity, stage, st);
+        } else {
+            if (gs.S == 3) borderBatchS<3, 0>(mode, mem, items, nItems, parity, stage, st);
+            else if (gs.S == 4) borderBatchS<4, 0>(mode, mem, items, nItems, parity, stage, st);
+            else borderBatchS<6, 0>(mode, mem, items, nItems, parity, stage, st);
+        }
+        return;
+    }
     // batched groups run without the LP vector limiter (mode 3)
     const int mode = gs.order == 1 ? 0 : (!(gs.limited && limitStage) ? 1 : (gs.venkat ? 4 : 2));
     if (gs.mixed) {
@@ -2408,6 +2720,14 @@ template <int S, int FLUX, int VISC, int MIX>
 static void faceBatchO(const GroupShape& gs, int limitStage, const MemberArgs* mem, const uint2* items, uint32_t n,
                        uint32_t lds, uint32_t par, uint32_t stg, hipStream_t s)
 {
+    if (gs.gradTiles) {
+        const dim3 g(n), b(BLK);
+        const int mode = !(gs.limited && limitStage) ? 1 : (gs.venkat ? 4 : 2);
+        if (mode == 1) hipLaunchKernelGGL((k_grad_tile_batch<S, FLUX, VISC, 1, MIX>), g, b, lds, s, mem, items, par, stg);
+        else if (mode == 4) hipLaunchKernelGGL((k_grad_tile_batch<S, FLUX, VISC, 4, MIX>), g, b, lds, s, mem, items, par, stg);
+        else hipLaunchKernelGGL((k_grad_tile_batch<S, FLUX, VISC, 2, MIX>), g, b, lds, s, mem, items, par, stg);
+        return;
+    }
     const dim3 g(n), b(BLK);
     if (gs.order == 1) hipLaunchKernelGGL((k_face_tile_batch<S, FLUX, VISC, 1, 0, MIX>), g, b, lds, s, mem, items, par, stg);
     else if (!(gs.limited && limitStage))

@@ -73,7 +73,7 @@ void serveLoop(ebb_server* sv)
             const int64_t id = sv->queue.front();
             Ticket& t = sv->tickets[id];
             Config cfg = t.c->c;
-            cfg.stagePath = 1;
+            cfg.stagePath = cfg.stagePath == 3 ? 3 : 1;
             const int64_t need = estimateDeviceBytes(t.m->b, cfg);
             int64_t info[6];
             ebb_batch_info(sv->b, info);

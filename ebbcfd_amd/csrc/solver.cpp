@@ -186,6 +186,8 @@ void stage(ebb_solver* s, int kind, double4* qin, const double4* qn, double4* qo
     s->stageStep.push_back(stepId);
     if (s->useTiles && !estate && kind != SK_RESIDUAL) {
         launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
+    } else if (s->useGradTiles && !estate && kind != SK_RESIDUAL) {
+        launchGradTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream);
     } else if (s->useFaceTiles && !estate && kind != SK_RESIDUAL) {
         launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
         launchFaceTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream);
@@ -342,8 +344,21 @@ int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& st
     const long long want = c.tileCells;
     if (want != -1 && (want < 1 || want > 1024)) return fail(s, EBB_ERR_ARG, "tileCells must be -1 or in 1..1024");
     std::vector<uint32_t> order;
-    uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 ? 768 : 128);
-    if (c.stagePath == 1 && want <= 0) {
+    uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 || c.stagePath == 3 ? 768 : 128);
+    if (c.stagePath == 3 && want <= 0) {
+        // automatic gradient tiles: the largest multiple of 16 up to 768 cells whose busiest
+        // tile keeps its gradient rows (12 values per own or ring cell) and face results
+        // within a third of the CU's 160 KB of LDS, so three workgroups stay resident
+        constexpr uint32_t LDS_TARGET = 53248;
+        const uint32_t rowBytes = 12 * (c.precision ? 4 : 8);
+        for (;;) {
+            tileOrder(ref, T, c.tileOrder, order, start);
+            const uint32_t need = gradTileLds(ref, order, start, rowBytes);
+            if (need <= LDS_TARGET || T <= 16) break;
+            const uint32_t next = uint32_t(uint64_t(T) * LDS_TARGET / need) / 16 * 16;
+            T = std::max<uint32_t>(16, std::min(next, T - 16));
+        }
+    } else if (c.stagePath == 1 && want <= 0) {
         // automatic face tiles: the largest multiple of 16 up to 768 cells whose busiest
         // tile keeps its face results (40 B per face) within a third of the CU's 160 KB
         // of LDS, so three workgroups stay resident (768 fits triangles; quads and larger
@@ -388,8 +403,15 @@ int buildFaceTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& 
     FaceTileHost h;
     std::string err;
     if (!makeFaceTiles(L, order, start, h, err)) return fail(s, EBB_ERR_MESH, err);
-    const uint32_t lds = 40 * (h.maxF + 1);
-    if (lds > 65536) return EBB_OK;   // does not fit: split kernels
+    const bool grad = s->cfg.stagePath == 3;
+    uint32_t lds = 40 * (h.maxF + 1), glDoubles = 0;
+    if (grad) {
+        if (!addGradTiles(L, h, err)) return fail(s, EBB_ERR_MESH, err);
+        const uint32_t rowBytes = 12 * (s->cfg.precision ? 4 : 8);
+        glDoubles = (h.maxG * rowBytes + 15) / 16 * 2;
+        lds += 8 * glDoubles;
+    }
+    if (lds > uint32_t(s->ldsPerBlock)) return EBB_OK;   // does not fit: split kernels
     HIPCHK(trackUpload(s, s->d_ftDesc, h.desc));
     HIPCHK(trackUpload(s, s->d_ftRef, h.fref));
     HIPCHK(trackUpload(s, s->d_ftIds, h.ids));
@@ -401,8 +423,26 @@ int buildFaceTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& 
     ft.cellStride = h.cellStride;
     ft.faceStride = h.faceStride;
     ft.desc = s->d_ftDesc.p; ft.ids = s->d_ftIds.p; ft.cf = s->d_ftCf.p; ft.fref = s->d_ftRef.p; ft.fid = s->d_ftFid.p;
+    if (grad) {
+        HIPCHK(trackUpload(s, s->d_ftSlot, h.fslot));
+        HIPCHK(trackUpload(s, s->d_ftHalo, h.halo));
+        HIPCHK(trackUpload(s, s->d_ftNbr, h.nbrT));
+        HIPCHK(trackUpload(s, s->d_ftNHalo, h.nHalo));
+        if (h.border.empty()) h.border.push_back(0);   // a valid pointer; nBorder stays 0
+        const uint32_t nB = h.sumH ? uint32_t(h.border.size()) : 0u;
+        HIPCHK(trackUpload(s, s->d_border, h.border));
+        ft.fslot = s->d_ftSlot.p; ft.halo = s->d_ftHalo.p; ft.nHalo = s->d_ftNHalo.p; ft.border = s->d_border.p;
+        ft.nbrT = s->d_ftNbr.p;
+        ft.haloStride = h.haloStride;
+        ft.nBorder = nB;
+        ft.glDoubles = glDoubles;
+        s->gradTileStats[0] = h.maxG;
+        s->gradTileStats[1] = double(h.sumH);
+        s->gradTileStats[2] = nB;
+    }
     s->ftiles = ft;
     s->useFaceTiles = true;
+    s->useGradTiles = grad;
     return EBB_OK;
 }
 
@@ -412,9 +452,10 @@ int buildTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& orde
     s->useTiles = false;
     s->tiles = TileArgs{};
     s->useFaceTiles = false;
+    s->useGradTiles = false;
     s->ftiles = FaceTileArgs{};
     if (L.N == 0 || s->cfg.stagePath == 0) return EBB_OK;
-    if (s->cfg.stagePath == 1) return buildFaceTiles(s, L, order, start);
+    if (s->cfg.stagePath == 1 || s->cfg.stagePath == 3) return buildFaceTiles(s, L, order, start);
     const int lp = (s->phys.order > 1 && s->phys.limited && s->phys.lpThresh > 0) ? 1 : 0;
     TileHost h;
     std::string err;
@@ -444,6 +485,7 @@ int retileFaceTiles(ebb_solver* s, long long tileCells)
     if (!s->useFaceTiles || !s->refOf.empty()) return fail(s, EBB_ERR_STATE, "only face tiles in the reference numbering can be re-tiled");
     auto drop = [&](auto& b) { s->deviceBytes -= int64_t((b.n ? b.n : 1) * sizeof(*b.p)); b.release(); };
     drop(s->d_ftDesc); drop(s->d_ftRef); drop(s->d_ftIds); drop(s->d_ftCf); drop(s->d_ftFid);
+    drop(s->d_ftSlot); drop(s->d_ftHalo); drop(s->d_ftNHalo); drop(s->d_border); drop(s->d_ftNbr);
     s->cfg.tileCells = tileCells;
     Layout ref, lay;
     makeLayout(s->b, ref);
@@ -493,8 +535,10 @@ int64_t estimateDeviceBytes(const BuiltMesh& b, const Config& c)
     if (c.localTimestep) bytes += N * 8;
     // face tiles: ~1.1 cells' and ~1.7 faces' worth of records per cell, plus the padding
     bytes += int64_t(1.1 * double(N)) * (4 + 2 * S) + int64_t(1.7 * double(N)) * (16 + 4) + 2 * 256 * 20;
+    // gradient tiles: an LDS-row pair per face record, each tile's ring and the border (~0.3 N each)
+    if (c.stagePath == 3) bytes += int64_t(1.7 * double(N)) * 4 + int64_t(0.6 * double(N)) * 4 + int64_t(1.1 * double(N)) * 4 * S + 4096;
     bytes += ((N + 15) / 16) * 64;                         // RMS leaf sums
-    const int64_t tiles = N / 700 + 1;
+    const int64_t tiles = N / (c.stagePath == 3 ? 250 : 700) + 1;
     bytes += (2 * std::max<int64_t>((N + 255) / 256, tiles) + 2) * 4 * 8;   // stage partials
     bytes += 256 * HIST_FIELDS * 8;
     const int64_t nchunks = (C / 16 + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
@@ -732,6 +776,14 @@ int createSolver(const ebb_mesh* m, const Config& cfg, int device, ebb_solver** 
     s->b = m->b;
     s->cfg = cfg;
     s->device = device;
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+            s->ldsPerBlock = lds;
+    }
+    // gradient tiles need gradients (order 2) in rows of 4 scalar limiters: other
+    // configurations take the face-tile path
+    if (s->cfg.stagePath == 3 && (s->cfg.order < 2 || (s->cfg.limited && s->cfg.lpThresh > 0))) s->cfg.stagePath = 1;
     if (s->cfg.limiter == 1 && (s->cfg.stagePath == 2 || (s->cfg.order > 1 && s->cfg.limited && s->cfg.lpThresh > 0))) {
         std::fprintf(stderr, "ebb_solver_create: the Venkatakrishnan limiter runs on the face-tile and split stage "
                              "paths, without the LP vector limiter\n");
@@ -1308,8 +1360,9 @@ int ebb_cell_gradients(ebb_solver* s, double* grad, double* lim)
     return guarded([&]() -> int {
         if (!s) return EBB_ERR_ARG;
         HIPCHK(hipSetDevice(s->device));
-        // the fused stage kernel keeps gradients in LDS: recompute them for the last stage input
-        if (s->useTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
+        // the fused and gradient-tile kernels keep gradients in LDS: recompute them for the last
+        // stage input
+        if (s->useTiles || s->useGradTiles) { int rc_ = refreshEdgeStates(s); if (rc_) return rc_; }
         if (grad) {
             int rc_ = s->phys.mixed ? downloadCellsF(s, s->d_grad32.p, grad, 8) : downloadCells(s, s->d_grad.p, grad, 8);
             if (rc_) return rc_;
@@ -1498,12 +1551,18 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
                 a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtUpdate; a.first = x.first; a.last = x.last;
                 a.rmsBlocks = (x.last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
                 a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
-            mixArgs(s, a, x.in, x.out);
+                mixArgs(s, a, x.in, x.out);
                 s->stageStep.push_back(s->stepsTotal);
                 if (s->useTiles) {
                     launchStageTiles(s->dm, s->phys, a, s->tiles, int(s->S), s->stream);
                     HIPCHK(hipEventRecord(ev[k++], s->stream));
                     part.push_back(3);
+                } else if (s->useGradTiles) {
+                    launchGradTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream, 1);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    launchGradTiles(s->dm, s->phys, a, s->ftiles, int(s->S), s->stream, 2);
+                    HIPCHK(hipEventRecord(ev[k++], s->stream));
+                    part.push_back(0); part.push_back(3);
                 } else if (s->useFaceTiles) {
                     launchGradient(s->dm, s->phys, a, int(s->S), s->stream);
                     HIPCHK(hipEventRecord(ev[k++], s->stream));
@@ -1564,6 +1623,7 @@ void ebb_solver_destroy(ebb_solver* s)
     s->d_acc.release(); s->d_R.release(); s->d_estate.release();
     s->d_tDesc.release(); s->d_tBlob.release();
     s->d_ftDesc.release(); s->d_ftRef.release(); s->d_ftIds.release(); s->d_ftCf.release(); s->d_ftFid.release(); s->d_node.release();
+    s->d_ftSlot.release(); s->d_ftHalo.release(); s->d_ftNHalo.release(); s->d_border.release(); s->d_ftNbr.release();
     s->d_devOf.release(); s->d_rmsBlocks.release();
     s->d_qA32.release(); s->d_qB32.release(); s->d_T32.release(); s->d_T232.release();
     s->d_grad32.release(); s->d_lim32.release();

@@ -104,6 +104,11 @@ struct ebb_solver {
     DBuf<double2> d_node;
     FaceTileArgs ftiles{};
     bool useFaceTiles = false;
+    // gradient tiles (face tiles whose kernel computes its own cells' gradients)
+    DBuf<uint32_t> d_ftSlot, d_ftHalo, d_ftNHalo, d_border, d_ftNbr;
+    bool useGradTiles = false;
+    double gradTileStats[3] = {0, 0, 0};   // largest rows per tile, ring cells summed over tiles, border cells
+    int ldsPerBlock = 65536;               // hipDeviceProp_t::sharedMemPerBlock of the solver's device
     // device cell numbering of the tiled paths: device cell d is reference cell
     // refOf[d] (empty: identical numbering); d_devOf maps back for the RMS sums
     std::vector<uint32_t> refOf, devOf;

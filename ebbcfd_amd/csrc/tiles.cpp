@@ -333,6 +333,93 @@ bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const st
     return true;
 }
 
+bool addGradTiles(const Layout& L, FaceTileHost& h, std::string& err)
+{
+    const uint32_t N = L.N, nT = h.nTiles;
+    h.fslot.assign(h.fref.size(), 0u);
+    h.nHalo.assign(nT, 0u);
+    h.halo.clear();
+    h.border.clear();
+    h.maxG = 0;
+    h.sumH = 0;
+    if (nT == 0) { h.haloStride = 1; h.halo.assign(1, 0u); return true; }
+    std::vector<uint32_t> stamp(N, ~0u), slot(N, 0u), ring, roff(nT + 1, 0u);
+    std::vector<uint8_t> inRing(N, 0);
+    uint32_t HS = 1;
+    for (uint32_t t = 0; t < nT; t++) {
+        const uint4 d = h.desc[t];
+        const uint32_t nC = d.x, nF = d.y;
+        for (uint32_t k = 0; k < nC; k++) {
+            const uint32_t c = h.ids[d.z + k];
+            stamp[c] = t;
+            slot[c] = k;
+        }
+        uint32_t nH = 0;
+        auto rowOf = [&](uint32_t c) {
+            if (stamp[c] != t) {   // a ring cell, first seen
+                stamp[c] = t;
+                slot[c] = nC + nH++;
+                ring.push_back(c);
+                inRing[c] = 1;
+            }
+            return slot[c];
+        };
+        for (uint32_t k = 0; k < nF; k++) {
+            const uint4 r = h.fref[d.w + k];
+            const uint32_t la = rowOf(r.x);
+            const uint32_t lb = r.y < N ? rowOf(r.y) : la;   // a boundary face: the ghost has no row
+            h.fslot[d.w + k] = la | (lb << 16);
+        }
+        if (nC + nH > 65535) { err = "gradient tile has too many cells; lower tileCells"; return false; }
+        h.nHalo[t] = nH;
+        roff[t + 1] = roff[t] + nH;
+        HS = std::max(HS, nH);
+        h.maxG = std::max(h.maxG, nC + nH);
+        h.sumH += nH;
+    }
+    // each own cell's neighbour row, at its position in ids (padding slots: cell 0's)
+    h.nbrT.assign(h.ids.size() * L.S, 0u);
+    for (size_t x = 0; x < h.ids.size(); x++)
+        for (uint32_t j = 0; j < L.S; j++) h.nbrT[x * L.S + j] = L.nbr[size_t(h.ids[x]) * L.S + j];
+    h.haloStride = HS;
+    h.halo.assign(size_t(nT) * HS, 0u);
+    for (uint32_t t = 0; t < nT; t++)
+        std::copy(ring.begin() + roff[t], ring.begin() + roff[t + 1], h.halo.begin() + size_t(t) * HS);
+    for (uint32_t c = 0; c < N; c++)
+        if (inRing[c]) h.border.push_back(c);
+    return true;
+}
+
+uint32_t gradTileLds(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
+                     uint32_t rowBytes)
+{
+    const uint32_t N = L.N, S = L.S, F = L.Fi + L.Fb;
+    if (N == 0 || start.size() < 2) return 0;
+    std::vector<uint32_t> stamp(N, ~0u), fstamp(F, ~0u);
+    uint32_t maxRows = 0, maxF = 0;
+    for (uint32_t t = 0; t + 1 < start.size(); t++) {
+        const uint32_t c0 = start[t], nC = start[t + 1] - c0;
+        for (uint32_t k = 0; k < nC; k++) stamp[order[c0 + k]] = t;
+        uint32_t nF = 0, nH = 0;
+        for (uint32_t k = 0; k < nC; k++)
+            for (uint32_t j = 0; j < S; j++) {
+                const uint32_t f = L.cface[size_t(order[c0 + k]) * S + j] & 0x7fffffffu;
+                if (f == F || fstamp[f] == t) continue;
+                fstamp[f] = t;
+                nF++;
+                if (f < L.Fi) {
+                    for (uint32_t c : {L.fcell[f].x, L.fcell[f].y})
+                        if (stamp[c] != t) { stamp[c] = t; nH++; }
+                }
+            }
+        maxRows = std::max(maxRows, nC + nH);
+        maxF = std::max(maxF, nF);
+    }
+    // the kernel's layout: the largest tile's rows, then the most faces of any tile
+    const uint64_t lds = (uint64_t(maxRows) * rowBytes + 15) / 16 * 16 + 40ull * (maxF + 1);
+    return uint32_t(std::min<uint64_t>(lds, 0xFFFFFFFFull));
+}
+
 bool makeTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start, TileHost& h,
                std::string& err)
 {

@@ -46,6 +46,10 @@ struct FaceTileHost {
     uint32_t nTiles = 0, maxF = 0;
     uint32_t cellStride = 0, faceStride = 0;   // tile t: cells from t * cellStride, faces from t * faceStride
     uint64_t sumF = 0;
+    // gradient tiles (addGradTiles): LDS rows of each face's cells, each tile's ring, the border
+    std::vector<uint32_t> fslot, halo, nHalo, border, nbrT;
+    uint32_t haloStride = 0, maxG = 0;
+    uint64_t sumH = 0;
 };
 
 // Cut the interior cells into tiles of at most T cells (TILE_GROW: compact regions
@@ -62,6 +66,16 @@ uint32_t maxTileFaces(const Layout& L, const std::vector<uint32_t>& order, const
 // Face tiles: each cell group with the faces of its cells.
 bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
                    FaceTileHost& h, std::string& err);
+
+// Gradient tiles on face tiles built by makeFaceTiles: each tile's own cells take LDS rows
+// 0..nC-1 in tile order, the cells across its faces that other tiles own (its ring) rows
+// nC.. in order of first appearance among its faces; the border is every ring cell.
+bool addGradTiles(const Layout& L, FaceTileHost& h, std::string& err);
+// Per-workgroup LDS of gradient tiles with `rowBytes` per gradient row: the rows of the tile
+// with the most rows plus the face results (40 bytes per face and a padding face) of the
+// tile with the most faces, without building the tiles.
+uint32_t gradTileLds(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
+                     uint32_t rowBytes);
 
 // Fused-kernel tiles: each cell group with its halo, tables and static data packed.
 bool makeTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start, TileHost& h,

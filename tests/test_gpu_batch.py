@@ -6,6 +6,8 @@ mesh and config bit for bit -- whatever else is in the batch, with or without gr
 capture, and after the autotuner (which must also restore every state).  Admission is
 by device footprint.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -56,6 +58,30 @@ def test_batch_equals_solo_runs(graph_steps):
         assert len(hb) == steps and st["steps"] == steps
         np.testing.assert_array_equal(hb, hs)
         np.testing.assert_array_equal(b.q(sid), s.q)
+        assert (st["t"], st["dt"]) == (t, dt)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_grad_tiles_equal_solo_runs(precision):
+    """Members on the gradient-tile path (border blocks as gradient items, tiles that compute
+    their own gradients) beside members on face tiles, in one graph-captured batch."""
+    cases = _cases()
+    steps = 60
+    cfgs = []
+    for k, (raw, cfg) in enumerate(cases):
+        c = copy.deepcopy(cfg)
+        c["extensions"] = dict(c.get("extensions", {}), stagePath="gradTiles" if k % 3 else "faceTiles",
+                               precision=precision)
+        cfgs.append(c)
+    b = Batch(graph_steps=8, keep_history=True)
+    ids = [b.add(build_mesh(raw), c) for (raw, _), c in zip(cases, cfgs)]
+    assert b.run(steps) == len(cases)
+    for (raw, _), c, sid in zip(cases, cfgs, ids):
+        s = Solver(build_mesh(raw), c)
+        hs, t, dt = s.run(steps)
+        np.testing.assert_array_equal(b.history(sid), hs)
+        np.testing.assert_array_equal(b.q(sid), s.q)
+        st = b.status(sid)
         assert (st["t"], st["dt"]) == (t, dt)
 
 

@@ -34,7 +34,7 @@ def workload():
     yield raw, cfg, hist, t, dt, om.q[: om.n_interior].copy()
 
 
-@pytest.mark.parametrize("path", ["faceTiles", "split", "fused"])
+@pytest.mark.parametrize("path", ["faceTiles", "split", "fused", "gradTiles"])
 def test_full_size_steps_match_oracle(workload, path):
     raw, cfg, hist_o, t_o, dt_o, q_o = workload
     c = copy.deepcopy(cfg)

@@ -2,7 +2,9 @@
 
 "faceTiles": the gradient kernel, then one kernel that computes every face of a
 tile of cells into LDS and updates those cells.  "fused": one kernel per stage
-that also computes ghosts and gradients/limiters for the tile and its halo.
+that also computes ghosts and gradients/limiters for the tile and its halo.  "gradTiles":
+the face tiles' kernel also computes its own cells' ghosts, gradients and limiters into LDS;
+a border kernel computes those of the cells other tiles need (each tile's ring).
 Faces on tile borders are computed by both tiles with the same device code, so
 every result must be bit-identical to the three-kernel path, for tile sizes
 down to one cell.
@@ -55,7 +57,9 @@ def with_path(cfg, path, n=-1, order="blocks", cells="reference"):
     ("faceTiles", -1, "blocks", "reference"), ("faceTiles", 32, "blocks", "tiles"),
     ("faceTiles", 29, "morton", "reference"), ("faceTiles", 1, "grow", "tiles"), ("faceTiles", 256, "grow", "reference"),
     ("fused", 1, "grow", "tiles"), ("fused", 13, "grow", "reference"), ("fused", 64, "blocks", "tiles"),
-    ("fused", 128, "morton", "reference")])
+    ("fused", 128, "morton", "reference"),
+    ("gradTiles", -1, "blocks", "reference"), ("gradTiles", 32, "blocks", "tiles"), ("gradTiles", 29, "morton", "reference"),
+    ("gradTiles", 1, "grow", "tiles"), ("gradTiles", 256, "grow", "reference")])
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_tiled_equals_split(name, path, tile, order, cells):
     raw, cfg = CASES[name]()
@@ -76,7 +80,7 @@ def test_tiled_equals_split(name, path, tile, order, cells):
     np.testing.assert_array_equal(fl_f, fl_s)
 
 
-@pytest.mark.parametrize("path", ["fused", "faceTiles"])
+@pytest.mark.parametrize("path", ["fused", "faceTiles", "gradTiles"])
 def test_tiled_matches_oracle_many_tiles(path):
     raw, cfg = cylinder_case(n_theta=96, n_r=40, integrator="RK4", flux="roeFlux")
     cfg = with_path(cfg, path, 32)
@@ -88,7 +92,7 @@ def test_tiled_matches_oracle_many_tiles(path):
     np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
 
 
-@pytest.mark.parametrize("path", ["fused", "faceTiles"])
+@pytest.mark.parametrize("path", ["fused", "faceTiles", "gradTiles"])
 def test_tiled_large_mesh_equals_split(path):
     raw, cfg = cylinder_case(n_theta=512, n_r=256, stretch=1.015, CFL=0.05)
     split = Solver(build_mesh(raw), with_path(cfg, "split"))
@@ -103,7 +107,8 @@ def test_tiled_large_mesh_equals_split(path):
     assert ts["fused_ms"] == 0 and ts["faces_ms"] > 0
 
 
-@pytest.mark.parametrize("path,tile", [("split", -1), ("fused", 5), ("fused", 128), ("faceTiles", 7)])
+@pytest.mark.parametrize("path,tile", [("split", -1), ("fused", 5), ("fused", 128), ("faceTiles", 7), ("gradTiles", 7),
+                                       ("gradTiles", -1)])
 def test_tiled_nan_cell_and_edge_reported_like_split(path, tile):
     raw, cfg = channel_case(integrator="RK2")
     ref = Solver(build_mesh(raw), with_path(cfg, "split"))
@@ -120,7 +125,7 @@ def test_tiled_nan_cell_and_edge_reported_like_split(path, tile):
     assert eg.value.index == er.value.index and eg.value.step == er.value.step
 
 
-@pytest.mark.parametrize("path", ["fused", "faceTiles"])
+@pytest.mark.parametrize("path", ["fused", "faceTiles", "gradTiles"])
 def test_renumbered_interfaces_match_split(path, tmp_path):
     """Tiled paths renumber the device cells; every interface still speaks the reference numbering."""
     raw, cfg = channel_case(integrator="RK2_TVD", viscosity=True)
@@ -170,7 +175,7 @@ def test_quad_mesh_takes_face_tiles_by_default():
     assert tt["fused_ms"] > 0 and tt["faces_ms"] == 0
 
 
-@pytest.mark.parametrize("path", ["split", "faceTiles", "fused"])
+@pytest.mark.parametrize("path", ["split", "faceTiles", "fused", "gradTiles"])
 @pytest.mark.parametrize("kind", ["tri1", "tri2", "quad1"])
 def test_tiny_meshes_match_oracle(kind, path):
     """One or two cells, all on the boundary: a tile shorter than a 16-cell block and an
@@ -191,3 +196,26 @@ def test_tiny_meshes_match_oracle(kind, path):
     np.testing.assert_array_equal(h_g, h_o)
     np.testing.assert_array_equal(s.q, om.q[: om.n_interior])
     assert t_g == t_o
+
+
+MIXED_CASES = ["cyl_ns_rk2tvd", "cyl_euler_roe_rk4", "rk4_local_dt", "quads_rk2tvd_visc", "pentagons_rk2tvd_visc"]
+
+
+@pytest.mark.parametrize("limiter", ["scalar", "venkatakrishnan"])
+@pytest.mark.parametrize("tile", [-1, 19])
+@pytest.mark.parametrize("name", MIXED_CASES)
+def test_grad_tiles_equal_face_tiles_mixed_and_venkat(name, tile, limiter):
+    """The mixed-precision and Venkatakrishnan forms of the gradient-tile path compute the
+    same FP32 (or FP64) gradients and faces as the face-tile path: bit-identical."""
+    raw, cfg = CASES[name]()
+    for prec in ("mixed", "fp64"):
+        c = copy.deepcopy(cfg)
+        c["extensions"] = dict(c.get("extensions", {}), precision=prec, limiter=limiter)
+        ft = Solver(build_mesh(raw), with_path(c, "faceTiles", tile))
+        gt = Solver(build_mesh(raw), with_path(c, "gradTiles", tile))
+        h_f, t_f, _ = ft.run(12)
+        h_g, t_g, _ = gt.run(12)
+        np.testing.assert_array_equal(h_g, h_f)
+        np.testing.assert_array_equal(gt.q_all(), ft.q_all())
+        assert t_g == t_f
+        assert gt.time_steps(1)["gradient_ms"] > 0
