@@ -41,9 +41,12 @@ FACE_BYTES_PER_CELL = 144.0     # state 32 + gradient 64 + limiter 32 + centroid
 # rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a plain 1 GiB copy (x2.0 and
 # x1.0), averaged over the limited and unlimited stage; the summary and the
 # commands are in profiles/hbm_traffic_cylinder_ns_4M.md.  Re-measured whenever
-# the kernel changes; reported only for the default configuration it was
-# measured on.
-MEASURED_TRAFFIC = {("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference"): 1.3674e9}
+# the kernel changes; reported only for the configurations it was measured on
+# (kernel, workload, stage path, tile cells, tile order, cell order, limiter).
+MEASURED_TRAFFIC = {
+    ("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference", "venkatakrishnan"): 1.3695e9,
+    ("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference", "scalar"): 1.3674e9,
+}
 
 
 def log(msg):
@@ -229,9 +232,11 @@ def serve_bench(args):
     log("serve: %d requests, %d cells in all, built in %.1f s" % (n, sum(m.n_interior for m in meshes),
                                                                time.perf_counter() - t0))
     rows = []
-    for rate in [float(r) for r in args.serve_rates.split(",")]:
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    for rate in rates:
         for mode, max_solves in (("batched", 256), ("one_at_a_time", 1)):
-            if mode == "one_at_a_time" and rate != float(args.serve_rates.split(",")[0]):
+            # one at a time is compared where both are saturated: the highest rate
+            if mode == "one_at_a_time" and rate != max(rates):
                 continue
             sv = Server(max_solves=max_solves, steps_per_round=args.serve_round, graph_steps=8)
             arr = np.cumsum(rng.exponential(1.0 / rate, n))
@@ -406,7 +411,7 @@ def main():
     ap.add_argument("--batched-solo-sample", type=int, default=32, help="solves timed one at a time")
     ap.add_argument("--serve", action="store_true", help="run the serving sweep instead")
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-rates", default="2,20,200", help="arrival rates (1/s); the first also runs one at a time")
+    ap.add_argument("--serve-rates", default="1,20,200", help="arrival rates (1/s); the highest also runs one at a time")
     ap.add_argument("--serve-max-steps", type=int, default=20000)
     ap.add_argument("--serve-round", type=int, default=200, help="steps per serving round")
     ap.add_argument("--serve-precision", default="mixed", choices=("mixed", "fp64"))
@@ -478,7 +483,7 @@ def main():
     if args.stretch is not None:
         workload += "_stretch%g" % args.stretch
     traffic = MEASURED_TRAFFIC.get((kernel, workload, args.stage_path, args.tile_cells, args.tile_order,
-                                    args.cell_order))
+                                    args.cell_order, args.limiter))
 
     dropin = dropin_leg(s, N, args.dropin_stages, stage_ms) if args.dropin_stages > 0 else None
     mixed = None
