@@ -246,7 +246,13 @@ def serve_bench(args):
                 if d > 0:
                     time.sleep(d)
                 sv.submit(meshes[k], reqs[k][1])
-            ok = sv.wait(n, 3000.0)
+            ok = False
+            t_wait = time.perf_counter()
+            while time.perf_counter() - t_wait < 3000.0:
+                ok = sv.wait(n, 30.0)
+                if ok:
+                    break
+                log("serve %s rate %g/s: %d of %d done" % (mode, rate, int(sv.stats()["done"]), n))
             st = sv.stats()
             polls = [sv.poll(k) for k in range(n)]
             sv.close()

@@ -215,8 +215,9 @@ int ebb_limiter1d(int device, int kind, int64_t n, const double* sp, const doubl
  * stage one gradient launch and one face/update launch over every mesh's blocks and tiles,
  * then one RMS launch and one epilogue launch, the whole step sequence captured as a HIP
  * graph.  Each mesh's states and history equal its solo ebb_run's bit for bit.  Members
- * use the face-tile stage path in the reference cell numbering; the LP vector limiter and
- * the fused stage path are not batched. */
+ * use the face-tile stage path (or the gradient tiles, extensions.stagePath = "gradTiles")
+ * in the reference cell numbering; the LP vector limiter and the fused stage path are not
+ * batched. */
 typedef struct ebb_batch ebb_batch;
 
 /* Device bytes ebb_solver_create would allocate for this mesh and config (an estimate
