@@ -1017,6 +1017,8 @@ int ebb_config_get(const ebb_config* c, const char* key, double* value)
         else if (ip) *value = double(*ip);
         else if (bp) *value = *bp ? 1.0 : 0.0;
         else if (std::string(key) == "precision") *value = c->c.precision;
+        else if (std::string(key) == "limiter") *value = c->c.limiter;
+        else if (std::string(key) == "stagePath") *value = c->c.stagePath;
         else if (std::string(key) == "integrator") *value = c->c.integratorKind();
         else if (std::string(key) == "flux") *value = c->c.fluxKind();
         else return EBB_ERR_ARG;
@@ -1034,6 +1036,8 @@ int ebb_config_set(ebb_config* c, const char* key, double value)
         else if (ip) *ip = (long long)value;
         else if (bp) *bp = value != 0.0;
         else if (std::string(key) == "precision") c->c.precision = int(value);
+        else if (std::string(key) == "limiter") c->c.limiter = int(value);
+        else if (std::string(key) == "stagePath") c->c.stagePath = int(value);
         else return EBB_ERR_ARG;
         return EBB_OK;
     });

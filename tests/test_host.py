@@ -221,6 +221,23 @@ def test_config_type_rules():
     assert load_config(json.dumps(odd))["order"] == 2
 
 
+def test_config_extension_keys():
+    """The extension keys this build adds: parsed into the config, unknown values refused."""
+    c = dict(REF_CFG)
+    c["extensions"] = {"precision": "mixed", "limiter": "venkatakrishnan", "venkatK": 2.5, "rmsDrop": 1e-8,
+                       "stagePath": "gradTiles", "maxSteps": 20000}
+    cfg = load_config(json.dumps(c))
+    assert cfg["precision"] == 1 and cfg["limiter"] == 1 and cfg["venkatK"] == 2.5 and cfg["rmsDrop"] == 1e-8
+    assert cfg["stagePath"] == 3 and cfg["maxSteps"] == 20000
+    plain = load_config(json.dumps(REF_CFG))
+    assert plain["precision"] == 0 and plain["limiter"] == 0 and plain["rmsDrop"] == 0 and plain["stagePath"] == 1
+    for key, value in (("precision", "fp16"), ("limiter", "minmod"), ("stagePath", "tiles")):
+        bad = dict(REF_CFG)
+        bad["extensions"] = {key: value}
+        with pytest.raises(Exception):
+            load_config(json.dumps(bad))
+
+
 def test_config_set_get_roundtrip():
     c = Config(REF_CFG)
     c["CFL"] = 0.3
