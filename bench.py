@@ -216,16 +216,20 @@ def batched_leg(args):
 
 def serve_bench(args):
     """Serving: Poisson arrivals of steady channel-bump Euler solves (sizes log-uniform in
-    [1e4, 2e6] cells, M U(0.3, 1.5)), each run to a 1e-8 drop of the rho RMS residual or the
-    step cap; continuous batching with admission by device footprint; p50/p99 time to
-    converged and solves/s per arrival rate, beside the same stream served one at a time."""
+    [1e4, 2e6] cells, M U(0.3, 1.5), local time steps, Venkatakrishnan limiter K = 5, FP64),
+    each run to a 1e-8 drop of the rho RMS residual or the step cap; continuous batching with
+    admission by device footprint; p50/p99 time to converged and solves/s per arrival rate,
+    beside the same stream served one at a time."""
     from ebbcfd_amd import Server, build_mesh
     rng = np.random.Generator(np.random.PCG64(args.serve_seed))
     n = args.serve_requests
     cells = np.exp(rng.uniform(np.log(1e4), np.log(2e6), n))
     t0 = time.perf_counter()
-    reqs = [bump_request(rng, int(c), (0.3, 1.5), args.serve_seed * 1000 + k, precision=args.serve_precision,
-                         rmsDrop=1e-8, maxSteps=args.serve_max_steps) for k, c in enumerate(cells)]
+    ext = {"rmsDrop": 1e-8, "maxSteps": args.serve_max_steps, "limiter": args.serve_limiter}
+    if args.serve_limiter == "venkatakrishnan":
+        ext["venkatK"] = args.serve_venkat_k
+    reqs = [bump_request(rng, int(c), (0.3, 1.5), args.serve_seed * 1000 + k, precision=args.serve_precision, **ext)
+            for k, c in enumerate(cells)]
     for _, cfg in reqs:
         cfg["localTimestep"] = True
     meshes = [build_mesh(raw) for raw, _ in reqs]
@@ -257,15 +261,20 @@ def serve_bench(args):
             polls = [sv.poll(k) for k in range(n)]
             sv.close()
             conv = sum(1 for p in polls if p["converged"])
+            steps = np.array([p["steps"] for p in polls])
+            wait = np.array([p["admit_s"] - p["arrival_s"] for p in polls])
             row = {"rate_per_s": rate, "mode": mode, "done": int(st["done"]), "failed": int(st["failed"]),
                    "p50_s": round(st["p50_s"], 3), "p99_s": round(st["p99_s"], 3),
-                   "solves_per_s": round(st["solves_per_s"], 3), "converged": conv, "complete": bool(ok)}
+                   "solves_per_s": round(st["solves_per_s"], 3), "converged": conv,
+                   "steps_min_median_max": [int(steps.min()), int(np.median(steps)), int(steps.max())],
+                   "admission_wait_p50_s": round(float(np.median(wait)), 3), "complete": bool(ok)}
             log("serve %s rate %g/s: %s" % (mode, rate, row))
             rows.append(row)
     out = {"metric": "batched solves/s + p99 latency (serving)", "unit": "solves/s",
            "config": {"workload": "channel_bump_poisson_loguniform_1e4_2e6", "requests": n, "max_steps": args.serve_max_steps,
-                      "rms_drop": 1e-8, "precision": args.serve_precision, "steps_per_round": args.serve_round,
-                      "seed": args.serve_seed},
+                      "rms_drop": 1e-8, "precision": args.serve_precision, "limiter": args.serve_limiter,
+                      "venkatK": args.serve_venkat_k if args.serve_limiter == "venkatakrishnan" else None,
+                      "steps_per_round": args.serve_round, "seed": args.serve_seed},
            "rows": rows}
     print(json.dumps(out), flush=True)
 
@@ -420,7 +429,11 @@ def main():
     ap.add_argument("--serve-rates", default="1,20,200", help="arrival rates (1/s); the highest also runs one at a time")
     ap.add_argument("--serve-max-steps", type=int, default=20000)
     ap.add_argument("--serve-round", type=int, default=200, help="steps per serving round")
-    ap.add_argument("--serve-precision", default="mixed", choices=("mixed", "fp64"))
+    ap.add_argument("--serve-precision", default="fp64", choices=("mixed", "fp64"),
+                    help="fp64: a 1e-8 drop of the rho RMS residual is below the mixed form's rounding floor")
+    ap.add_argument("--serve-limiter", default="venkatakrishnan", choices=("venkatakrishnan", "scalar"),
+                    help="the scalar limiter stalls these steady solves near a 1e-3 drop")
+    ap.add_argument("--serve-venkat-k", type=float, default=5.0)
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters) and exit")
