@@ -6,8 +6,8 @@ by up to 10 % of h (seed 1), exact Dirichlet boundaries and source
 (app/ms.d:131-141, 285-298), Roe flux (source/flux.d:224-303), least-squares
 gradients with the scalar limiter (source/solve.d:324-403), RK4
 (source/integrators.d:54-134), CFL 0.5, 1,000 steps.  The whole per-step
-history and the final states must equal the oracle's exactly; the tolerance
-north_star states (1e-11 relative) is asserted as well.
+history and the final states must equal the oracle's exactly; the path's stated
+FP64 tolerance (1e-11 relative) is asserted as well.
 """
 import numpy as np
 import pytest
