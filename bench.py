@@ -240,7 +240,7 @@ def serve_bench(args):
     for rate in rates:
         for mode, max_solves in (("batched", 256), ("one_at_a_time", 1)):
             # one at a time is compared where both are saturated: the highest rate
-            if mode == "one_at_a_time" and rate != max(rates):
+            if mode == "one_at_a_time" and (rate != max(rates) or not args.serve_solo):
                 continue
             sv = Server(max_solves=max_solves, steps_per_round=args.serve_round, graph_steps=8)
             arr = np.cumsum(rng.exponential(1.0 / rate, n))
@@ -434,6 +434,7 @@ def main():
     ap.add_argument("--serve-limiter", default="venkatakrishnan", choices=("venkatakrishnan", "scalar"),
                     help="the scalar limiter stalls these steady solves near a 1e-3 drop")
     ap.add_argument("--serve-venkat-k", type=float, default=5.0)
+    ap.add_argument("--serve-solo", type=int, default=1, help="also serve the stream one at a time (0: skip)")
     ap.add_argument("--serve-seed", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters) and exit")

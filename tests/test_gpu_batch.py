@@ -125,6 +125,29 @@ def test_batch_convergence_stop_and_max_steps():
     np.testing.assert_array_equal(b.q(a), s.q)
 
 
+def test_batch_member_error_is_reported_and_isolated():
+    """A member that diverges stops with the error its solo run reports (kind and reference
+    edge or cell); the members beside it run on and equal their solo runs."""
+    cases = _cases()[:3]
+    raw_bad, cfg_bad = _channel(30, 10, 0.05, 7, 0.8, 2.0)
+    cfg_bad["CFL"] = 40.0                     # far above the explicit limit: NaN within a few steps
+    cfg_bad["cflAdjust"] = False
+    solo = Solver(build_mesh(raw_bad), cfg_bad)
+    with pytest.raises(Exception) as e:
+        solo.run(200)
+    b = Batch(graph_steps=8, keep_history=True)
+    ids = [b.add(build_mesh(raw), cfg) for raw, cfg in cases]
+    bad = b.add(build_mesh(raw_bad), cfg_bad)
+    assert b.run(200) == len(cases)
+    st = b.status(bad)
+    assert not st["running"] and int(st["error"]) == e.value.code and int(st["error_index"]) == e.value.index
+    for (raw, cfg), sid in zip(cases, ids):
+        s = Solver(build_mesh(raw), cfg)
+        hs, _, _ = s.run(200)
+        np.testing.assert_array_equal(b.history(sid), hs)
+        np.testing.assert_array_equal(b.q(sid), s.q)
+
+
 def test_batch_admission_by_footprint():
     raw, cfg = _channel(60, 20, 0.05, 5, 0.5, 0.0)
     m = build_mesh(raw)
