@@ -411,6 +411,7 @@ int ebb_batch_add(ebb_batch* b, const ebb_mesh* m, const ebb_config* c, int64_t*
         cfg.stagePath = cfg.stagePath == 3 ? 3 : 1;
         cfg.tileOrder = 2;
         cfg.renumber = false;
+        cfg.spreadTiles = false;   // the members fill the GPU together: full-size tiles
         if (b->tileCells != -1) cfg.tileCells = b->tileCells;
         const int64_t need = estimateDeviceBytes(m->b, cfg);
         if (b->used + need > b->limit) return EBB_ERR_ALLOC;

@@ -85,6 +85,8 @@ struct Config {
     int stagePath = 1;                // 0: gradient, face, update kernels; 1: gradient + face/update tiles;
                                       // 2: one fused kernel per stage; 3: border gradients + gradient tiles
     long long tileCells = -1;         // cells per tile of the tiled paths (-1: automatic)
+    bool spreadTiles = true;          // automatic tiles: smaller tiles on small meshes, so they reach
+                                      // every CU (off for batched members, which fill the GPU together)
     int tileOrder = 2;                // 0: greedily grown regions, 1: Morton order of centroids,
                                       // 2: Morton order of aligned 16-cell blocks
     bool renumber = false;            // tiled paths: number device cells tile by tile

@@ -74,6 +74,7 @@ void serveLoop(ebb_server* sv)
             Ticket& t = sv->tickets[id];
             Config cfg = t.c->c;
             cfg.stagePath = cfg.stagePath == 3 ? 3 : 1;
+            cfg.spreadTiles = false;
             const int64_t need = estimateDeviceBytes(t.m->b, cfg);
             int64_t info[6];
             ebb_batch_info(sv->b, info);

@@ -345,6 +345,13 @@ int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& st
     if (want != -1 && (want < 1 || want > 1024)) return fail(s, EBB_ERR_ARG, "tileCells must be -1 or in 1..1024");
     std::vector<uint32_t> order;
     uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 || c.stagePath == 3 ? 768 : 128);
+    if (want <= 0 && c.spreadTiles && (c.stagePath == 1 || c.stagePath == 3)) {
+        // a mesh too small to give every CU two tiles of 768 cells gets smaller tiles (down to
+        // 32 cells, in whole 16-cell blocks), so it still spreads over the GPU: a tile's
+        // workgroup takes about the same time for any tile size up to one round of lanes
+        const uint64_t share = uint64_t(ref.N) / (2 * uint64_t(std::max(s->cus, 1))) / 16 * 16;
+        T = uint32_t(std::max<uint64_t>(32, std::min<uint64_t>(T, share)));
+    }
     if (c.stagePath == 3 && want <= 0) {
         // automatic gradient tiles: the largest multiple of 16 up to 768 cells whose busiest
         // tile keeps its gradient rows (12 values per own or ring cell) and face results
@@ -538,7 +545,9 @@ int64_t estimateDeviceBytes(const BuiltMesh& b, const Config& c)
     // gradient tiles: an LDS-row pair per face record, each tile's ring and the border (~0.3 N each)
     if (c.stagePath == 3) bytes += int64_t(1.7 * double(N)) * 4 + int64_t(0.6 * double(N)) * 4 + int64_t(1.1 * double(N)) * 4 * S + 4096;
     bytes += ((N + 15) / 16) * 64;                         // RMS leaf sums
-    const int64_t tiles = N / (c.stagePath == 3 ? 250 : 700) + 1;
+    // tiles: ~700 cells each, or at least two per CU on small meshes (down to 32 cells)
+    const int64_t tiles = std::max<int64_t>(N / (c.stagePath == 3 ? 250 : 700) + 1,
+                                            c.spreadTiles ? std::min<int64_t>(N / 32 + 1, 512) : 0);
     bytes += (2 * std::max<int64_t>((N + 255) / 256, tiles) + 2) * 4 * 8;   // stage partials
     bytes += 256 * HIST_FIELDS * 8;
     const int64_t nchunks = (C / 16 + RMS_CHUNK_BLOCKS - 1) / RMS_CHUNK_BLOCKS + 1;
@@ -777,9 +786,11 @@ int createSolver(const ebb_mesh* m, const Config& cfg, int device, ebb_solver** 
     s->cfg = cfg;
     s->device = device;
     {
-        int lds = 0;
+        int lds = 0, cus = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
             s->ldsPerBlock = lds;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+            s->cus = cus;
     }
     // gradient tiles need gradients (order 2) in rows of 4 scalar limiters: other
     // configurations take the face-tile path

@@ -49,6 +49,9 @@ struct DBuf {
         if (count == 0) count = 1;
         hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
         if (e == hipSuccess) e = hipMemset(p, 0, count * sizeof(T));
+        // the fill runs on the null stream, which the library's non-blocking streams do not
+        // wait for: it must finish before any of them writes the buffer
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         return e;
     }
     hipError_t upload(const std::vector<T>& v)
@@ -109,6 +112,7 @@ struct ebb_solver {
     bool useGradTiles = false;
     double gradTileStats[3] = {0, 0, 0};   // largest rows per tile, ring cells summed over tiles, border cells
     int ldsPerBlock = 65536;               // hipDeviceProp_t::sharedMemPerBlock of the solver's device
+    int cus = 256;                         // its compute units
     // device cell numbering of the tiled paths: device cell d is reference cell
     // refOf[d] (empty: identical numbering); d_devOf maps back for the RMS sums
     std::vector<uint32_t> refOf, devOf;
