@@ -207,6 +207,10 @@ __host__ __device__ inline TileDyn tileDyn(uint32_t nG, uint32_t nL, int LW)
 // tile, so a workgroup's first loads (two rounds of cell ids, one of face records) need not
 // wait for the descriptor and stay in bounds.
 constexpr uint32_t FACE_TILE_LANES = 256;
+// LDS per face slot of a face tile's results: len*flux and len*sMax in FP64 (40 B), or on the
+// mixed path the FP32 flux and sMax with the FP64 length (28 B); a tile's LDS holds
+// faceStride + 1 slots (the padding face last)
+__host__ __device__ constexpr uint32_t faceLdsBytes(int mixed) { return mixed ? 28u : 40u; }
 
 struct FaceTileArgs {
     uint32_t nTiles;

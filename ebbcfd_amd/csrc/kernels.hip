@@ -1512,12 +1512,63 @@ DEV void edgeGeometry(bool interior, double2 p0, double2 p1, double2& n, double2
     mid = make_double2(0.5 * (p0.x + p1.x), 0.5 * (p0.y + p1.y));
 }
 
+// A tile's face results in LDS.  FP64 path: len*flux and len*sMax per face (40 B).  Mixed
+// path: the FP32 flux and sMax and the FP64 length (28 B per face); the cell phase forms the
+// same FP64 products len*F, so its sums are those of the 40-byte layout bit for bit, in
+// 30 % less LDS (four workgroups per CU at 768-cell tiles instead of three).  `cap`: face
+// slots of the largest tile plus the padding face.
+template <int MIX>
+struct FaceLds;
+template <>
+struct FaceLds<0> {
+    double* v;
+    DEV FaceLds(double* base, uint32_t) : v(base) {}
+    DEV void put(uint32_t k, double len, const double F[4], double sMax) const
+    {
+        double* slot = v + 5 * k;
+        slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
+        slot[4] = len * sMax;
+    }
+    DEV void zero(uint32_t k, uint32_t tid) const { if (tid < 5) v[5 * k + tid] = 0.0; }
+    DEV void get(uint32_t k, double o[5]) const
+    {
+        const double* slot = v + 5 * k;
+        o[0] = slot[0]; o[1] = slot[1]; o[2] = slot[2]; o[3] = slot[3]; o[4] = slot[4];
+    }
+};
+template <>
+struct FaceLds<1> {
+    float4* f;
+    double* len;
+    float* s;
+    DEV FaceLds(double* base, uint32_t cap)
+        : f(reinterpret_cast<float4*>(base)), len(reinterpret_cast<double*>(f + cap)), s(reinterpret_cast<float*>(len + cap))
+    {
+    }
+    DEV void put(uint32_t k, double l, const float F[4], float sMax) const
+    {
+        f[k] = make_float4(F[0], F[1], F[2], F[3]);
+        len[k] = l;
+        s[k] = sMax;
+    }
+    DEV void zero(uint32_t k, uint32_t tid) const
+    {
+        if (tid == 0) { f[k] = make_float4(0.f, 0.f, 0.f, 0.f); len[k] = 0.0; s[k] = 0.f; }
+    }
+    DEV void get(uint32_t k, double o[5]) const
+    {
+        const float4 F = f[k];
+        const double l = len[k];
+        o[0] = l * double(F.x); o[1] = l * double(F.y); o[2] = l * double(F.z); o[3] = l * double(F.w);
+        o[4] = l * double(s[k]);
+    }
+};
 // The cell phase of a face tile: each own cell sums its faces' results from LDS in its own
 // face order and applies the stage update; the workgroup's (min dt, max |R|) pair goes to
 // partial slot `pslot`.  giNext/giNext2: the device ids of the first two cell rounds.
-template <int S>
+template <int S, int MIX>
 DEV void tileCellPhase(const DevMesh& m, const PhysParams& p, const StageArgs& a, const FaceTileArgs& t, const uint4& d,
-                       uint32_t pslot, const double* fsd, uint32_t giNext, uint32_t giNext2)
+                       uint32_t pslot, const FaceLds<MIX>& fl, uint32_t giNext, uint32_t giNext2)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t nC = d.x;
@@ -1571,7 +1622,8 @@ DEV void tileCellPhase(const DevMesh& m, const PhysParams& p, const StageArgs& a
 #pragma unroll
             for (int j = 0; j < S; j++) {
                 const uint32_t fe = cfC[j];
-                const double* lf = fsd + 5 * (fe & 0x7fffu);
+                double lf[5];
+                fl.get(fe & 0x7fffu, lf);
                 if (fe >> 15) { r[0] = r[0] - lf[0]; r[1] = r[1] - lf[1]; r[2] = r[2] - lf[2]; r[3] = r[3] - lf[3]; }
                 else { r[0] = r[0] + lf[0]; r[1] = r[1] + lf[1]; r[2] = r[2] + lf[2]; r[3] = r[3] + lf[3]; }
                 sAve += lf[4];
@@ -1602,6 +1654,7 @@ DEV void faceTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
 {
     using R = typename std::conditional<MIX == 1, float, double>::type;
     const FaceSrc<MIX> src{a};
+    const FaceLds<MIX> fl(fsd, t.faceStride + 1);
     const uint32_t tid = threadIdx.x;
     // The first two cell rounds' ids are loaded ahead of the faces; the next round's
     // face record is loaded one round ahead, so each round waits for one memory latency
@@ -1614,7 +1667,7 @@ DEV void faceTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
     const uint4 d = t.desc[tile];
     // the loop's active flag is read after those loads are issued (all in bounds)
     if (a.checkActive && !a.sc->active) return;
-    const uint32_t nC = d.x, nF = d.y;
+    const uint32_t nF = d.y;
     for (uint32_t k = tid; k < nF; k += BLK) {
         const uint4 r = rNext;   // cell a, cell b, node 0, node 1
         if (k + BLK < nF) rNext = t.fref[d.w + k + BLK];
@@ -1662,12 +1715,11 @@ DEV void faceTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
         if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
             atomicMin(&a.sc->errEdge,
                       (uint64_t(a.stageId) << 32) | (isB ? uint64_t(b) : uint64_t(m.Fb) + t.fid[d.w + k]));
-        double* slot = fsd + 5 * k;
-        slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
-        slot[4] = len * sMax;
+        if constexpr (MIX == 1) fl.put(k, len, Fr, sMr);
+        else fl.put(k, len, F, sMax);
     }
-    if (tid < 5) fsd[5 * nF + tid] = 0.0;   // padding face
-    tileCellPhase<S>(m, p, a, t, d, pslot, fsd, giNext, giNext2);
+    fl.zero(nF, tid);   // padding face
+    tileCellPhase<S, MIX>(m, p, a, t, d, pslot, fl, giNext, giNext2);
 }
 
 template <int S, int FLUX, int VISC, int ORDER, int LP, int MIX>
@@ -1791,7 +1843,7 @@ DEV void gradTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
     if (a.checkActive && !a.sc->active) return;
     const uint32_t nC = d.x, nF = d.y;
     R* gl = reinterpret_cast<R*>(lds);
-    double* fsd = lds + t.glDoubles;
+    const FaceLds<MIX> fl(lds + t.glDoubles, t.faceStride + 1);
     // the ring's rows first: their loads are in flight while the own gradients compute
     for (uint32_t h = tid; h < nH; h += BLK) {
         const uint32_t y = t.halo[size_t(tile) * t.haloStride + h];
@@ -1872,12 +1924,11 @@ DEV void gradTileBody(const DevMesh& m, const PhysParams& p, const StageArgs& a,
         if (a.recordErrors && (isnan(F[0]) || isnan(F[1]) || isnan(F[2]) || isnan(F[3])))
             atomicMin(&a.sc->errEdge,
                       (uint64_t(a.stageId) << 32) | (isB ? uint64_t(b) : uint64_t(m.Fb) + t.fid[d.w + k]));
-        double* slot = fsd + 5 * k;
-        slot[0] = len * F[0]; slot[1] = len * F[1]; slot[2] = len * F[2]; slot[3] = len * F[3];
-        slot[4] = len * sMax;
+        if constexpr (MIX == 1) fl.put(k, len, Fr, sMr);
+        else fl.put(k, len, F, sMax);
     }
-    if (tid < 5) fsd[5 * nF + tid] = 0.0;   // padding face
-    tileCellPhase<S>(m, p, a, t, d, pslot, fsd, giNext, giNext2);
+    fl.zero(nF, tid);   // padding face
+    tileCellPhase<S, MIX>(m, p, a, t, d, pslot, fl, giNext, giNext2);
 }
 
 // FP64: at most 168 VGPRs, three waves per SIMD (the LDS of three workgroups fits a CU)

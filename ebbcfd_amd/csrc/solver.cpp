@@ -360,7 +360,7 @@ int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& st
         const uint32_t rowBytes = 12 * (c.precision ? 4 : 8);
         for (;;) {
             tileOrder(ref, T, c.tileOrder, order, start);
-            const uint32_t need = gradTileLds(ref, order, start, rowBytes);
+            const uint32_t need = gradTileLds(ref, order, start, rowBytes, faceLdsBytes(c.precision));
             if (need <= LDS_TARGET || T <= 16) break;
             const uint32_t next = uint32_t(uint64_t(T) * LDS_TARGET / need) / 16 * 16;
             T = std::max<uint32_t>(16, std::min(next, T - 16));
@@ -369,11 +369,12 @@ int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& st
         // automatic face tiles: the largest multiple of 16 up to 768 cells whose busiest
         // tile keeps its face results (40 B per face) within a third of the CU's 160 KB
         // of LDS, so three workgroups stay resident (768 fits triangles; quads and larger
-        // polygons have more faces per cell)
-        constexpr uint32_t LDS_TARGET = 53248;
+        // polygons have more faces per cell); mixed precision keeps 28 B per face and fits
+        // four workgroups, its registers' limit
+        const uint32_t LDS_TARGET = c.precision ? 40960 : 53248;
         for (;;) {
             tileOrder(ref, T, c.tileOrder, order, start);
-            const uint32_t need = 40 * (maxTileFaces(ref, order, start) + 1);
+            const uint32_t need = faceLdsBytes(c.precision) * (maxTileFaces(ref, order, start) + 1);
             if (need <= LDS_TARGET || T <= 16) break;
             const uint32_t next = uint32_t(uint64_t(T) * LDS_TARGET / need) / 16 * 16;
             T = std::max<uint32_t>(16, std::min(next, T - 16));
@@ -411,7 +412,8 @@ int buildFaceTiles(ebb_solver* s, const Layout& L, const std::vector<uint32_t>& 
     std::string err;
     if (!makeFaceTiles(L, order, start, h, err)) return fail(s, EBB_ERR_MESH, err);
     const bool grad = s->cfg.stagePath == 3;
-    uint32_t lds = 40 * (h.maxF + 1), glDoubles = 0;
+    // face results of the largest tile plus the padding face, 16-byte aligned
+    uint32_t lds = (faceLdsBytes(s->cfg.precision) * (h.faceStride + 1) + 15) / 16 * 16, glDoubles = 0;
     if (grad) {
         if (!addGradTiles(L, h, err)) return fail(s, EBB_ERR_MESH, err);
         const uint32_t rowBytes = 12 * (s->cfg.precision ? 4 : 8);

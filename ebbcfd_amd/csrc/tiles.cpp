@@ -391,7 +391,7 @@ bool addGradTiles(const Layout& L, FaceTileHost& h, std::string& err)
 }
 
 uint32_t gradTileLds(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
-                     uint32_t rowBytes)
+                     uint32_t rowBytes, uint32_t faceBytes)
 {
     const uint32_t N = L.N, S = L.S, F = L.Fi + L.Fb;
     if (N == 0 || start.size() < 2) return 0;
@@ -416,7 +416,7 @@ uint32_t gradTileLds(const Layout& L, const std::vector<uint32_t>& order, const 
         maxF = std::max(maxF, nF);
     }
     // the kernel's layout: the largest tile's rows, then the most faces of any tile
-    const uint64_t lds = (uint64_t(maxRows) * rowBytes + 15) / 16 * 16 + 40ull * (maxF + 1);
+    const uint64_t lds = (uint64_t(maxRows) * rowBytes + 15) / 16 * 16 + (uint64_t(faceBytes) * (maxF + 1) + 15) / 16 * 16;
     return uint32_t(std::min<uint64_t>(lds, 0xFFFFFFFFull));
 }
 

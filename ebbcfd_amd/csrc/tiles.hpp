@@ -72,10 +72,10 @@ bool makeFaceTiles(const Layout& L, const std::vector<uint32_t>& order, const st
 // nC.. in order of first appearance among its faces; the border is every ring cell.
 bool addGradTiles(const Layout& L, FaceTileHost& h, std::string& err);
 // Per-workgroup LDS of gradient tiles with `rowBytes` per gradient row: the rows of the tile
-// with the most rows plus the face results (40 bytes per face and a padding face) of the
+// with the most rows plus the face results (`faceBytes` per face and a padding face) of the
 // tile with the most faces, without building the tiles.
 uint32_t gradTileLds(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start,
-                     uint32_t rowBytes);
+                     uint32_t rowBytes, uint32_t faceBytes);
 
 // Fused-kernel tiles: each cell group with its halo, tables and static data packed.
 bool makeTiles(const Layout& L, const std::vector<uint32_t>& order, const std::vector<uint32_t>& start, TileHost& h,

@@ -76,10 +76,11 @@ int main(int argc, char** argv)
                 }
                 if (std::vector<uint32_t>(ring.begin(), ring.end()) != h.border) bad++;
                 if (h.fslot.size() != h.fref.size() || h.nbrT.size() != h.ids.size() * L.S) bad++;
-                for (const uint32_t rowBytes : {48u, 96u}) {
-                    const uint32_t built = (h.maxG * rowBytes + 15) / 16 * 16 + 40 * (maxF + 1);
-                    if (gradTileLds(L, order, start, rowBytes) != built) bad++;
-                }
+                for (const uint32_t rowBytes : {48u, 96u})
+                    for (const uint32_t faceBytes : {28u, 40u}) {
+                        const uint32_t built = (h.maxG * rowBytes + 15) / 16 * 16 + (faceBytes * (maxF + 1) + 15) / 16 * 16;
+                        if (gradTileLds(L, order, start, rowBytes, faceBytes) != built) bad++;
+                    }
                 std::printf("%s T %d order %d: tiles %u, border %zu of %u cells, %s\n", argv[a], T, kind, h.nTiles,
                             h.border.size(), L.N, bad == before ? "ok" : "VIOLATIONS");
             }
