@@ -764,6 +764,7 @@ int buildDevice(ebb_solver* s)
         }
     }
     HIPCHK(hipMemcpy(s->d_qA.p, q0.data(), C * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(nullptr));   // landed before the solver's stream reads it
     s->cur = 0;
     refreshShadow(s);
     s->t0 = 0;

@@ -58,6 +58,9 @@ struct DBuf {
     {
         hipError_t e = alloc(v.size());
         if (e == hipSuccess && !v.empty()) e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        // a copy from pageable memory may return before its transfer lands; the kernels that
+        // read the buffer run on non-blocking streams
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         return e;
     }
     void release()
