@@ -54,6 +54,7 @@ struct ebb_server {
     std::map<int64_t, Ticket> tickets;
     std::vector<int64_t> running;
     int64_t nextTicket = 0, nDone = 0, rounds = 0;
+    int64_t usedBytes = 0;   // device bytes the batch has admitted, as of the loop's last change
     double gpuSeconds = 0;
     bool stop = false;
     std::thread loop;
@@ -107,6 +108,9 @@ void serveLoop(ebb_server* sv)
             t2.state = T_RUNNING;
             t2.tAdmit = sv->now();
             sv->running.push_back(id);
+            int64_t inf[6];
+            ebb_batch_info(sv->b, inf);   // the loop thread owns the batch
+            sv->usedBytes = inf[2];
         }
         if (sv->running.empty()) continue;
         std::vector<int64_t> run = sv->running;
@@ -138,7 +142,10 @@ void serveLoop(ebb_server* sv)
             ebb_batch_remove(sv->b, tp->member);
             finished.emplace_back(id, std::move(q));
         }
+        int64_t inf[6];
+        ebb_batch_info(sv->b, inf);
         lk.lock();
+        sv->usedBytes = inf[2];
         sv->gpuSeconds += gs;
         sv->rounds++;
         for (auto& f : finished) {
@@ -262,13 +269,13 @@ int ebb_server_stats(ebb_server* sv, double out[12])
                 failed++;
             }
         }
-        int64_t info[6];
-        ebb_batch_info(sv->b, info);
+        // the batch itself belongs to the loop thread: its admitted bytes are read from the
+        // copy the loop keeps under the lock
         out[0] = double(sv->tickets.size()); out[1] = double(done); out[2] = double(failed);
         out[3] = double(sv->running.size()); out[4] = double(sv->queue.size());
         out[5] = quantile(lat, 0.5); out[6] = quantile(lat, 0.99);
         out[7] = done && last > first ? double(done) / (last - first) : 0.0;
-        out[8] = double(info[2]); out[9] = sv->gpuSeconds; out[10] = double(sv->rounds); out[11] = sv->now();
+        out[8] = double(sv->usedBytes); out[9] = sv->gpuSeconds; out[10] = double(sv->rounds); out[11] = sv->now();
         return EBB_OK;
     });
 }
