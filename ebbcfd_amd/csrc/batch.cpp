@@ -383,7 +383,9 @@ int ebb_estimate_device_bytes(const ebb_mesh* m, const ebb_config* c, int64_t* b
 int ebb_batch_create(int device, int64_t device_bytes_limit, int graph_steps, int keep_history, ebb_batch** out)
 {
     return guarded([&]() -> int {
-        if (!out || device_bytes_limit <= 0 || graph_steps < 0 || (graph_steps & 1)) return EBB_ERR_ARG;
+        // a graph may not outrun a member's history ring (runs are cut to whole graphs of it)
+        if (!out || device_bytes_limit <= 0 || graph_steps < 0 || (graph_steps & 1) || graph_steps > HIST_RING_ROWS)
+            return EBB_ERR_ARG;
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return EBB_ERR_DEVICE;
         auto b = std::make_unique<ebb_batch>();

@@ -279,6 +279,7 @@ struct DiagArgs {
     DevScalars* sc;
 };
 constexpr int HIST_FIELDS = 10;
+constexpr int HIST_RING_ROWS = 256;   // rows of a solver's device history ring
 
 // Batched runs (many meshes per launch): the per-mesh arguments of every kernel of a
 // time-loop step, kept in device memory, one record per mesh.  A work item (mesh, local

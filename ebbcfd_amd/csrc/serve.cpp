@@ -57,13 +57,14 @@ struct ebb_server {
     int64_t usedBytes = 0;   // device bytes the batch has admitted, as of the loop's last change
     double gpuSeconds = 0;
     bool stop = false;
+    bool dead = false;       // the loop thread ended on an exception: no request is served
     std::thread loop;
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
 
 namespace {
 
-void serveLoop(ebb_server* sv)
+void serveRounds(ebb_server* sv)
 {
     std::unique_lock<std::mutex> lk(sv->mu);
     for (;;) {
@@ -161,6 +162,29 @@ void serveLoop(ebb_server* sv)
     }
 }
 
+// The loop thread.  An exception (host memory for a result, say) ends the loop and fails
+// every request not yet done; their solves stay in the batch until the server is destroyed.
+void serveLoop(ebb_server* sv)
+{
+    try {
+        serveRounds(sv);
+    } catch (...) {
+        std::lock_guard<std::mutex> g(sv->mu);   // the unwound rounds released it
+        sv->dead = true;
+        for (auto& kv : sv->tickets) {
+            Ticket& t = kv.second;
+            if (t.state != T_QUEUED && t.state != T_RUNNING) continue;
+            t.state = T_FAILED;
+            t.err = EBB_ERR_ALLOC;
+            t.tDone = sv->now();
+            sv->nDone++;
+        }
+        sv->queue.clear();
+        sv->running.clear();
+        sv->doneCv.notify_all();
+    }
+}
+
 double quantile(std::vector<double> v, double p)
 {
     if (v.empty()) return 0;
@@ -199,6 +223,7 @@ int ebb_server_submit(ebb_server* sv, const ebb_mesh* m, const ebb_config* c, in
     return guarded([&]() -> int {
         if (!sv || !m || !c || !ticket) return EBB_ERR_ARG;
         std::lock_guard<std::mutex> g(sv->mu);
+        if (sv->dead) return EBB_ERR_STATE;
         const int64_t id = sv->nextTicket++;
         Ticket& t = sv->tickets[id];
         t.m = m;

@@ -127,7 +127,7 @@ struct ebb_solver {
     DevScalars* d_sc = nullptr;
     DevScalars* h_sc = nullptr;  // pinned
     double* h_hist = nullptr;    // pinned
-    int64_t histCap = 256;
+    int64_t histCap = HIST_RING_ROWS;
     // time-loop bookkeeping
     int cur = 0;                 // 0: qA holds the current state, 1: qB
     double4* lastInput = nullptr;

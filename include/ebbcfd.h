@@ -225,8 +225,8 @@ typedef struct ebb_batch ebb_batch;
 int ebb_estimate_device_bytes(const ebb_mesh* m, const ebb_config* c, int64_t* bytes);
 
 /* A batch on HIP device `device` admitting solvers while their device bytes stay within
- * device_bytes_limit.  graph_steps: steps per captured HIP graph (even; 0: plain
- * launches).  keep_history: also copy each step's history row to the host. */
+ * device_bytes_limit.  graph_steps: steps per captured HIP graph (even, at most 256; 0:
+ * plain launches).  keep_history: also copy each step's history row to the host. */
 int ebb_batch_create(int device, int64_t device_bytes_limit, int graph_steps, int keep_history, ebb_batch** out);
 
 /* Admit a solve (ufvmSetup on its own device state).  EBB_ERR_ALLOC when its footprint
@@ -275,7 +275,9 @@ void ebb_batch_destroy(ebb_batch* b);
 typedef struct ebb_server ebb_server;
 int ebb_server_create(int device, int64_t device_bytes_limit, int max_solves, int steps_per_round, int graph_steps,
                       int keep_results, ebb_server** out);
-/* Queue a request; *ticket names it.  Its arrival time is now. */
+/* Queue a request; *ticket names it.  Its arrival time is now.  EBB_ERR_STATE once the
+ * loop has ended on a host error (every request not yet done then failed with
+ * EBB_ERR_ALLOC). */
 int ebb_server_submit(ebb_server* sv, const ebb_mesh* m, const ebb_config* c, int64_t* ticket);
 /* out[0..15]: state (0 queued, 1 running, 2 done, 3 failed), error code, arrival, admission
  * and completion times (s since the server started; -1 before), interior cells, then the

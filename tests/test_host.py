@@ -152,6 +152,11 @@ def test_c_abi_error_codes(tmp_path):
     for name in ("ebb_state_upload", "ebb_state_download", "ebb_run_reset"):
         assert getattr(L, name)(None, *([None] if name != "ebb_run_reset" else [0.0, 0.0])) == 1
     assert L.ebb_history(None, 0, 0, None) == 1
+    # batch arguments are checked before any device call: odd graph lengths, and graphs
+    # longer than the 256-row history ring a run is cut to
+    for steps in (-2, 3, 258):
+        assert L.ebb_batch_create(0, C.c_int64(1 << 30), steps, 0, C.byref(vp)) == 1
+    assert L.ebb_server_create(0, C.c_int64(1 << 30), 0, 100, 8, 0, C.byref(vp)) == 1
 
     def create(nodes, eoff, enodes, bnodes):
         nodes = np.ascontiguousarray(nodes, np.float64)
