@@ -334,6 +334,59 @@ def copy_calibration():
     log("copied %d bytes 5 times" % (x.numel() * 8))
 
 
+def ramp_probe(args):
+    """Short timed windows after GPU idle: the workload's solver, `--warmup` steps, then for
+    each idle gap a `--steps`-step run after the gap and another right after it.  The
+    headline's timed region starts `--warmup` steps after seconds of host-only mesh build,
+    so a short run shows how much of it falls in the GPU's clock ramp."""
+    from ebbcfd_amd import Solver, build_mesh, meshgen
+    raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1)
+    cfg = cylinder_config()
+    cfg["extensions"] = {"stagePath": args.stage_path, "limiter": args.limiter, "venkatK": args.venkat_k}
+    s = Solver(build_mesh(raw), cfg)
+    K = args.steps
+
+    def timed(x=None):
+        x = x or s
+        t0 = time.perf_counter()
+        x.run(K)
+        return round((time.perf_counter() - t0) / K * 1e3, 4)
+
+    s.run(args.warmup)
+    rows = [{"idle_s": "mesh build", "first_ms_per_step": timed(), "next_ms_per_step": timed()}]
+    s.run(100)
+    rows.append({"idle_s": 0.0, "first_ms_per_step": timed(), "next_ms_per_step": timed()})
+    for idle in (0.01, 0.1, 0.5, 3.0):
+        time.sleep(idle)
+        rows.append({"idle_s": idle, "first_ms_per_step": timed(), "next_ms_per_step": timed()})
+    for r in rows:
+        log("after %s idle: %.4f ms/step, then %.4f" % (r["idle_s"], r["first_ms_per_step"], r["next_ms_per_step"]))
+    # the fixed cost of one run call: back-to-back runs of 1-64 steps (minimum of three each)
+    s.run(100)
+    sweep = {}
+    for n in (1, 2, 4, 8, 16, 32, 64):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            s.run(n)
+            ts.append(time.perf_counter() - t0)
+        sweep[n] = round(min(ts) * 1e3, 4)
+    x = np.array(list(sweep), float)
+    slope, icpt = np.polyfit(x, np.array(list(sweep.values())), 1)
+    log("runs of 1-64 steps: %s ms; fit %.4f ms per call + %.4f ms per step" % (sweep, icpt, slope))
+    # a second solver, set up right after the first one's steps (the GPU idles meanwhile)
+    s.run(20)
+    s2 = Solver(s.mesh, cfg)
+    s2.run(args.warmup)
+    fresh = {"first_ms_per_step": timed(s2)}
+    fresh["next_ms_per_step"] = timed(s2)
+    log("fresh solver after %d warm-up steps: %.4f ms/step, then %.4f"
+        % (args.warmup, fresh["first_ms_per_step"], fresh["next_ms_per_step"]))
+    print(json.dumps({"probe": "clock ramp", "steps_per_window": K, "warmup": args.warmup, "rows": rows,
+                      "run_ms_by_steps": sweep, "fit_ms_per_call": round(float(icpt), 4),
+                      "fit_ms_per_step": round(float(slope), 4), "second_solver": fresh}), flush=True)
+
+
 def host_threads():
     """CPUs the mesh build may use: affinity set, capped by OMP_NUM_THREADS and 32."""
     t = len(os.sched_getaffinity(0))
@@ -441,7 +494,12 @@ def main():
     ap.add_argument("--ingest", action="store_true",
                     help="time the mesh ingest (gmsh read + geometry precompute) of the workload's mesh instead")
     ap.add_argument("--ingest-repeats", type=int, default=3)
+    ap.add_argument("--ramp-probe", action="store_true",
+                    help="only time --steps-step windows after GPU idle gaps of 0-3 s and exit")
     args = ap.parse_args()
+    if args.ramp_probe:
+        ramp_probe(args)
+        return
     if args.ingest:
         ingest_bench(args)
         return
