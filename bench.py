@@ -301,7 +301,7 @@ def dropin_leg(s, N, stages, ms_per_stage_loop):
 def c1_leg(steps):
     """Latency of the manufactured-solution Euler workload: 8,192 jittered triangles,
     Roe, scalar limiter, RK4, CFL 0.5 -- cache-resident, reported for latency."""
-    from ebbcfd_amd import Solver, build_mesh, meshgen
+    from ebbcfd_amd import Batch, Solver, build_mesh, meshgen
     raw = meshgen.mms_square(64, 0.1, 1)
     q = [0.1, 0.0, 1.1, 1.0]
     cfg = {"mesh": "mms.msh", "dt": 1.0e-5, "tEnd": 1.0e9, "flux": "roeFlux", "plotIter": -1, "saveIter": -1,
@@ -309,17 +309,27 @@ def c1_leg(steps):
            "initialConditions": q, "viscosity": False,
            "physicalConfig": {"Pr": 0.71, "R": 1.0, "gamma": 1.4, "Re": 1000.0, "L": 1.0},
            "boudaryConditions": [{"tag": "Freestream", "type": "dirichlet", "q": q}], "extensions": {"mms": True}}
-    s = Solver(build_mesh(raw), cfg)
+    mesh = build_mesh(raw)
+    s = Solver(mesh, cfg)
     s.run(50)
     t0 = time.perf_counter()
     h, _, _ = s.run(steps)
     el = time.perf_counter() - t0
     assert len(h) == steps
     tk = s.time_steps(50)
-    log("C1: %d RK4 steps of 8192 cells: %.2f us/step (stage kernels %.2f us/step)"
-        % (steps, el / steps * 1e6, tk["step_ms"] * 1e3))
+    # the same solve replayed as 8-step HIP graphs (a one-member batch runs the same kernels)
+    b = Batch(graph_steps=8)
+    b.add(mesh, cfg)
+    b.run(56)
+    t0 = time.perf_counter()
+    b.run(steps)
+    el_graph = time.perf_counter() - t0
+    del b
+    log("C1: %d RK4 steps of 8192 cells: %.2f us/step (stage kernels %.2f us/step; replayed as 8-step graphs "
+        "%.2f us/step)" % (steps, el / steps * 1e6, tk["step_ms"] * 1e3, el_graph / steps * 1e6))
     return {"workload": "mms_euler_64x64x2_jittered_rk4", "cells": 8192, "steps": steps,
             "us_per_step": round(el / steps * 1e6, 2), "stage_kernels_us_per_step": round(tk["step_ms"] * 1e3, 2),
+            "graph_replay_us_per_step": round(el_graph / steps * 1e6, 2),
             "Mcell_stage_per_s": round(8192 * 4 * steps / el / 1e6, 2)}
 
 

@@ -137,10 +137,35 @@ MemberArgs memberArgs(ebb_solver* s)
     return M;
 }
 
+// Automatic tiles follow the cells running together: a member whose tiles were sized for
+// less than half or more than twice the cells now sharing the launches is cut again (the
+// results do not depend on the tile size).  Only members below ~400k cells ever change.
+int followSpread(ebb_batch* b)
+{
+    if (b->tileCells != -1) return EBB_OK;
+    int64_t total = 0;
+    for (auto& mb : b->members) if (mb.running) total += int64_t(mb.s->N);
+    for (auto& mb : b->members) {
+        ebb_solver* s = mb.s;
+        if (!mb.running || s->cfg.tileCells != -1 || !s->cfg.spreadTiles) continue;
+        const uint32_t was = spreadTileCells(uint64_t(s->N) + uint64_t(std::max<int64_t>(s->cfg.spreadOver, 0)), s->cus);
+        const uint32_t now = spreadTileCells(uint64_t(total), s->cus);
+        if (now <= 2 * was && was <= 2 * now) continue;
+        s->cfg.spreadOver = total - int64_t(s->N);
+        int rc = retileFaceTiles(s, -1);
+        if (rc) return rc;
+        const int64_t nb = std::max(mb.bytes, s->deviceBytes);
+        b->used += nb - mb.bytes;
+        mb.bytes = nb;
+    }
+    return EBB_OK;
+}
+
 // Work lists and launch records of the running members.
 int rebuild(ebb_batch* b)
 {
     dropGraph(b);
+    { int rc = followSpread(b); if (rc) return rc; }
     b->groups.clear();
     b->hmem.clear();
     b->slotOf.clear();
@@ -413,7 +438,12 @@ int ebb_batch_add(ebb_batch* b, const ebb_mesh* m, const ebb_config* c, int64_t*
         cfg.stagePath = cfg.stagePath == 3 ? 3 : 1;
         cfg.tileOrder = 2;
         cfg.renumber = false;
-        cfg.spreadTiles = false;   // the members fill the GPU together: full-size tiles
+        // automatic tiles are sized for the cells of the running members and this one
+        // together: a member admitted alone or beside small ones spreads over the GPU as a
+        // solo run does; one joining members that already fill it takes full-size tiles
+        cfg.spreadTiles = true;
+        cfg.spreadOver = 0;
+        for (auto& mb : b->members) if (mb.running) cfg.spreadOver += int64_t(mb.s->N);
         if (b->tileCells != -1) cfg.tileCells = b->tileCells;
         const int64_t need = estimateDeviceBytes(m->b, cfg);
         if (b->used + need > b->limit) return EBB_ERR_ALLOC;

@@ -86,7 +86,9 @@ struct Config {
                                       // 2: one fused kernel per stage; 3: border gradients + gradient tiles
     long long tileCells = -1;         // cells per tile of the tiled paths (-1: automatic)
     bool spreadTiles = true;          // automatic tiles: smaller tiles on small meshes, so they reach
-                                      // every CU (off for batched members, which fill the GPU together)
+                                      // every CU
+    int64_t spreadOver = 0;           // cells of other solves whose tiles share this one's launches
+                                      // (batched members): counted with the mesh's own when spreading
     int tileOrder = 2;                // 0: greedily grown regions, 1: Morton order of centroids,
                                       // 2: Morton order of aligned 16-cell blocks
     bool renumber = false;            // tiled paths: number device cells tile by tile

@@ -75,8 +75,8 @@ void serveRounds(ebb_server* sv)
             const int64_t id = sv->queue.front();
             Ticket& t = sv->tickets[id];
             Config cfg = t.c->c;
-            cfg.stagePath = cfg.stagePath == 3 ? 3 : 1;
-            cfg.spreadTiles = false;
+            cfg.stagePath = cfg.stagePath == 3 ? 3 : 1;   // as ebb_batch_add will run it
+            cfg.spreadTiles = true;
             const int64_t need = estimateDeviceBytes(t.m->b, cfg);
             int64_t info[6];
             ebb_batch_info(sv->b, info);

@@ -333,6 +333,14 @@ int clearErrors(ebb_solver* s)
 // reference numbering; with renumbering the interior cells are then numbered tile
 // by tile.  `lay` receives the device-order layout, tile t is the device cells
 // tileCellsOrder[start[t] .. start[t+1]).
+// Down to 32 cells, in whole 16-cell blocks: a tile's workgroup takes about the same time
+// for any tile size up to one round of lanes, so smaller tiles only buy more workgroups.
+uint32_t spreadTileCells(uint64_t cells, int cus)
+{
+    const uint64_t share = cells / (2 * uint64_t(std::max(cus, 1))) / 16 * 16;
+    return uint32_t(std::max<uint64_t>(32, std::min<uint64_t>(768, share)));
+}
+
 int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& start,
               std::vector<uint32_t>& tileCellsOrder)
 {
@@ -346,11 +354,10 @@ int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& st
     std::vector<uint32_t> order;
     uint32_t T = want > 0 ? uint32_t(want) : (c.stagePath == 1 || c.stagePath == 3 ? 768 : 128);
     if (want <= 0 && c.spreadTiles && (c.stagePath == 1 || c.stagePath == 3)) {
-        // a mesh too small to give every CU two tiles of 768 cells gets smaller tiles (down to
-        // 32 cells, in whole 16-cell blocks), so it still spreads over the GPU: a tile's
-        // workgroup takes about the same time for any tile size up to one round of lanes
-        const uint64_t share = uint64_t(ref.N) / (2 * uint64_t(std::max(s->cus, 1))) / 16 * 16;
-        T = uint32_t(std::max<uint64_t>(32, std::min<uint64_t>(T, share)));
+        // a mesh too small to give every CU two tiles of 768 cells gets smaller tiles, so it
+        // still spreads over the GPU.  A batched member counts the cells running beside it:
+        // the launches are shared
+        T = std::min(T, spreadTileCells(uint64_t(ref.N) + uint64_t(std::max<int64_t>(c.spreadOver, 0)), s->cus));
     }
     if (c.stagePath == 3 && want <= 0) {
         // automatic gradient tiles: the largest multiple of 16 up to 768 cells whose busiest

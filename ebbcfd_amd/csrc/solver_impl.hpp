@@ -182,6 +182,8 @@ int checkErrors(ebb_solver* s);
 int64_t estimateDeviceBytes(const BuiltMesh& b, const Config& c);
 // face tiles cut again with tileCells per tile (batched runs' tuning)
 int retileFaceTiles(ebb_solver* s, long long tileCells);
+// automatic face-tile size that spreads `cells` over two tiles per CU (16-cell blocks, 32-768)
+uint32_t spreadTileCells(uint64_t cells, int cus);
 int fail(ebb_solver* s, int code, const std::string& msg);
 // creates a solver (ufvmSetup + upload) as ebb_solver_create does, with config overrides applied
 int createSolver(const ebb_mesh* m, const Config& c, int device, ebb_solver** out);

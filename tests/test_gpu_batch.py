@@ -103,6 +103,40 @@ def test_batch_members_join_and_leave():
         np.testing.assert_array_equal(b.q(sid), s.q)
 
 
+def test_batch_member_retiled_as_batch_grows_and_shrinks():
+    """Automatic tiles follow the cells running together: a small solve admitted alone gets
+    32-cell tiles (as a solo run); when a ~60k-cell solve joins, the cells sharing the
+    launches grow more than twofold and the small solve is cut into larger tiles; when it
+    leaves, back.  Both must still equal their solo runs bit for bit."""
+    def spread(cells, cus):   # cells per automatic face tile (two tiles per CU, 16-cell blocks, 32-768)
+        return max(32, min(768, cells // (2 * cus) // 16 * 16))
+
+    raw_s, cfg_s = _channel(40, 12, 0.05, 5, 0.5, 2.0)
+    raw_b, cfg_b = _channel(300, 100, 0.06, 6, 0.7, -1.0)
+    ms, mb = build_mesh(raw_s), build_mesh(raw_b)
+    # the join and the leave cross the twofold re-tiling threshold on any GPU of 32 to 304
+    # CUs (MI355X: 256, where the small solve goes from 32-cell tiles to 112 and back)
+    for cus in (32, 64, 128, 256, 304):
+        alone, joined = spread(ms.n_interior, cus), spread(ms.n_interior + mb.n_interior, cus)
+        assert joined > 2 * alone, (cus, alone, joined)
+    b = Batch(graph_steps=8, keep_history=True)
+    small = b.add(ms, cfg_s)
+    b.run(30)
+    big = b.add(mb, cfg_b)
+    b.run(30)
+    hb_big, q_big = b.history(big), b.q(big)
+    b.remove(big)
+    b.run(30)
+    s = Solver(build_mesh(raw_s), cfg_s)
+    hs, _, _ = s.run(90)
+    np.testing.assert_array_equal(b.history(small), hs)
+    np.testing.assert_array_equal(b.q(small), s.q)
+    s = Solver(build_mesh(raw_b), cfg_b)
+    hs, _, _ = s.run(30)
+    np.testing.assert_array_equal(hb_big, hs)
+    np.testing.assert_array_equal(q_big, s.q)
+
+
 def test_batch_convergence_stop_and_max_steps():
     raw, cfg = _channel(30, 10, 0.05, 3, 0.5, 1.0, limited=False, localTimestep=True)
     cfg["extensions"] = {"rmsDrop": 1e-3}
