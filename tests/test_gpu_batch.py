@@ -13,7 +13,7 @@ import pytest
 
 from ebbcfd_amd import Batch, Solver, build_mesh, estimate_device_bytes, meshgen
 
-from helpers import channel_case, cylinder_case, mms_case
+from helpers import channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mixed_mesh, mms_case
 
 pytestmark = pytest.mark.gpu
 
@@ -73,6 +73,37 @@ def test_batch_grad_tiles_equal_solo_runs(precision):
         c["extensions"] = dict(c.get("extensions", {}), stagePath="gradTiles" if k % 3 else "faceTiles",
                                precision=precision)
         cfgs.append(c)
+    b = Batch(graph_steps=8, keep_history=True)
+    ids = [b.add(build_mesh(raw), c) for (raw, _), c in zip(cases, cfgs)]
+    assert b.run(steps) == len(cases)
+    for (raw, _), c, sid in zip(cases, cfgs, ids):
+        s = Solver(build_mesh(raw), c)
+        hs, t, dt = s.run(steps)
+        np.testing.assert_array_equal(b.history(sid), hs)
+        np.testing.assert_array_equal(b.q(sid), s.q)
+        st = b.status(sid)
+        assert (st["t"], st["dt"]) == (t, dt)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_mixed_element_shapes_equal_solo_runs(precision):
+    """One batch holding meshes of every cell shape the device lays out: triangles (3 slots
+    per cell), triangles beside quads (4), pentagons at hanging nodes (6) and the reference's
+    viscous flat-plate quads (fullState, viscidWall, constP, symmetry; Roe, Euler steps).
+    Each shape is its own launch group; every member still equals its solo run."""
+    cases = [
+        channel_case(integrator="RK4"),
+        (mixed_mesh(), channel_case(integrator="RK2_TVD", viscosity=True)[1]),
+        (hanging_node_mesh(), channel_case(integrator="RK2", viscosity=True)[1]),
+        flat_plate_fixture(0),
+        (mixed_mesh(nx=30, ny=10, seed=9), channel_case(integrator="RK4", flux="rusanovFlux")[1]),
+    ]
+    cfgs = []
+    for raw, cfg in cases:
+        c = copy.deepcopy(cfg)
+        c["extensions"] = dict(c.get("extensions", {}), precision=precision)
+        cfgs.append(c)
+    steps = 40
     b = Batch(graph_steps=8, keep_history=True)
     ids = [b.add(build_mesh(raw), c) for (raw, _), c in zip(cases, cfgs)]
     assert b.run(steps) == len(cases)
