@@ -13,7 +13,8 @@ import pytest
 
 from ebbcfd_amd import Batch, Solver, build_mesh, estimate_device_bytes, meshgen
 
-from helpers import channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mixed_mesh, mms_case
+from helpers import (base_config, channel_case, cylinder_case, flat_plate_fixture, hanging_node_mesh, mixed_mesh,
+                     mms_case, tiny_mesh)
 
 pytestmark = pytest.mark.gpu
 
@@ -23,7 +24,6 @@ def _channel(nx, ny, h, seed, M, aoa, **over):
     q = [M, aoa, 1.0, 1.0]
     bcs = [{"tag": "Inflow", "type": "fullState", "q": q}, {"tag": "Outflow", "type": "fullState", "q": q},
            {"tag": "Lower", "type": "inviscidWall", "q": q}, {"tag": "Upper", "type": "inviscidWall", "q": q}]
-    from helpers import base_config
     cfg = base_config(initialConditions=q, boudaryConditions=bcs, forceBoundary="Lower", flux="roeFlux",
                       integrator="RK4", **over)
     return raw, cfg
@@ -114,6 +114,31 @@ def test_batch_mixed_element_shapes_equal_solo_runs(precision):
         np.testing.assert_array_equal(b.q(sid), s.q)
         st = b.status(sid)
         assert (st["t"], st["dt"]) == (t, dt)
+
+
+def test_batch_tiny_members_equal_solo_runs():
+    """Members of one triangle, two triangles and one quad (every cell on the boundary: a
+    tile shorter than a 16-cell block, an RMS sum of the element tail only) beside an
+    ordinary mesh, on both stage paths."""
+    q = [0.3, 10.0, 1.0, 1.0]
+    cases = []
+    for kind in ("tri1", "tri2", "quad1"):
+        for path in ("faceTiles", "gradTiles"):
+            cfg = base_config(initialConditions=q, boudaryConditions=[{"tag": "Farfield", "type": "fullState", "q": q}],
+                              forceBoundary="Farfield", integrator="RK2_TVD")
+            cfg["extensions"] = {"stagePath": path}
+            cases.append((tiny_mesh(kind), cfg))
+    cases.append(channel_case(integrator="RK4"))
+    steps = 20
+    b = Batch(graph_steps=8, keep_history=True)
+    ids = [b.add(build_mesh(raw), cfg) for raw, cfg in cases]
+    assert b.run(steps) == len(cases)
+    for (raw, cfg), sid in zip(cases, ids):
+        s = Solver(build_mesh(raw), cfg)
+        hs, t, dt = s.run(steps)
+        np.testing.assert_array_equal(b.history(sid), hs)
+        np.testing.assert_array_equal(b.q(sid), s.q)
+        assert (b.status(sid)["t"], b.status(sid)["dt"]) == (t, dt)
 
 
 def test_batch_members_join_and_leave():
