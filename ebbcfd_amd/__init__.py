@@ -5,4 +5,4 @@ include/ebbcfd.h); this package is its Python host mirror plus mesh generators.
 """
 from .meshgen import RawMesh, channel_bump, cylinder_ogrid, mms_square, write_gmsh  # noqa: F401
 from .solver import (HIST_FIELDS, Batch, Config, Server, Solver, UMesh2, build_mesh,  # noqa: F401
-                     estimate_device_bytes, import_mesh, limiter1d, load_config)
+                     estimate_device_bytes, import_mesh, limiter1d, load_config, pinned_array)

@@ -19,7 +19,7 @@ SYMBOLS = (
     "ebb_history", "ebb_history_length", "ebb_edge_states", "ebb_cell_gradients", "ebb_save_solution",
     "ebb_save_solution_as",
     "ebb_load_solution", "ebb_write_forces", "ebb_last_error", "ebb_time_steps", "ebb_solver_device_bytes",
-    "ebb_solver_destroy", "ebb_limiter1d", "ebb_estimate_device_bytes", "ebb_batch_create", "ebb_batch_add", "ebb_batch_remove",
+    "ebb_solver_destroy", "ebb_host_alloc", "ebb_host_free", "ebb_limiter1d", "ebb_estimate_device_bytes", "ebb_batch_create", "ebb_batch_add", "ebb_batch_remove",
     "ebb_batch_run", "ebb_batch_status", "ebb_batch_state_download", "ebb_batch_history_length", "ebb_batch_history",
     "ebb_batch_info", "ebb_batch_tune", "ebb_batch_device_bytes", "ebb_batch_destroy", "ebb_server_create",
     "ebb_server_submit", "ebb_server_poll", "ebb_server_result", "ebb_server_wait", "ebb_server_stats",
@@ -90,6 +90,9 @@ def lib():
     L.ebb_solver_destroy.argtypes = [vp]
     L.ebb_solver_destroy.restype = None
     L.ebb_limiter1d.argtypes = [C.c_int, C.c_int, i64, P(d), P(d), P(d)]
+    L.ebb_host_alloc.argtypes = [C.c_size_t, P(vp)]
+    L.ebb_host_free.argtypes = [vp]
+    L.ebb_host_free.restype = None
     L.ebb_estimate_device_bytes.argtypes = [vp, vp, P(i64)]
     L.ebb_batch_create.argtypes = [C.c_int, i64, C.c_int, C.c_int, P(vp)]
     L.ebb_batch_add.argtypes = [vp, vp, vp, P(i64)]

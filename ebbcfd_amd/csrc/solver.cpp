@@ -1127,6 +1127,28 @@ static int downloadCellsF(ebb_solver* s, const float* devPtr, double* host, size
     return EBB_OK;
 }
 
+int ebb_host_alloc(size_t bytes, void** out)
+{
+    return guarded([&]() -> int {
+        if (!out) return EBB_ERR_ARG;
+        *out = nullptr;
+        if (bytes == 0) return EBB_ERR_ARG;
+        void* p = nullptr;
+        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();   // the failure is reported here, not by the next call
+            return EBB_ERR_ALLOC;
+        }
+        *out = p;
+        return EBB_OK;
+    });
+}
+
+void ebb_host_free(void* p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
 int ebb_state_upload(ebb_solver* s, const double* q)
 {
     return guarded([&]() -> int {

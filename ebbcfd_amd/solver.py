@@ -181,9 +181,12 @@ class Solver:
         return out
 
     # operator
-    def ufvm_solver(self, limit=True, dt_update=True, rmax=0.0):
-        """One residual evaluation; returns (R, newDt, Rmax)."""
-        R = np.zeros((self.n_interior, 4))
+    def ufvm_solver(self, limit=True, dt_update=True, rmax=0.0, out=None):
+        """One residual evaluation; returns (R, newDt, Rmax).  `out`: an n_interior x 4
+        float64 array to receive R (e.g. a pinned_array), else a new one."""
+        R = np.zeros((self.n_interior, 4)) if out is None else out
+        if R.shape != (self.n_interior, 4) or R.dtype != np.float64 or not R.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float64 array of shape (%d, 4)" % self.n_interior)
         nd, rm = C.c_double(0), C.c_double(rmax)
         _check(lib().ebb_residual(self.h, int(limit), int(dt_update), _p(R), C.byref(nd), C.byref(rm)),
                "ufvmSolver", self.h)
@@ -255,6 +258,33 @@ class Solver:
     @property
     def device_bytes(self):
         return lib().ebb_solver_device_bytes(self.h)
+
+
+class _HostBlock:
+    """Page-locked host memory from ebb_host_alloc, freed when the last array viewing it goes."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        _check(lib().ebb_host_alloc(C.c_size_t(nbytes), C.byref(p)), "host alloc")
+        self.p = p.value
+
+    def __del__(self):
+        if self.p:
+            lib().ebb_host_free(C.c_void_p(self.p))
+            self.p = None
+
+
+def pinned_array(shape, dtype=np.float64):
+    """A zero-filled numpy array in page-locked host memory (ebb_host_alloc): state uploads,
+    downloads and residuals copied through it run at the PCIe DMA rate."""
+    shape = tuple(np.atleast_1d(shape).tolist())
+    n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    blk = _HostBlock(max(n, 1))
+    buf = (C.c_char * max(n, 1)).from_address(blk.p)
+    buf._block = blk   # the array's base keeps the memory alive
+    arr = np.frombuffer(buf, dtype=np.uint8, count=n).view(dtype).reshape(shape)
+    arr[...] = 0
+    return arr
 
 
 def estimate_device_bytes(mesh, config):

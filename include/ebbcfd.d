@@ -50,6 +50,8 @@ int ebb_save_solution_as(ebb_solver* s, const(char)* path, double t, double dt);
 int ebb_write_forces(ebb_solver* s, const(char)* path, long first, long count, int append);
 int ebb_time_steps(ebb_solver* s, int steps, double* out6);
 long ebb_solver_device_bytes(const(ebb_solver)* s);
+int ebb_host_alloc(size_t bytes, void** p);                                // page-locked q / R arrays
+void ebb_host_free(void* p);
 int ebb_limiter1d(int device, int kind, long n, const(double)* sp, const(double)* sm, double* result);
 
 // batched solves (many meshes per launch) and the serving loop

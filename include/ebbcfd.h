@@ -120,6 +120,14 @@ void ebb_config_destroy(ebb_config* c);
  * to HIP device `device`.  The solver keeps its own copy of mesh and config. */
 int ebb_solver_create(const ebb_mesh* m, const ebb_config* c, int device, ebb_solver** out);
 
+/* Page-locked host memory for the arrays a caller hands to ebb_state_upload,
+ * ebb_state_download and ebb_residual: copies from and to it run at the PCIe DMA rate
+ * instead of through the runtime's staging of pageable memory.  No counterpart in the
+ * reference (a D caller that keeps its integrator allocates its q and R arrays with
+ * these instead of the GC).  *out is NULL and EBB_ERR_ALLOC when it cannot be had. */
+int ebb_host_alloc(size_t bytes, void** out);
+void ebb_host_free(void* p);
+
 /* Interior states in reference cell order, n_interior x 4. */
 int ebb_state_upload(ebb_solver* s, const double* q);
 int ebb_state_download(ebb_solver* s, double* q);

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from ebbcfd_amd import Solver, build_mesh, import_mesh, meshgen
+from ebbcfd_amd import Solver, build_mesh, import_mesh, meshgen, pinned_array
 
 from helpers import ROOT, base_config, channel_case, oracle_cfg, perturb
 
@@ -230,3 +230,26 @@ def test_total_conditions_inflow_without_root_stops_like_reference():
     with pytest.raises(Exception) as er:
         s2.run(5)
     assert er.value.code == 5 and er.value.index == eo.value.index
+
+
+def test_pinned_host_buffers_give_the_same_results():
+    """The drop-in at ufvmSolver with page-locked host arrays (ebb_host_alloc): the state
+    uploaded from one and R downloaded into one equal the pageable path bit for bit, on
+    repeated calls into the same buffers."""
+    raw, cfg = channel_case(nx=40, ny=12, viscosity=True)
+    s = Solver(build_mesh(raw), cfg)
+    q0 = perturb(s.q, 0.02, seed=3)
+    qp, Rp = pinned_array((s.n_interior, 4)), pinned_array((s.n_interior, 4))
+    assert not Rp.any()
+    for k in range(3):
+        qk = perturb(q0, 0.01, seed=10 + k)
+        s.q = qk
+        R, nd, rm = s.ufvm_solver()
+        qp[...] = qk
+        s.q = qp
+        R2, nd2, rm2 = s.ufvm_solver(out=Rp)
+        assert R2 is Rp
+        np.testing.assert_array_equal(Rp, R)
+        assert (nd2, rm2) == (nd, rm)
+        np.testing.assert_array_equal(s.q, qk)
+    del qp, Rp   # frees the page-locked memory

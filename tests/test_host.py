@@ -157,6 +157,9 @@ def test_c_abi_error_codes(tmp_path):
     for steps in (-2, 3, 258):
         assert L.ebb_batch_create(0, C.c_int64(1 << 30), steps, 0, C.byref(vp)) == 1
     assert L.ebb_server_create(0, C.c_int64(1 << 30), 0, 100, 8, 0, C.byref(vp)) == 1
+    assert L.ebb_host_alloc(0, C.byref(vp)) == 1 and not vp.value   # nothing to allocate
+    assert L.ebb_host_alloc(16, None) == 1
+    L.ebb_host_free(None)                                           # a no-op
 
     def create(nodes, eoff, enodes, bnodes):
         nodes = np.ascontiguousarray(nodes, np.float64)
