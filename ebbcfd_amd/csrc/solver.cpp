@@ -1133,6 +1133,11 @@ int ebb_host_alloc(size_t bytes, void** out)
         if (!out) return EBB_ERR_ARG;
         *out = nullptr;
         if (bytes == 0) return EBB_ERR_ARG;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+            (void)hipGetLastError();
+            return EBB_ERR_DEVICE;   // page-locked memory is the HIP runtime's
+        }
         void* p = nullptr;
         const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
         if (e != hipSuccess || !p) {

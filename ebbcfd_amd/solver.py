@@ -264,13 +264,15 @@ class _HostBlock:
     """Page-locked host memory from ebb_host_alloc, freed when the last array viewing it goes."""
 
     def __init__(self, nbytes):
+        self.p = None
+        self._free = lib().ebb_host_free   # bound now: module globals may be gone when this is freed
         p = C.c_void_p()
         _check(lib().ebb_host_alloc(C.c_size_t(nbytes), C.byref(p)), "host alloc")
         self.p = p.value
 
     def __del__(self):
         if self.p:
-            lib().ebb_host_free(C.c_void_p(self.p))
+            self._free(C.c_void_p(self.p))
             self.p = None
 
 

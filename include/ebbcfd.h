@@ -124,7 +124,8 @@ int ebb_solver_create(const ebb_mesh* m, const ebb_config* c, int device, ebb_so
  * ebb_state_download and ebb_residual: copies from and to it run at the PCIe DMA rate
  * instead of through the runtime's staging of pageable memory.  No counterpart in the
  * reference (a D caller that keeps its integrator allocates its q and R arrays with
- * these instead of the GC).  *out is NULL and EBB_ERR_ALLOC when it cannot be had. */
+ * these instead of the GC).  *out is NULL on failure: EBB_ERR_DEVICE without a HIP
+ * device, EBB_ERR_ALLOC when the memory cannot be had. */
 int ebb_host_alloc(size_t bytes, void** out);
 void ebb_host_free(void* p);
 
