@@ -44,8 +44,12 @@ FACE_BYTES_PER_CELL = 144.0     # state 32 + gradient 64 + limiter 32 + centroid
 # the kernel changes; reported only for the configurations it was measured on
 # (kernel, workload, stage path, tile cells, tile order, cell order, limiter).
 MEASURED_TRAFFIC = {
-    ("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference", "venkatakrishnan"): 1.3695e9,
-    ("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference", "scalar"): 1.3674e9,
+    ("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference", "forward",
+     "venkatakrishnan"): 1.3695e9,
+    ("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "reference", "forward",
+     "scalar"): 1.3674e9,
+    ("k_face_tile", "cylinder_ns_ogrid_2048x1024x2", "faceTiles", -1, "blocks", "tiles", "tilesBackward",
+     "venkatakrishnan"): 1.3687e9,
 }
 
 
@@ -486,7 +490,10 @@ def main():
     ap.add_argument("--tile-cells", type=int, default=-1, help="cells per tile of the tiled paths (-1: automatic)")
     ap.add_argument("--tile-order", default="blocks", choices=("blocks", "grow", "morton"),
                     help="how tiles are cut: Morton order of 16-cell blocks, greedily grown regions, Morton order")
-    ap.add_argument("--cell-order", default="reference", choices=("reference", "tiles"),
+    ap.add_argument("--kernel-order", default="tilesBackward",
+                    choices=("forward", "tilesBackward", "gradientBackward", "bothBackward"),
+                    help="which kernels walk their tiles / cell blocks last to first")
+    ap.add_argument("--cell-order", default="tiles", choices=("reference", "tiles"),
                     help="device numbering of the cells: the mesh's, or tile by tile")
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision form of the workload (0: skip)")
     ap.add_argument("--c2-steps", type=int, default=5000,
@@ -546,7 +553,8 @@ def main():
     raw = meshgen.cylinder_ogrid(args.n_theta, args.n_r, seed=1, **kw)
     cfg = cylinder_config()
     cfg["extensions"] = {"stagePath": args.stage_path, "tileCells": args.tile_cells, "tileOrder": args.tile_order,
-                         "cellOrder": args.cell_order, "limiter": args.limiter, "venkatK": args.venkat_k}
+                         "cellOrder": args.cell_order, "limiter": args.limiter, "venkatK": args.venkat_k,
+                         "kernelOrder": args.kernel_order}
     mesh = build_mesh(raw)
     N = mesh.n_interior
     log("mesh built: %d cells, %d edges" % (N, mesh.n_edges))
@@ -586,9 +594,18 @@ def main():
     if args.stretch is not None:
         workload += "_stretch%g" % args.stretch
     traffic = MEASURED_TRAFFIC.get((kernel, workload, args.stage_path, args.tile_cells, args.tile_order,
-                                    args.cell_order, args.limiter))
+                                    args.cell_order, args.kernel_order, args.limiter))
 
-    dropin = dropin_leg(s, N, args.dropin_stages, stage_ms) if args.dropin_stages > 0 else None
+    dropin = None
+    if args.dropin_stages > 0:
+        # the drop-in keeps the reference's cell order on the device, so q and R cross PCIe as
+        # they are (a renumbering solver would permute them on the host)
+        sd = s
+        if args.cell_order != "reference":
+            ext = dict(cfg["extensions"], cellOrder="reference", kernelOrder="forward")
+            sd = Solver(mesh, dict(cfg, extensions=ext))
+        dropin = dropin_leg(sd, N, args.dropin_stages, stage_ms)
+        del sd
     mixed = None
     if args.mixed:
         mixed = mixed_leg(mesh, cfg, args, N, value)
@@ -627,7 +644,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "cells": N,
                    "stage_path": args.stage_path, "tile_cells": args.tile_cells, "tile_order": args.tile_order,
-                   "cell_order": args.cell_order,
+                   "cell_order": args.cell_order, "kernel_order": args.kernel_order,
                    "edges": F, "flux": "rusanovFlux",
                    "limiter": ("venkatakrishnan (K=%g)" % args.venkat_k) if args.limiter == "venkatakrishnan"
                    else "scalar (solve.d:360-403)",

@@ -244,6 +244,13 @@ bool parseConfig(const std::string& json, Config& c, std::string& err)
             if (e->i != -1 && (e->i < 1 || e->i > 1024)) { err = "tileCells must be -1 (automatic) or in 1..1024"; return false; }
             c.tileCells = e->i;
         }
+        if ((e = v->get("kernelOrder")) && e->kind == JVal::Str) {
+            if (e->s == "forward") c.kernelOrder = 0;
+            else if (e->s == "tilesBackward") c.kernelOrder = 1;
+            else if (e->s == "gradientBackward") c.kernelOrder = 2;
+            else if (e->s == "bothBackward") c.kernelOrder = 3;
+            else { err = "unknown kernelOrder " + e->s; return false; }
+        }
         if ((e = v->get("tileOrder")) && e->kind == JVal::Str) {
             if (e->s == "grow") c.tileOrder = 0;
             else if (e->s == "morton") c.tileOrder = 1;

@@ -141,6 +141,8 @@ struct StageArgs {
     int recordErrors;
     uint32_t stageId;      // global stage counter for error ordering
     uint32_t stepId;
+    int order;             // bit 0: the face/update tile kernel walks each XCD's slice of tiles
+                           // last to first; bit 1: the gradient kernel its slice of cell blocks
     ForceArgs force;       // gradient kernels: the pending boundary force, if any
 };
 

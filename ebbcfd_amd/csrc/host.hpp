@@ -89,6 +89,7 @@ struct Config {
                                       // every CU
     int64_t spreadOver = 0;           // cells of other solves whose tiles share this one's launches
                                       // (batched members): counted with the mesh's own when spreading
+    int kernelOrder = 0;              // StageArgs::order: which kernels walk their work backwards
     int tileOrder = 2;                // 0: greedily grown regions, 1: Morton order of centroids,
                                       // 2: Morton order of aligned 16-cell blocks
     bool renumber = false;            // tiled paths: number device cells tile by tile

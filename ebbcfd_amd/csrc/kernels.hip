@@ -42,6 +42,15 @@ DEV uint32_t xcdBlock(uint32_t b, uint32_t nb)
     return x * per + (x < rem ? x : rem) + k;
 }
 
+// The same slices, each walked last to first (the k-th workgroup of XCD x takes the k-th
+// item from its slice's end; k < the slice's length for every b < nb)
+DEV uint32_t xcdBlockBack(uint32_t b, uint32_t nb)
+{
+    const uint32_t nx = 8, per = nb / nx, rem = nb % nx, x = b % nx, k = b / nx;
+    const uint32_t len = per + (x < rem ? 1u : 0u);
+    return x * per + (x < rem ? x : rem) + (len - 1 - k);
+}
+
 // The face, reconstruction and limiter arithmetic below is written once for the
 // real type R it computes in: double on the FP64 path, float on the mixed path
 // (extensions.precision = "mixed").  Every literal is converted to R, so the double
@@ -760,7 +769,8 @@ __global__ __launch_bounds__(BLK) void k_gradient(DevMesh m, PhysParams p, Stage
         pendingForce<BLK, FORCE_LDS>(a.force, a.sc, reinterpret_cast<double2*>(&L.sg[0][0]));
         return;
     }
-    gradientBody<S, MODE, MIX>(m, p, a, xcdBlock(blockIdx.x - fw, gridDim.x - fw), L);
+    const uint32_t b = blockIdx.x - fw, nb = gridDim.x - fw;
+    gradientBody<S, MODE, MIX>(m, p, a, (a.order & 2) ? xcdBlockBack(b, nb) : xcdBlock(b, nb), L);
 }
 
 // ---------------------------------------------------------------- faces
@@ -1726,7 +1736,8 @@ template <int S, int FLUX, int VISC, int ORDER, int LP, int MIX>
 __global__ __launch_bounds__(BLK) void k_face_tile(DevMesh m, PhysParams p, StageArgs a, FaceTileArgs t)
 {
     extern __shared__ double fsd[];
-    faceTileBody<S, FLUX, VISC, ORDER, LP, MIX>(m, p, a, t, xcdBlock(blockIdx.x, gridDim.x), blockIdx.x, fsd);
+    const uint32_t tile = (a.order & 1) ? xcdBlockBack(blockIdx.x, gridDim.x) : xcdBlock(blockIdx.x, gridDim.x);
+    faceTileBody<S, FLUX, VISC, ORDER, LP, MIX>(m, p, a, t, tile, blockIdx.x, fsd);
 }
 
 template <int S, int FLUX, int VISC>

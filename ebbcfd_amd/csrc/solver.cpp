@@ -150,6 +150,7 @@ static StageArgs baseStageArgs(ebb_solver* s, int kind, double4* qin, const doub
     a.celldt = s->d_celldt.p; a.partial = s->d_partial.p; a.sc = s->d_sc;
     a.kind = kind; a.limit = limit; a.dtUpdate = dtUpdate; a.first = first; a.last = last;
     a.recordErrors = 1; a.stageId = stageId; a.stepId = stepId;
+    a.order = s->cfg.kernelOrder;
     mixArgs(s, a, qin, qout);
     return a;
 }
@@ -1040,6 +1041,7 @@ int ebb_config_get(const ebb_config* c, const char* key, double* value)
         else if (std::string(key) == "precision") *value = c->c.precision;
         else if (std::string(key) == "limiter") *value = c->c.limiter;
         else if (std::string(key) == "stagePath") *value = c->c.stagePath;
+        else if (std::string(key) == "kernelOrder") *value = c->c.kernelOrder;
         else if (std::string(key) == "integrator") *value = c->c.integratorKind();
         else if (std::string(key) == "flux") *value = c->c.fluxKind();
         else return EBB_ERR_ARG;
@@ -1056,10 +1058,17 @@ int ebb_config_set(ebb_config* c, const char* key, double value)
         if (dp) *dp = value;
         else if (ip) *ip = (long long)value;
         else if (bp) *bp = value != 0.0;
-        else if (std::string(key) == "precision") c->c.precision = int(value);
-        else if (std::string(key) == "limiter") c->c.limiter = int(value);
-        else if (std::string(key) == "stagePath") c->c.stagePath = int(value);
-        else return EBB_ERR_ARG;
+        else {
+            // enumerated keys take one of their values' numbers
+            const std::string k(key);
+            const int v = int(value);
+            const bool whole = double(v) == value;
+            if (k == "precision" && whole && (v == 0 || v == 1)) c->c.precision = v;
+            else if (k == "limiter" && whole && (v == 0 || v == 1)) c->c.limiter = v;
+            else if (k == "stagePath" && whole && v >= 0 && v <= 3) c->c.stagePath = v;
+            else if (k == "kernelOrder" && whole && v >= 0 && v <= 3) c->c.kernelOrder = v;
+            else return EBB_ERR_ARG;
+        }
         return EBB_OK;
     });
 }
@@ -1603,6 +1612,7 @@ int ebb_time_steps(ebb_solver* s, int steps, double out[6])
                 a.kind = x.kind; a.limit = x.limit; a.dtUpdate = x.dtUpdate; a.first = x.first; a.last = x.last;
                 a.rmsBlocks = (x.last && s->rmsFused) ? s->d_rmsBlocks.p : nullptr;
                 a.checkActive = 0; a.recordErrors = 1; a.stageId = s->stageCounter++; a.stepId = uint32_t(s->stepsTotal);
+                a.order = s->cfg.kernelOrder;
                 mixArgs(s, a, x.in, x.out);
                 s->stageStep.push_back(s->stepsTotal);
                 if (s->useTiles) {

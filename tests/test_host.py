@@ -233,13 +233,23 @@ def test_config_extension_keys():
     """The extension keys this build adds: parsed into the config, unknown values refused."""
     c = dict(REF_CFG)
     c["extensions"] = {"precision": "mixed", "limiter": "venkatakrishnan", "venkatK": 2.5, "rmsDrop": 1e-8,
-                       "stagePath": "gradTiles", "maxSteps": 20000}
+                       "stagePath": "gradTiles", "maxSteps": 20000, "kernelOrder": "tilesBackward"}
     cfg = load_config(json.dumps(c))
     assert cfg["precision"] == 1 and cfg["limiter"] == 1 and cfg["venkatK"] == 2.5 and cfg["rmsDrop"] == 1e-8
-    assert cfg["stagePath"] == 3 and cfg["maxSteps"] == 20000
+    assert cfg["stagePath"] == 3 and cfg["maxSteps"] == 20000 and cfg["kernelOrder"] == 1
+    for name, number in (("forward", 0), ("tilesBackward", 1), ("gradientBackward", 2), ("bothBackward", 3)):
+        c["extensions"] = {"kernelOrder": name}
+        assert load_config(json.dumps(c))["kernelOrder"] == number
     plain = load_config(json.dumps(REF_CFG))
     assert plain["precision"] == 0 and plain["limiter"] == 0 and plain["rmsDrop"] == 0 and plain["stagePath"] == 1
-    for key, value in (("precision", "fp16"), ("limiter", "minmod"), ("stagePath", "tiles")):
+    assert plain["kernelOrder"] == 0
+    # enumerated keys set by number take only their values' numbers
+    plain["kernelOrder"] = 3
+    assert plain["kernelOrder"] == 3
+    for key, value in (("kernelOrder", 4), ("stagePath", 4), ("precision", 2), ("limiter", -1), ("stagePath", 1.5)):
+        with pytest.raises(Exception):
+            plain[key] = value
+    for key, value in (("precision", "fp16"), ("limiter", "minmod"), ("stagePath", "tiles"), ("kernelOrder", "reverse")):
         bad = dict(REF_CFG)
         bad["extensions"] = {key: value}
         with pytest.raises(Exception):
