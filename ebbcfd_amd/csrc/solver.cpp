@@ -325,15 +325,6 @@ int clearErrors(ebb_solver* s)
     return EBB_OK;
 }
 
-// Pick the tile size (tileCells, or automatically the largest multiple of 16 up
-// to 128 whose two blob buffers and working set fit 64 KB of LDS) and upload
-// the tiles.
-// Tiled paths: choose the tile size (tileCells, or automatically: 768 cells per
-// face tile; for the fused kernel the largest multiple of 16 up to 128 whose two
-// blob buffers and working set fit 64 KB of LDS) and cut the tiles on the
-// reference numbering; with renumbering the interior cells are then numbered tile
-// by tile.  `lay` receives the device-order layout, tile t is the device cells
-// tileCellsOrder[start[t] .. start[t+1]).
 // Down to 32 cells, in whole 16-cell blocks: a tile's workgroup takes about the same time
 // for any tile size up to one round of lanes, so smaller tiles only buy more workgroups.
 uint32_t spreadTileCells(uint64_t cells, int cus)
@@ -342,6 +333,12 @@ uint32_t spreadTileCells(uint64_t cells, int cus)
     return uint32_t(std::max<uint64_t>(32, std::min<uint64_t>(768, share)));
 }
 
+// Tiled paths: choose the tile size (tileCells, or automatically: 768 cells per
+// face tile; for the fused kernel the largest multiple of 16 up to 128 whose two
+// blob buffers and working set fit 64 KB of LDS) and cut the tiles on the
+// reference numbering; with renumbering the interior cells are then numbered tile
+// by tile.  `lay` receives the device-order layout, tile t is the device cells
+// tileCellsOrder[start[t] .. start[t+1]).
 int planTiles(ebb_solver* s, Layout& ref, Layout& lay, std::vector<uint32_t>& start,
               std::vector<uint32_t>& tileCellsOrder)
 {
@@ -1059,15 +1056,17 @@ int ebb_config_set(ebb_config* c, const char* key, double value)
         else if (ip) *ip = (long long)value;
         else if (bp) *bp = value != 0.0;
         else {
-            // enumerated keys take one of their values' numbers
+            // enumerated keys take one of their values' numbers (checked before converting)
             const std::string k(key);
-            const int v = int(value);
-            const bool whole = double(v) == value;
-            if (k == "precision" && whole && (v == 0 || v == 1)) c->c.precision = v;
-            else if (k == "limiter" && whole && (v == 0 || v == 1)) c->c.limiter = v;
-            else if (k == "stagePath" && whole && v >= 0 && v <= 3) c->c.stagePath = v;
-            else if (k == "kernelOrder" && whole && v >= 0 && v <= 3) c->c.kernelOrder = v;
-            else return EBB_ERR_ARG;
+            auto take = [&](int hi, int& dst) {
+                if (!(value >= 0 && value <= hi && value == std::floor(value))) return false;
+                dst = int(value);
+                return true;
+            };
+            const bool ok = (k == "precision" && take(1, c->c.precision)) || (k == "limiter" && take(1, c->c.limiter)) ||
+                            (k == "stagePath" && take(3, c->c.stagePath)) ||
+                            (k == "kernelOrder" && take(3, c->c.kernelOrder));
+            if (!ok) return EBB_ERR_ARG;
         }
         return EBB_OK;
     });
