@@ -363,6 +363,67 @@ def copy_calibration():
     log("copied %d bytes 5 times" % (x.numel() * 8))
 
 
+def shedding_frequency(hist, t_from):
+    """Strouhal number (D = U = 1) and peak-to-peak lift of a lift history after t_from:
+    the mean period between upward zero crossings of (lift - mean), each crossing time
+    interpolated linearly between its two samples."""
+    t, lift = hist[:, 0], hist[:, 8]
+    m = t > t_from
+    tt, ll = t[m], lift[m] - lift[m].mean()
+    k = np.where((ll[:-1] < 0) & (ll[1:] >= 0))[0]
+    if len(k) < 3:
+        return None, None, float(ll.max() - ll.min()) if len(ll) else None
+    tc = tt[k] - ll[k] * (tt[k + 1] - tt[k]) / (ll[k + 1] - ll[k])
+    per = np.diff(tc)
+    return float(1.0 / per.mean()), per, float(ll.max() - ll.min())
+
+
+def strouhal_bench(args):
+    """Vortex shedding, FP64 against mixed precision: the workload's physics (Re = 100,
+    M = 0.2, Rusanov, Venkatakrishnan K = 1, TVD-RK2) on a 256 x 128 O-grid with radial
+    stretch 1.04 (wall spacing 7.8e-3), where the reference's CFL time step is stable at
+    CFL 0.5 and `--strouhal-steps` steps reach t ~ 100, some ten shedding periods.  CFL is
+    held at 0.5 (no cflAdjust), so both runs take the same time steps up to rounding."""
+    from ebbcfd_amd import Solver, build_mesh, meshgen
+    raw = meshgen.cylinder_ogrid(256, 128, seed=1, stretch=1.04)
+    mesh = build_mesh(raw)
+    out = {}
+    hists = {}
+    for prec in ("fp64", "mixed"):
+        cfg = cylinder_config(CFL=0.5)
+        cfg["cflAdjust"] = False
+        cfg["extensions"] = {"limiter": "venkatakrishnan", "venkatK": 1.0, "precision": prec}
+        s = Solver(mesh, cfg)
+        parts, done, t0 = [], 0, time.perf_counter()
+        while done < args.strouhal_steps:
+            h, t, dt = s.run(min(50000, args.strouhal_steps - done))
+            parts.append(h)
+            done += len(h)
+            log("%s: %d steps, t = %.2f, lift %.5f, drag %.5f (%.1f s)" % (prec, done, t, h[-1, 8], h[-1, 7],
+                                                                          time.perf_counter() - t0))
+        hists[prec] = np.concatenate(parts)
+        St, per, amp = shedding_frequency(hists[prec], args.strouhal_from)
+        out[prec] = {"St": St, "lift_peak_to_peak": amp, "t_end": float(t), "seconds": round(time.perf_counter() - t0, 1),
+                     "periods": None if per is None else [round(float(x), 6) for x in per]}
+        del s
+    a, b = hists["fp64"], hists["mixed"]
+    n = min(len(a), len(b))
+    dl = np.abs(a[:n, 8] - b[:n, 8]).max() / np.abs(a[:n, 8]).max()
+    dd = np.abs(a[:n, 7] - b[:n, 7]).max() / np.abs(a[:n, 7]).max()
+    dt_rel = np.abs(a[:n, 0] - b[:n, 0]).max() / a[n - 1, 0]
+    err = None
+    if out["fp64"]["St"] and out["mixed"]["St"]:
+        err = abs(out["mixed"]["St"] - out["fp64"]["St"]) / out["fp64"]["St"]
+    log("St: FP64 %s, mixed %s, relative error %s; lift peak-to-peak %s vs %s"
+        % (out["fp64"]["St"], out["mixed"]["St"], err, out["fp64"]["lift_peak_to_peak"], out["mixed"]["lift_peak_to_peak"]))
+    print(json.dumps({"metric": "Strouhal number, mixed vs FP64", "steps": args.strouhal_steps,
+                      "config": {"workload": "cylinder_ns_ogrid_256x128x2_stretch1.04", "Re": 100, "M": 0.2, "CFL": 0.5,
+                                 "cflAdjust": False, "limiter": "venkatakrishnan (K=1)", "St_from_t": args.strouhal_from},
+                      "fp64": out["fp64"], "mixed": out["mixed"], "St_rel_error": err,
+                      "lift_max_diff_rel": float(dl), "drag_max_diff_rel": float(dd), "time_max_diff_rel": float(dt_rel)}),
+          flush=True)
+
+
 def ramp_probe(args):
     """Short timed windows after GPU idle: the workload's solver, `--warmup` steps, then for
     each idle gap a `--steps`-step run after the gap and another right after it.  The
@@ -526,11 +587,18 @@ def main():
     ap.add_argument("--ingest", action="store_true",
                     help="time the mesh ingest (gmsh read + geometry precompute) of the workload's mesh instead")
     ap.add_argument("--ingest-repeats", type=int, default=3)
+    ap.add_argument("--strouhal", action="store_true",
+                    help="only run the shedding comparison (FP64 and mixed precision on a 256x128 cylinder) and exit")
+    ap.add_argument("--strouhal-steps", type=int, default=400000)
+    ap.add_argument("--strouhal-from", type=float, default=40.0, help="shedding frequency measured after this t")
     ap.add_argument("--ramp-probe", action="store_true",
                     help="only time --steps-step windows after GPU idle gaps of 0-3 s and exit")
     args = ap.parse_args()
     if args.ramp_probe:
         ramp_probe(args)
+        return
+    if args.strouhal:
+        strouhal_bench(args)
         return
     if args.ingest:
         ingest_bench(args)
