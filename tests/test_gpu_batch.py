@@ -159,16 +159,18 @@ def test_batch_members_join_and_leave():
         np.testing.assert_array_equal(b.q(sid), s.q)
 
 
-def test_batch_member_retiled_as_batch_grows_and_shrinks():
+@pytest.mark.parametrize("path,precision", [("faceTiles", "fp64"), ("gradTiles", "fp64"), ("gradTiles", "mixed")])
+def test_batch_member_retiled_as_batch_grows_and_shrinks(path, precision):
     """Automatic tiles follow the cells running together: a small solve admitted alone gets
     32-cell tiles (as a solo run); when a ~60k-cell solve joins, the cells sharing the
-    launches grow more than twofold and the small solve is cut into larger tiles; when it
-    leaves, back.  Both must still equal their solo runs bit for bit."""
+    launches grow more than twofold and the small solve is cut into larger tiles (on the
+    gradient-tile path with new ring and border tables); when it leaves, back.  Both must
+    still equal their solo runs bit for bit."""
     def spread(cells, cus):   # cells per automatic face tile (two tiles per CU, 16-cell blocks, 32-768)
         return max(32, min(768, cells // (2 * cus) // 16 * 16))
 
-    raw_s, cfg_s = _channel(40, 12, 0.05, 5, 0.5, 2.0)
-    raw_b, cfg_b = _channel(300, 100, 0.06, 6, 0.7, -1.0)
+    raw_s, cfg_s = _channel(40, 12, 0.05, 5, 0.5, 2.0, extensions={"stagePath": path, "precision": precision})
+    raw_b, cfg_b = _channel(300, 100, 0.06, 6, 0.7, -1.0, extensions={"stagePath": path, "precision": precision})
     ms, mb = build_mesh(raw_s), build_mesh(raw_b)
     # the join and the leave cross the twofold re-tiling threshold on any GPU of 32 to 304
     # CUs (MI355X: 256, where the small solve goes from 32-cell tiles to 112 and back)
