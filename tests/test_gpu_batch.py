@@ -255,8 +255,15 @@ def test_batch_admission_by_footprint():
     assert b.info()["solves"] == 2
 
 
-def test_batch_tuner_keeps_states():
-    cases = _cases()[:4]
+@pytest.mark.parametrize("path", ["faceTiles", "gradTiles"])
+def test_batch_tuner_keeps_states(path):
+    """The autotuner re-tiles every member for each trial (and restores every state); the
+    members then still equal their solo runs."""
+    cases = []
+    for raw, cfg in _cases()[:4]:
+        c = copy.deepcopy(cfg)
+        c["extensions"] = dict(c.get("extensions", {}), stagePath=path)
+        cases.append((raw, c))
     b = Batch(graph_steps=8, keep_history=True)
     ids = [b.add(build_mesh(raw), cfg) for raw, cfg in cases]
     b.run(16)
