@@ -49,6 +49,31 @@ def test_served_results_equal_solo_runs():
     sv.close()
 
 
+def test_served_mix_of_kinds_equals_solo_runs():
+    """Requests of different kinds served together -- FP64 and mixed precision, face tiles
+    and gradient tiles, the scalar and the Venkatakrishnan limiter -- form separate launch
+    groups in one batch; each result still equals its request run alone."""
+    kinds = [{}, {"precision": "mixed"}, {"stagePath": "gradTiles"},
+             {"stagePath": "gradTiles", "precision": "mixed", "limiter": "venkatakrishnan", "venkatK": 5.0},
+             {"limiter": "venkatakrishnan", "venkatK": 1.0}]
+    reqs = []
+    for k, ext in enumerate(kinds):
+        raw, cfg = _request(400 + k, [3000, 7000, 1500, 20000, 5000][k], rms_drop=1e-3, max_steps=600)
+        cfg["extensions"] = dict(cfg["extensions"], **ext)
+        reqs.append((raw, cfg))
+    meshes = [build_mesh(raw) for raw, _ in reqs]
+    sv = Server(max_solves=8, steps_per_round=40, graph_steps=8, keep_results=True)
+    tickets = [sv.submit(m, c) for m, (_, c) in zip(meshes, reqs)]
+    assert sv.wait(len(reqs), 300.0)
+    for k, t in enumerate(tickets):
+        p = sv.poll(t)
+        s = Solver(meshes[k], reqs[k][1])
+        h, _, _ = s.run()
+        assert p["state"] == 2 and p["steps"] == len(h), (k, p, len(h))
+        np.testing.assert_array_equal(sv.result(t), s.q)
+    sv.close()
+
+
 def test_admission_waits_for_memory():
     reqs = [_request(200 + k, 8000, rms_drop=1e-3, max_steps=400) for k in range(4)]
     meshes = [build_mesh(raw) for raw, _ in reqs]
