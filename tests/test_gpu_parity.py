@@ -102,6 +102,9 @@ STEP_CASES = {
     "venkat_rk4_roe": lambda: channel_case(integrator="RK4", extensions={"limiter": "venkatakrishnan", "venkatK": 0.5}),
     "venkat_cyl_ns_rk2tvd": lambda: cylinder_case(extensions={"limiter": "venkatakrishnan", "venkatK": 1.0}),
     "venkat_quads_mixed": lambda: (mixed_mesh(seed=3), channel_case(integrator="RK2", extensions={"limiter": "venkatakrishnan"})[1]),
+    # as the serving requests run: Roe, RK4, local time steps, K = 5
+    "venkat_k5_local_dt_rk4": lambda: channel_case(nx=40, ny=12, M=1.2, integrator="RK4", localTimestep=True,
+                                                   extensions={"limiter": "venkatakrishnan", "venkatK": 5.0}),
 }
 
 
