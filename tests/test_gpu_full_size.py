@@ -36,11 +36,13 @@ def workload():
     yield raw, cfg, hist, t, dt, om.q[: om.n_interior].copy()
 
 
-@pytest.mark.parametrize("path", ["faceTiles", "split", "fused", "gradTiles"])
-def test_full_size_steps_match_oracle(workload, path):
+@pytest.mark.parametrize("path,cells,korder", [("faceTiles", "reference", "forward"), ("split", "reference", "forward"),
+                                               ("fused", "reference", "forward"), ("gradTiles", "reference", "forward"),
+                                               ("faceTiles", "tiles", "tilesBackward")])
+def test_full_size_steps_match_oracle(workload, path, cells, korder):
     raw, cfg, hist_o, t_o, dt_o, q_o = workload
     c = copy.deepcopy(cfg)
-    c["extensions"] = {"stagePath": path}
+    c["extensions"] = {"stagePath": path, "cellOrder": cells, "kernelOrder": korder}
     s = Solver(build_mesh(raw), c)
     hist_g, t_g, dt_g = s.run(STEPS)
     np.testing.assert_array_equal(hist_g, hist_o)
