@@ -385,6 +385,8 @@ def strouhal_bench(args):
     CFL 0.5 and `--strouhal-steps` steps reach t ~ 100, some ten shedding periods.  CFL is
     held at 0.5 (no cflAdjust), so both runs take the same time steps up to rounding."""
     from ebbcfd_amd import Solver, build_mesh, meshgen
+    if args.strouhal_steps <= 0:
+        sys.exit("--strouhal-steps must be positive")
     raw = meshgen.cylinder_ogrid(256, 128, seed=1, stretch=1.04)
     mesh = build_mesh(raw)
     out = {}
